@@ -1,0 +1,234 @@
+"""Seeded synthetic scenes in the reference's ``.tri`` format.
+
+The ``.tri`` format is the one ``ReadTriangle`` parses (rad.cpp:47-111): per
+triangle the word ``Triangle``, three ``Frgb`` ints (reflectance, rho =
+Frgb * 0.5 / 255, shade.cpp:9 and shade.cpp:246), three ``Brgb`` ints (a
+triangle is a light iff one channel equals 255, rad.h:47-49) and three vertices.
+
+Every coordinate is computed with IEEE +, -, *, / in float64 and rounded once
+to float32 (no libm transcendentals: a random yaw uses the rational
+parametrisation cos = (1-u^2)/(1+u^2), sin = 2u/(1+u^2)), so a seed gives the
+same bits on every host.  Grid vertices of one face are computed once and
+shared, so coplanar neighbours match exactly, as the neighbour search in
+InitRadiosity requires (rad.cpp:303-322).
+
+Scenes
+  room_scene(seed)  closed 10 x 8 x 6 room, 6 gridded walls, 2 axis-aligned
+                    boxes (sides U[1,3]) with one quad per face (so some
+                    triangles have area >= 1 and pre-partition has work),
+                    one emissive ceiling quad (Brgb 255 255 255), per-face
+                    reflectance U[0.2,0.8]; about 2k triangles.
+  hall_scene(seed)  100 x 100 x 30 hall, 400 yawed boxes (sides U[1,8]),
+                    8 emissive ceiling quads with emission U[0.5,2] RGB,
+                    reflectance U[0.1,0.9]; about 200k triangles.
+"""
+from __future__ import annotations
+
+import dataclasses
+import gzip
+import io
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014) -- a tiny, portable PRNG."""
+
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+
+@dataclasses.dataclass
+class Scene:
+    p: np.ndarray       # float32 [N, 3, 3] vertices
+    frgb: np.ndarray    # int32 [N, 3]
+    brgb: np.ndarray    # int32 [N, 3]
+
+    @property
+    def n(self) -> int:
+        return int(self.p.shape[0])
+
+
+class _Builder:
+    def __init__(self):
+        self.p, self.f, self.b = [], [], []
+
+    def grid(self, origin, du, dv, nu, nv, frgb, brgb_of=None, flip=False):
+        """Triangulate the parallelogram origin + s*du + t*dv, s,t in [0,1],
+        into nu x nv cells of two right triangles.  The normal
+        (p1-p0) x (p2-p0) is du x dv, or its opposite with ``flip``."""
+        o = np.asarray(origin, np.float64)
+        du = np.asarray(du, np.float64)
+        dv = np.asarray(dv, np.float64)
+        i = np.arange(nu + 1, dtype=np.float64)[:, None, None]
+        j = np.arange(nv + 1, dtype=np.float64)[None, :, None]
+        verts = (o + du * (i / nu) + dv * (j / nv)).astype(np.float32)
+        for a in range(nu):
+            for c in range(nv):
+                v00, v10 = verts[a, c], verts[a + 1, c]
+                v01, v11 = verts[a, c + 1], verts[a + 1, c + 1]
+                tris = ((v00, v10, v11), (v00, v11, v01))
+                br = brgb_of(a, c) if brgb_of else (0, 0, 0)
+                for t in tris:
+                    if flip:
+                        t = (t[0], t[2], t[1])
+                    self.p.append(np.stack(t))
+                    self.f.append(frgb)
+                    self.b.append(br)
+
+    def scene(self) -> Scene:
+        return Scene(np.stack(self.p).astype(np.float32),
+                     np.asarray(self.f, np.int32).reshape(-1, 3),
+                     np.asarray(self.b, np.int32).reshape(-1, 3))
+
+
+def _frgb(rng: SplitMix64, lo: float, hi: float):
+    # rho = Frgb * 0.5 / 255  =>  Frgb = round(510 * rho)
+    return tuple(int(510.0 * rng.uniform(lo, hi) + 0.5) for _ in range(3))
+
+
+def _cells(length: float, h: float) -> int:
+    return max(1, int(length / h + 0.5))
+
+
+def _room_walls(bld, rng, X, Y, Z, h, rho, light_rect=None, light_brgb=None):
+    """Six inward-facing walls of [0,X]x[0,Y]x[0,Z]; returns nothing.
+    light_rect: list of (i0, i1, j0, j1) cell ranges on the ceiling grid."""
+    nx, ny, nz = _cells(X, h), _cells(Y, h), _cells(Z, h)
+    # floor z=0, normal +z: du=x, dv=y
+    bld.grid((0, 0, 0), (X, 0, 0), (0, Y, 0), nx, ny, _frgb(rng, *rho))
+    # ceiling z=Z, normal -z
+    def ceil_b(a, c):
+        for k, (i0, i1, j0, j1) in enumerate(light_rect or ()):
+            if i0 <= a < i1 and j0 <= c < j1:
+                return light_brgb[k]
+        return (0, 0, 0)
+    bld.grid((0, 0, Z), (X, 0, 0), (0, Y, 0), nx, ny, _frgb(rng, *rho),
+             brgb_of=ceil_b, flip=True)
+    # x=0 wall normal +x: du=y, dv=z
+    bld.grid((0, 0, 0), (0, Y, 0), (0, 0, Z), ny, nz, _frgb(rng, *rho))
+    bld.grid((X, 0, 0), (0, Y, 0), (0, 0, Z), ny, nz, _frgb(rng, *rho), flip=True)
+    # y=0 wall normal +y: du=z, dv=x
+    bld.grid((0, 0, 0), (0, 0, Z), (X, 0, 0), nz, nx, _frgb(rng, *rho))
+    bld.grid((0, Y, 0), (0, 0, Z), (X, 0, 0), nz, nx, _frgb(rng, *rho), flip=True)
+    return nx, ny
+
+
+def _box(bld, rng, cx, cy, a, b, c, cs, sn, h, rho):
+    """A box standing on z=0, footprint a x b rotated by (cs, sn) about its
+    centre, height c; 4 side faces and a top, outward normals, faces gridded
+    with cell size about h (h=None: one cell per face)."""
+    corners = []
+    for dx, dy in ((-a / 2, -b / 2), (a / 2, -b / 2), (a / 2, b / 2), (-a / 2, b / 2)):
+        corners.append((cx + cs * dx - sn * dy, cy + sn * dx + cs * dy, 0.0))
+    for k in range(4):
+        A, B = corners[k], corners[(k + 1) % 4]
+        du = (B[0] - A[0], B[1] - A[1], 0.0)
+        side = a if k % 2 == 0 else b
+        nu = 1 if h is None else _cells(side, h)
+        nv = 1 if h is None else _cells(c, h)
+        # du x (0,0,c) points outward for counter-clockwise corners
+        bld.grid(A, du, (0, 0, c), nu, nv, _frgb(rng, *rho))
+    A = (corners[0][0], corners[0][1], c)
+    du = (corners[1][0] - corners[0][0], corners[1][1] - corners[0][1], 0.0)
+    dv = (corners[3][0] - corners[0][0], corners[3][1] - corners[0][1], 0.0)
+    nu = 1 if h is None else _cells(a, h)
+    nv = 1 if h is None else _cells(b, h)
+    bld.grid(A, du, dv, nu, nv, _frgb(rng, *rho))
+
+
+def room_scene(seed: int = 1) -> Scene:
+    """Closed 10 x 8 x 6 room with two boxes and one ceiling light quad."""
+    rng = SplitMix64(seed)
+    bld = _Builder()
+    h = 0.625
+    nx, ny = _cells(10, h), _cells(8, h)
+    i0, j0 = nx // 2 - 2, ny // 2 - 1
+    _room_walls(bld, rng, 10.0, 8.0, 6.0, h, (0.2, 0.8),
+                light_rect=[(i0, i0 + 4, j0, j0 + 3)], light_brgb=[(255, 255, 255)])
+    for _ in range(2):
+        a, b, c = rng.uniform(1, 3), rng.uniform(1, 3), rng.uniform(1, 3)
+        cx, cy = rng.uniform(1 + a / 2, 9 - a / 2), rng.uniform(1 + b / 2, 7 - b / 2)
+        _box(bld, rng, cx, cy, a, b, c, 1.0, 0.0, None, (0.2, 0.8))
+    return bld.scene()
+
+
+def hall_scene(seed: int = 2, n_boxes: int = 400, box_h: float = 0.65,
+               wall_h: float = 2.0) -> Scene:
+    """100 x 100 x 30 hall with ``n_boxes`` yawed boxes and 8 light quads."""
+    rng = SplitMix64(seed)
+    bld = _Builder()
+    X, Y, Z = 100.0, 100.0, 30.0
+    nx, ny = _cells(X, wall_h), _cells(Y, wall_h)
+    rects, brgbs = [], []
+    for k in range(8):
+        # 2 x 4 layout of 2x2-cell quads, away from the walls
+        gi, gj = k % 4, k // 4
+        i0 = int((gi + 0.5) * nx / 4) - 1
+        j0 = int((gj + 0.5) * ny / 2) - 1
+        rects.append((i0, i0 + 2, j0, j0 + 2))
+        e = [rng.uniform(0.5, 2.0) for _ in range(3)]
+        # a light needs one channel equal to 255 (rad.h:47-49): R = 255 and
+        # G, B keep the emission ratios; the absolute magnitude is lost
+        brgbs.append((255, int(255.0 * e[1] / e[0] + 0.5), int(255.0 * e[2] / e[0] + 0.5)))
+    _room_walls(bld, rng, X, Y, Z, wall_h, (0.1, 0.9), light_rect=rects, light_brgb=brgbs)
+    for _ in range(n_boxes):
+        a, b, c = rng.uniform(1, 8), rng.uniform(1, 8), rng.uniform(1, 8)
+        u = rng.uniform(-1.0, 1.0)
+        cs, sn = (1.0 - u * u) / (1.0 + u * u), 2.0 * u / (1.0 + u * u)
+        cx, cy = rng.uniform(8.0, X - 8.0), rng.uniform(8.0, Y - 8.0)
+        _box(bld, rng, cx, cy, a, b, c, cs, sn, box_h, (0.1, 0.9))
+    return bld.scene()
+
+
+def format_tri(scene: Scene) -> str:
+    """Text in the .tri layout; '%.9g' round-trips every float32 exactly
+    through fscanf("%f") (rad.cpp:66-73)."""
+    out = io.StringIO()
+    p = scene.p.astype(np.float64)
+    for k in range(scene.n):
+        f, b = scene.frgb[k], scene.brgb[k]
+        out.write("Triangle\n%d %d %d %d %d %d\n" % (f[0], f[1], f[2], b[0], b[1], b[2]))
+        for v in range(3):
+            out.write("%.9g %.9g %.9g\n" % (p[k, v, 0], p[k, v, 1], p[k, v, 2]))
+    return out.getvalue()
+
+
+def write_tri(path: str, scene: Scene) -> None:
+    data = format_tri(scene)
+    if path.endswith(".gz"):
+        with gzip.open(path, "wt") as fh:
+            fh.write(data)
+    else:
+        with open(path, "w") as fh:
+            fh.write(data)
+
+
+def read_tri(path: str) -> Scene:
+    """Parse a .tri file the way ReadTriangle does: a keyword, 6 ints, 9 floats
+    per triangle, whitespace-separated (rad.cpp:59-75)."""
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "rt") as fh:
+        toks = fh.read().split()
+    n = len(toks) // 16
+    p = np.empty((n, 3, 3), np.float32)
+    f = np.empty((n, 3), np.int32)
+    b = np.empty((n, 3), np.int32)
+    for k in range(n):
+        t = toks[16 * k: 16 * k + 16]
+        f[k] = [int(x) for x in t[1:4]]
+        b[k] = [int(x) for x in t[4:7]]
+        p[k] = np.asarray([float(x) for x in t[7:16]], np.float64).astype(np.float32).reshape(3, 3)
+    return Scene(p, f, b)

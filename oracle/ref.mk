@@ -1,0 +1,50 @@
+# Builds the unmodified reference (morris821028/hw-radiosity) from its sources
+# where they lie, into oracle/_ref/ only.  Used to pin the oracle; never shipped.
+#
+#   make -f oracle/ref.mk            (from the repository root)
+#
+# Flags are the reference Makefile's (Makefile:7) plus `-include vector`,
+# because src/rad.cpp:387 uses std::vector without including <vector>.
+# Two numeric variants are produced:
+#   REF-FMA    the shipped flags: GCC contracts a*b+c into FMA on this host
+#   REF-NOFMA  the same with -ffp-contract=off (source-order arithmetic)
+# refdump links the reference's own translation units with oracle/refdump.cpp
+# (a dump harness), renaming the reference's main() with -Dmain=radref_main.
+
+REF     ?= /root/reference
+HERE    := $(patsubst %/,%,$(dir $(lastword $(MAKEFILE_LIST))))
+OUT     := $(HERE)/_ref
+CXX     := g++
+BASE    := -O2 -fopenmp -std=c++11 -march=native -msse -msse2 -Wall -Wno-unused-result \
+           -Wno-misleading-indentation -Wno-unused-function -I$(REF)/include -include vector
+FMA_FL  := $(BASE)
+NOFMA_FL:= $(BASE) -ffp-contract=off
+SRCS    := $(REF)/src/shade.cpp $(REF)/src/raycast.cpp $(REF)/src/rad.cpp
+HDRS    := $(wildcard $(REF)/include/*.h)
+
+.PHONY: all clean
+all: $(OUT)/rad $(OUT)/rad_nofma $(OUT)/refdump $(OUT)/refdump_nofma
+
+$(OUT):
+	mkdir -p $(OUT)
+
+$(OUT)/rad: $(SRCS) $(HDRS) | $(OUT)
+	$(CXX) $(FMA_FL) $(SRCS) -o $@ -lm
+
+$(OUT)/rad_nofma: $(SRCS) $(HDRS) | $(OUT)
+	$(CXX) $(NOFMA_FL) $(SRCS) -o $@ -lm
+
+$(OUT)/%.fma.o: $(REF)/src/%.cpp $(HDRS) | $(OUT)
+	$(CXX) $(FMA_FL) -Dmain=radref_main -c $< -o $@
+
+$(OUT)/%.nofma.o: $(REF)/src/%.cpp $(HDRS) | $(OUT)
+	$(CXX) $(NOFMA_FL) -Dmain=radref_main -c $< -o $@
+
+$(OUT)/refdump: $(HERE)/refdump.cpp $(OUT)/shade.fma.o $(OUT)/raycast.fma.o $(OUT)/rad.fma.o
+	$(CXX) $(FMA_FL) $^ -o $@ -lm
+
+$(OUT)/refdump_nofma: $(HERE)/refdump.cpp $(OUT)/shade.nofma.o $(OUT)/raycast.nofma.o $(OUT)/rad.nofma.o
+	$(CXX) $(NOFMA_FL) $^ -o $@ -lm
+
+clean:
+	rm -rf $(OUT)
