@@ -1,0 +1,271 @@
+// C ABI of the engine (include/radiosity_gpu.h).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/radiosity_gpu.h"
+#include "device.h"
+#include "fin.h"
+#include "octree.h"
+#include "scene.h"
+
+using namespace rgpu;
+
+struct rad_ctx {
+	HostScene scene;
+	FlatOctree tree;
+	std::unique_ptr<DeviceEngine> dev;
+	FinWriter writer;
+	std::string err;
+	bool loaded = false, prepared = false, uploaded = false, device_newer = false;
+	double ms_setup = 0, ms_upload = 0;
+};
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t0)
+{
+	return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+rad_status fail(rad_ctx *c, rad_status s, const std::string &msg)
+{
+	c->err = msg;
+	return s;
+}
+
+rad_status after_load(rad_ctx *c)
+{
+	const auto t0 = Clock::now();
+	HostScene &s = c->scene;
+	c->tree = build_octree(s.pool.data(), s.n_orig, s.G0, s.G1, s.o.fma != 0, s.o.jobs > 0 ? std::max(s.o.jobs, 8) : 0);
+	s.log("[INFO] 3D-DDA #TreeNode = %d, #Trilist = %d\n", c->tree.n_nodes, c->tree.n_list);
+	c->ms_setup += ms_since(t0);
+	c->loaded = true;
+	c->prepared = c->uploaded = c->device_newer = false;
+	return RAD_OK;
+}
+
+rad_status sync_host(rad_ctx *c)
+{
+	if (c->device_newer) {
+		if (c->dev->download(c->scene, c->err)) return RAD_EDEVICE;
+		c->device_newer = false;
+	}
+	return RAD_OK;
+}
+
+rad_status write_snapshot(rad_ctx *c, const std::string &path, bool final_file)
+{
+	rad_status st = sync_host(c);
+	if (st != RAD_OK) return st;
+	const HostScene &s = c->scene;
+	std::string werr;
+	if (c->writer.write(path, s.pool.data(), s.trinum, s.o.old_format != 0, werr) != 0) {
+		std::fprintf(stderr, "%s\n", werr.c_str());
+		if (!final_file) return RAD_OK;  // the reference skips an unwritable snapshot
+		// and writes the final result to stdout instead (rad.cpp:131-138)
+		if (c->writer.write("/dev/stdout", s.pool.data(), s.trinum, s.o.old_format != 0, werr) != 0)
+			return fail(c, RAD_EIO, werr);
+		return RAD_OK;
+	}
+	if (s.o.zip) {
+		std::printf("Compress output file %s .\n", path.c_str());
+		const std::string cmd = "gzip " + path;
+		// rad.cpp:229-232 prints OK when system() returns non-zero
+		std::puts(std::system(cmd.c_str()) ? "OK" : "Failed");
+	}
+	if (s.o.interactive) s.log("[WARN] -interactive: the WebGL viewer is not part of this engine\n");
+	return RAD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rad_status rad_create(rad_ctx **out, int argc, const char *const *argv)
+{
+	if (!out) return RAD_EINVAL;
+	*out = nullptr;
+	std::unique_ptr<rad_ctx> c(new rad_ctx());
+	for (int i = 0; i < argc; i++) {
+		std::string err;
+		if (!parse_option(c->scene.o, argc, argv, i, err)) {
+			std::fprintf(stderr, "rad_create: %s\n", err.c_str());
+			return RAD_EINVAL;
+		}
+	}
+	*out = c.release();
+	return RAD_OK;
+}
+
+void rad_destroy(rad_ctx *c) { delete c; }
+
+const char *rad_last_error(const rad_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+rad_status rad_load_tri(rad_ctx *c, const char *path)
+{
+	if (!c || !path) return RAD_EINVAL;
+	const auto t0 = Clock::now();
+	c->ms_setup = 0;
+	if (c->scene.load_tri(path, c->err)) return RAD_EIO;
+	c->ms_setup += ms_since(t0);
+	return after_load(c);
+}
+
+rad_status rad_set_triangles(rad_ctx *c, int n, const float *p9, const int *frgb3, const int *brgb3)
+{
+	if (!c || n < 0 || (n > 0 && (!p9 || !frgb3 || !brgb3))) return RAD_EINVAL;
+	const auto t0 = Clock::now();
+	c->ms_setup = 0;
+	if (c->scene.set_triangles(n, p9, frgb3, brgb3, c->err)) return RAD_EINVAL;
+	c->ms_setup += ms_since(t0);
+	return after_load(c);
+}
+
+rad_status rad_prepare_host(rad_ctx *c)
+{
+	if (!c) return RAD_EINVAL;
+	if (!c->loaded) return fail(c, RAD_ESTATE, "no scene loaded");
+	if (c->prepared) return RAD_OK;
+	if (c->scene.o.write_cycle <= 0) return fail(c, RAD_EINVAL, "-write_cycle must be positive");
+	const auto t0 = Clock::now();
+	c->scene.init_radiosity();
+	c->scene.prepartition();
+	c->ms_setup += ms_since(t0);
+	c->prepared = true;
+	return RAD_OK;
+}
+
+rad_status rad_upload(rad_ctx *c)
+{
+	if (!c) return RAD_EINVAL;
+	if (!c->prepared) return fail(c, RAD_ESTATE, "rad_prepare_host first");
+	const auto t0 = Clock::now();
+	if (!c->dev) {
+		DeviceEngine *d = DeviceEngine::create(c->err);
+		if (!d) return RAD_EDEVICE;
+		c->dev.reset(d);
+	}
+	c->dev->timing = c->scene.o.kernel_timing != 0;
+	if (c->dev->upload(c->scene, c->tree, c->err)) return RAD_EDEVICE;
+	c->ms_upload = ms_since(t0);
+	c->uploaded = true;
+	c->device_newer = false;
+	return RAD_OK;
+}
+
+rad_status rad_shoot(rad_ctx *c, int max_shots, int *shots, int *converged)
+{
+	if (!c || !shots || !converged || max_shots < 0) return RAD_EINVAL;
+	if (!c->uploaded) return fail(c, RAD_ESTATE, "rad_upload first");
+	if (c->dev->shoot(max_shots, shots, converged, c->err)) return RAD_EDEVICE;
+	if (*shots > 0) c->device_newer = true;
+	return RAD_OK;
+}
+
+rad_status rad_run(rad_ctx *c, const char *prefix, int *iterations)
+{
+	if (!c || !prefix) return RAD_EINVAL;
+	rad_status st;
+	if ((st = rad_prepare_host(c)) != RAD_OK) return st;
+	if (!c->uploaded && (st = rad_upload(c)) != RAD_OK) return st;
+	const HostScene &s = c->scene;
+	const int W = s.o.write_cycle;
+	int done = 0;  // shots so far
+	for (;;) {
+		const int next = (done / W + 1) * W;  // loop number of the next snapshot
+		int got = 0, conv = 0;
+		if ((st = rad_shoot(c, next - done, &got, &conv)) != RAD_OK) return st;
+		done += got;
+		// the reference announces loop L before trying shot L
+		if (done + 1 >= next) s.log("[INFO] Loop %d BEGIN\n", next);
+		if (conv) break;
+		if ((st = write_snapshot(c, std::string(prefix) + "." + std::to_string(next), false)) != RAD_OK) return st;
+		s.log("[INFO] END Loop. There are %i triangles.\n", s.trinum);
+	}
+	s.log("[INFO] Complete in #Iterations %d\n", done + 1);
+	const auto t0 = Clock::now();
+	if ((st = write_snapshot(c, std::string(prefix) + ".fin", true)) != RAD_OK) return st;
+	s.log("[INFO] Store file took %f sec. time.\n", ms_since(t0) / 1000.0);
+	if (iterations) *iterations = done + 1;
+	return RAD_OK;
+}
+
+rad_status rad_write_fin(rad_ctx *c, const char *path)
+{
+	if (!c || !path) return RAD_EINVAL;
+	if (!c->loaded) return fail(c, RAD_ESTATE, "no scene loaded");
+	rad_status st = sync_host(c);
+	if (st != RAD_OK) return st;
+	if (c->writer.write(path, c->scene.pool.data(), c->scene.trinum, c->scene.o.old_format != 0, c->err)) return RAD_EIO;
+	return RAD_OK;
+}
+
+rad_status rad_records(rad_ctx *c, int *trinum, void *records, float *g6)
+{
+	if (!c || !trinum) return RAD_EINVAL;
+	if (!c->loaded) return fail(c, RAD_ESTATE, "no scene loaded");
+	rad_status st = sync_host(c);
+	if (st != RAD_OK) return st;
+	*trinum = c->scene.trinum;
+	if (records) std::memcpy(records, c->scene.pool.data(), sizeof(Patch) * (size_t)c->scene.trinum);
+	if (g6) {
+		std::memcpy(g6, c->scene.G0, 12);
+		std::memcpy(g6 + 3, c->scene.G1, 12);
+	}
+	return RAD_OK;
+}
+
+rad_status rad_tree_info(rad_ctx *c, int *out3, float *mgl)
+{
+	if (!c || !out3 || !mgl) return RAD_EINVAL;
+	if (!c->loaded) return fail(c, RAD_ESTATE, "no scene loaded");
+	out3[0] = c->tree.n_nodes;
+	out3[1] = c->tree.n_list;
+	out3[2] = c->tree.grid_num;
+	*mgl = c->tree.min_grid;
+	return RAD_OK;
+}
+
+rad_status rad_cast(rad_ctx *c, int n, const void *rays, int *hit, int *cells, int *tests)
+{
+	if (!c || n < 0 || (n > 0 && (!rays || !hit))) return RAD_EINVAL;
+	if (!c->uploaded) return fail(c, RAD_ESTATE, "rad_upload first");
+	if (c->dev->cast(n, (const RayIn *)rays, hit, cells, tests, c->err)) return RAD_EDEVICE;
+	return RAD_OK;
+}
+
+rad_status rad_ff_pairs(rad_ctx *c, int n, const int *pairs, float *ff3)
+{
+	if (!c || n < 0 || (n > 0 && (!pairs || !ff3))) return RAD_EINVAL;
+	if (!c->uploaded) return fail(c, RAD_ESTATE, "rad_upload first");
+	if (c->dev->ff_pairs(n, pairs, ff3, c->err)) return RAD_EDEVICE;
+	return RAD_OK;
+}
+
+rad_status rad_counters(rad_ctx *c, long long *out)
+{
+	if (!c || !out) return RAD_EINVAL;
+	for (int k = 0; k < RAD_CNT_N; k++) out[k] = 0;
+	if (c->dev) c->dev->counters(out);
+	return RAD_OK;
+}
+
+rad_status rad_timings(rad_ctx *c, double *out6)
+{
+	if (!c || !out6) return RAD_EINVAL;
+	out6[0] = c->ms_setup;
+	out6[1] = c->ms_upload;
+	out6[2] = c->dev ? c->dev->shoot_ms() : 0;
+	out6[3] = c->dev ? (double)c->dev->kernel_launches() : 0;
+	out6[4] = c->dev ? c->dev->kernel_ms() : 0;
+	out6[5] = c->dev ? (double)c->dev->kernel_timed() : 0;
+	return RAD_OK;
+}
+
+}  // extern "C"

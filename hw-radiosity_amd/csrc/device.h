@@ -1,0 +1,60 @@
+// Device side of the engine (HIP, gfx950).  Host code sees plain C++ only.
+#pragma once
+
+#include <string>
+
+#include "octree.h"
+#include "scene.h"
+
+namespace rgpu {
+
+// Device counters, indices into DeviceEngine::counters().
+enum {
+	CNT_RAYS = 0,      // rays traced
+	CNT_REF_RAYS = 1,  // rays the reference casts (adds AdaptCalFF's discarded full-source ray)
+	CNT_CELLS = 2,     // leaf cells visited
+	CNT_TESTS = 3,     // ray/triangle tests
+	CNT_LEVELS = 4,    // octree levels the reference descends for those cells
+	CNT_PRECISION = 5, // "Precision Exceeded" steps (raycast.cpp:344-350)
+	CNT_STALLED = 6,   // rays the reference never finishes (hit = -2)
+	CNT_RECEIVERS = 7, // receiver updates (Shade calls)
+	CNT_N = 8
+};
+
+// One ray for the cast entry point: origin, direction, banned and target
+// original (RayHitted's arguments, raycast.cpp:335).
+struct RayIn {
+	float p[3], v[3];
+	int ban, target;
+};
+
+class DeviceEngine {
+public:
+	~DeviceEngine();
+	// Selects the current HIP device and creates the engine's stream.
+	static DeviceEngine *create(std::string &err);
+	// Copies the scene (after pre-partition) and the octree to the device and
+	// selects the first source.
+	int upload(const HostScene &s, const FlatOctree &t, std::string &err);
+	// Runs up to max_shots shooting iterations on the device.
+	int shoot(int max_shots, int *shots, int *converged, std::string &err);
+	// Copies accB, deltaaccB and deltaB back into the host pool.
+	int download(HostScene &s, std::string &err);
+	int cast(int n, const RayIn *rays, int *hit, int *cells, int *tests, std::string &err);
+	int ff_pairs(int n, const int *pairs, float *ff3, std::string &err);
+	void counters(long long *out);
+	double shoot_ms() const { return shoot_ms_; }  // device time of the shooting kernels so far
+	double kernel_ms() const { return kernel_ms_; }  // device time of the fused shoot kernel alone
+	long long kernel_launches() const { return kernel_launches_; }
+	long long kernel_timed() const { return kernel_timed_; }
+	bool timing = false;  // time each shoot kernel with events (for the roofline)
+
+private:
+	DeviceEngine() = default;
+	struct Impl;
+	Impl *d_ = nullptr;
+	double shoot_ms_ = 0, kernel_ms_ = 0;
+	long long kernel_launches_ = 0, kernel_timed_ = 0;
+};
+
+}  // namespace rgpu

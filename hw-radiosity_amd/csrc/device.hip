@@ -1,0 +1,831 @@
+// Device kernels and buffers of the shooting loop (gfx950).
+//
+// Layout in HBM
+//   originals (octree hit tests), AoS float4:   tri_nc[2*i] = {n.xyz, c.x},
+//       tri_nc[2*i+1] = {c.y, c.z, -, -}, tri_se[3*i+s] = side plane s
+//   octree: cell_leaf[gt^3] (TreeNodeNum tabulated), leaf_box[2*l] =
+//       {g0.xyz, first item}, {g1.xyz, count | depth<<24}, items[]
+//   receivers (every pool slot), SoA by vertex: vx/vy/vz[3*r+v],
+//       acc/dacc[3*r+v] = accB/deltaaccB of vertex v (float4),
+//       nrm[r] = {n.xyz, parent}, cA[r] = {c.xyz, area}, rho[r] = Frgb as
+//       floats, dB[r] = deltaB
+//
+// One shooting iteration (rad.cpp:406-451) is two launches: k_prep (one
+// lane) turns the best candidate key into a source descriptor (whole
+// source and its two halves, AdaptCalFF/PartitionSource shade.cpp:141-168,
+// 76-99) and clears that source's deltaB; k_shoot gives each (receiver,
+// vertex) a lane, 21 receivers per 64-lane wave, casts the vertex's rays
+// (CalFF shade.cpp:107-127, RayHitted raycast.cpp:335-360), applies
+// CalRadiosity (shade.cpp:244-259) with the three vertices combined by
+// lane shuffles in the reference's order, and reduces the next iteration's
+// candidate key (listCanditate rad.cpp:387-404) by wave and block maxima
+// and one atomicMax per block.  The host only polls a done flag every few
+// dozen iterations.
+//
+// Every float operation is IEEE single or double in source order
+// (-ffp-contract=off, correctly rounded division and square root); with
+// F=true an explicit fma sits at each of the reference compiler's
+// contraction sites, so results are bit-identical to the reference.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "device.h"
+
+namespace rgpu {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
+constexpr int kMaxSrc = 256;
+constexpr int kMaxWalk = 1 << 24;
+constexpr double kPiD = 3.1415926535897932;
+
+struct DScene {
+	const float4 *__restrict__ tri_nc;
+	const float4 *__restrict__ tri_se;
+	const int *__restrict__ cell_leaf;
+	const float4 *__restrict__ leaf_box;
+	const int *__restrict__ items;
+	float g0x, g0y, g0z, g1x, g1y, g1z;
+	double mgl;
+	int gt;
+};
+
+struct DRecv {
+	const float *__restrict__ vx;
+	const float *__restrict__ vy;
+	const float *__restrict__ vz;
+	const float4 *__restrict__ nrm;
+	const float4 *__restrict__ cA;
+	const float4 *__restrict__ rho;
+	float4 *acc;
+	float4 *dacc;
+	float4 *dB;
+	int nslots;
+};
+
+struct SrcPatch {
+	float c[3], n[3], area, nlen;
+};
+
+struct SrcDesc {
+	int phys, log, split, pad;
+	float dB[3], pad2;
+	SrcPatch whole, half[2];
+};
+
+struct Loop {
+	unsigned long long key;  // best candidate of the coming iteration
+	int count;               // candidates (multi-source selection)
+	int done;
+	int shots;
+	int nsrc;
+	int pad[2];
+	SrcDesc src[kMaxSrc];
+};
+
+struct Cnt {
+	int rays, ref_rays, cells, tests, levels, precision, stalled;
+};
+
+// ---- arithmetic -----------------------------------------------------------
+template <bool F> __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz)
+{
+	if (F) return __builtin_fmaf(az, bz, __builtin_fmaf(ax, bx, ay * by));
+	return ax * bx + ay * by + az * bz;
+}
+template <bool F> __device__ __forceinline__ float axpy(float t, float v, float p)
+{
+	if (F) return __builtin_fmaf(t, v, p);
+	return p + t * v;
+}
+
+// Candidate key: unshot energy (order-preserving bits, -0 folded to +0) in
+// the high word; the slot in the low word so that the maximum takes the
+// highest slot on ties (V0's sort, rad.cpp:398-402), or with LOWEST the
+// lowest slot (MaxDeltaRad's strict >, rad.cpp:373).
+template <bool LOWEST> __device__ __forceinline__ unsigned long long cand_key(float sum, int slot)
+{
+	const float s = sum + 0.0f;
+	const uint32_t b = __float_as_uint(s);
+	const uint32_t ob = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+	const uint32_t lo = LOWEST ? (0xffffffffu - (uint32_t)slot) : (uint32_t)slot;
+	return ((unsigned long long)ob << 32) | lo;
+}
+template <bool LOWEST> __device__ __forceinline__ int key_slot(unsigned long long k)
+{
+	const uint32_t lo = (uint32_t)k;
+	return LOWEST ? (int)(0xffffffffu - lo) : (int)lo;
+}
+
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long k)
+{
+	for (int off = 32; off > 0; off >>= 1) {
+		const unsigned long long o = __shfl_xor(k, off);
+		k = o > k ? o : k;
+	}
+	return k;
+}
+__device__ __forceinline__ int wave_sum(int x)
+{
+	for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+	return x;
+}
+
+// ---- RayHitted ------------------------------------------------------------
+// The walk of raycast.cpp:335-360 over the tabulated octree: find the leaf
+// of q, test its list (TriListHitted/TriHitted, raycast.cpp:283-333) and
+// return the nearest hit of the first leaf with any hit; else step q to
+// just past the leaf's nearest exit plane (NextPoint, raycast.cpp:235-251).
+template <bool F>
+__device__ int walk(const DScene &S, float px, float py, float pz, float vx, float vy, float vz, int ban, Cnt &c)
+{
+	float t = 0.f, qx = px, qy = py, qz = pz;
+	int prev = -1;
+	for (int steps = 0;;) {
+		if (qx < S.g0x || qx > S.g1x || qy < S.g0y || qy > S.g1y || qz < S.g0z || qz > S.g1z) return -1;
+		const int gx = (int)(((double)qx - (double)S.g0x) / S.mgl);
+		const int gy = (int)(((double)qy - (double)S.g0y) / S.mgl);
+		const int gz = (int)(((double)qz - (double)S.g0z) / S.mgl);
+		if ((unsigned)gx >= (unsigned)S.gt || (unsigned)gy >= (unsigned)S.gt || (unsigned)gz >= (unsigned)S.gt) {
+			c.stalled++;  // outside the tabulated descent: cannot happen for q inside [G0, G1]
+			return -2;
+		}
+		const int leaf = S.cell_leaf[(gx * S.gt + gy) * S.gt + gz];
+		if (++steps > kMaxWalk) { c.stalled++; return -2; }
+		if (leaf == prev) {
+			c.precision++;
+			const float nt = (float)((double)t + 1.0);
+			if (nt == t) { c.stalled++; return -2; }  // the reference loops forever here
+			t = nt;
+			qx = axpy<F>(t, vx, px);
+			qy = axpy<F>(t, vy, py);
+			qz = axpy<F>(t, vz, pz);
+			continue;
+		}
+		const float4 b0 = S.leaf_box[2 * leaf], b1 = S.leaf_box[2 * leaf + 1];
+		const int begin = __float_as_int(b0.w), cd = __float_as_int(b1.w), cnt = cd & 0xffffff;
+		c.cells++;
+		c.levels += cd >> 24;
+		float tmin = 1000000.0f;
+		int best = -1;
+		for (int e = 0; e < cnt; e++) {
+			const int id = S.items[begin + e];
+			if (id == ban) continue;
+			c.tests++;
+			const float4 a = S.tri_nc[2 * id], b = S.tri_nc[2 * id + 1];
+			const float a1 = dot3<F>(vx, vy, vz, a.x, a.y, a.z);
+			const float a2 = dot3<F>(a.w - px, b.x - py, b.y - pz, a.x, a.y, a.z);
+			const float tt = a2 / a1;
+			if ((double)tt < 0.0001) continue;
+			if (tt >= tmin) continue;
+			const float hx = axpy<F>(tt, vx, px), hy = axpy<F>(tt, vy, py), hz = axpy<F>(tt, vz, pz);
+			const float4 s0 = S.tri_se[3 * id];
+			if (dot3<F>(hx, hy, hz, s0.x, s0.y, s0.z) + s0.w < 0.0f) continue;
+			const float4 s1 = S.tri_se[3 * id + 1];
+			if (dot3<F>(hx, hy, hz, s1.x, s1.y, s1.z) + s1.w < 0.0f) continue;
+			const float4 s2 = S.tri_se[3 * id + 2];
+			if (dot3<F>(hx, hy, hz, s2.x, s2.y, s2.z) + s2.w < 0.0f) continue;
+			tmin = tt;
+			best = id;
+		}
+		if (best >= 0) return best;
+		prev = leaf;
+		float tm = 1000000.0f;
+		if ((double)fabsf(vx) > 1e-6) {
+			const float s = vx > 0 ? (b1.x - px) / vx : (b0.x - px) / vx;
+			tm = s < tm ? s : tm;
+		}
+		if ((double)fabsf(vy) > 1e-6) {
+			const float s = vy > 0 ? (b1.y - py) / vy : (b0.y - py) / vy;
+			tm = s < tm ? s : tm;
+		}
+		if ((double)fabsf(vz) > 1e-6) {
+			const float s = vz > 0 ? (b1.z - pz) / vz : (b0.z - pz) / vz;
+			tm = s < tm ? s : tm;
+		}
+		tm = (float)((double)tm + 0.1);
+		qx = axpy<F>(tm, vx, px);
+		qy = axpy<F>(tm, vy, py);
+		qz = axpy<F>(tm, vz, pz);
+		t = tm;
+	}
+}
+
+// CalFF (shade.cpp:107-127): point-to-patch form factor times visibility.
+// cast=false evaluates the prefix only and reports whether the reference
+// would cast a ray (for the reference-equivalent ray count).
+template <bool F>
+__device__ float point_ff(const DScene &S, const SrcPatch &s, int logsrc, float nx, float ny, float nz, float nlen,
+                          int logdst, float px, float py, float pz, bool cast, Cnt &c)
+{
+	const float dx = s.c[0] - px, dy = s.c[1] - py, dz = s.c[2] - pz;
+	const float ds = dot3<F>(dx, dy, dz, s.n[0], s.n[1], s.n[2]);
+	const float dd = -dot3<F>(dx, dy, dz, nx, ny, nz);
+	if ((ds < 0) != (dd < 0)) return 0.0f;
+	const float r2 = dot3<F>(dx, dy, dz, dx, dy, dz);
+	const float dl = __builtin_sqrtf(r2);
+	const float c1 = ds / dl / s.nlen;
+	const float c2 = dd / dl / nlen;
+	float ff = c1 * c2;
+	if (ff <= 0.0f) return 0.0f;
+	ff = (float)((double)ff * ((double)s.area / ((double)r2 * kPiD)));
+	if (ff <= 0.0f) return 0.0f;
+	c.ref_rays++;
+	if (!cast) return ff;
+	c.rays++;
+	return walk<F>(S, px, py, pz, dx, dy, dz, logdst, c) == logsrc ? ff : 0.0f;
+}
+
+// Source descriptor of slot `idx`: the whole patch and, when its longest
+// squared edge is at least 0.1, the two halves AdaptCalFF shoots from.
+template <bool F> __device__ void make_source(const DRecv &R, int idx, SrcDesc &d)
+{
+	float p[3][3];
+	for (int v = 0; v < 3; v++) {
+		p[v][0] = R.vx[3 * idx + v];
+		p[v][1] = R.vy[3 * idx + v];
+		p[v][2] = R.vz[3 * idx + v];
+	}
+	const float4 nr = R.nrm[idx], ca = R.cA[idx], db = R.dB[idx];
+	d.phys = idx;
+	d.log = __float_as_int(nr.w);
+	d.dB[0] = db.x;
+	d.dB[1] = db.y;
+	d.dB[2] = db.z;
+	const float nlen = __builtin_sqrtf(dot3<F>(nr.x, nr.y, nr.z, nr.x, nr.y, nr.z));
+	d.whole.c[0] = ca.x; d.whole.c[1] = ca.y; d.whole.c[2] = ca.z;
+	d.whole.n[0] = nr.x; d.whole.n[1] = nr.y; d.whole.n[2] = nr.z;
+	d.whole.area = ca.w;
+	d.whole.nlen = nlen;
+	int edge = 0;
+	float maxlen = 0.0f;
+	for (int v = 0; v < 3; v++) {
+		const int w = (v + 1) % 3;
+		const float ex = p[w][0] - p[v][0], ey = p[w][1] - p[v][1], ez = p[w][2] - p[v][2];
+		const float l = dot3<F>(ex, ey, ez, ex, ey, ez);
+		if (maxlen < l) { maxlen = l; edge = v; }
+	}
+	d.split = !((double)maxlen < 0.1);
+	if (!d.split) return;
+	const int i0 = edge, i1 = (edge + 1) % 3, i2 = (edge + 2) % 3;
+	float m[3];
+	for (int k = 0; k < 3; k++) m[k] = (float)((double)(p[i0][k] + p[i1][k]) / 2.0);
+	for (int k = 0; k < 3; k++) {
+		d.half[0].c[k] = (float)((double)(m[k] + p[i2][k] + p[i0][k]) / 3.0);
+		d.half[1].c[k] = (float)((double)(m[k] + p[i1][k] + p[i2][k]) / 3.0);
+	}
+	for (int h = 0; h < 2; h++) {
+		d.half[h].n[0] = nr.x; d.half[h].n[1] = nr.y; d.half[h].n[2] = nr.z;
+		d.half[h].area = (float)((double)ca.w / 2.0);
+		d.half[h].nlen = nlen;
+	}
+}
+
+// ---- kernels --------------------------------------------------------------
+template <bool LOWEST>
+__global__ __launch_bounds__(kBlock) void k_select(DRecv R, Loop *L, float converge)
+{
+	__shared__ unsigned long long wmax[kWavesPerBlock];
+	const int r = blockIdx.x * kBlock + threadIdx.x;
+	unsigned long long key = 0;
+	if (r < R.nslots && __float_as_int(R.nrm[r].w) >= 0) {
+		const float4 db = R.dB[r];
+		const float sum = db.x + db.y + db.z;
+		if (sum > converge) key = cand_key<LOWEST>(sum, r);
+	}
+	key = wave_max(key);
+	if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = key;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int w = 1; w < kWavesPerBlock; w++) key = wmax[w] > key ? wmax[w] : key;
+		if (key) atomicMax(&L->key, key);
+	}
+}
+
+template <bool F, bool LOWEST> __global__ void k_prep(DRecv R, Loop *L)
+{
+	if (threadIdx.x != 0 || L->done) return;
+	const unsigned long long key = L->key;
+	L->key = 0;
+	if (key == 0) {
+		L->done = 1;
+		return;
+	}
+	const int idx = key_slot<LOWEST>(key);
+	make_source<F>(R, idx, L->src[0]);
+	R.dB[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+	L->nsrc = 1;
+	L->shots++;
+}
+
+// Multi-source selection (-parallel_src k > 1): collect candidate keys, then
+// one block takes the k = min(ceil(|C|/5), parallel_src) largest in
+// descending order (listCanditate, rad.cpp:398-402).
+__global__ __launch_bounds__(kBlock) void k_collect(DRecv R, Loop *L, unsigned long long *cand, float converge)
+{
+	if (L->done) return;
+	const int r = blockIdx.x * kBlock + threadIdx.x;
+	if (r >= R.nslots || __float_as_int(R.nrm[r].w) < 0) return;
+	const float4 db = R.dB[r];
+	const float sum = db.x + db.y + db.z;
+	if (sum > converge) cand[atomicAdd(&L->count, 1)] = cand_key<false>(sum, r);
+}
+
+template <bool F> __global__ __launch_bounds__(1024) void k_topk(DRecv R, Loop *L, const unsigned long long *cand, int parallel_src)
+{
+	__shared__ unsigned long long wmax[1024 / kWave];
+	__shared__ int picked[kMaxSrc];
+	if (L->done) return;
+	const int count = L->count;
+	const int k = min((count + 4) / 5, parallel_src);
+	if (k <= 0) {
+		__syncthreads();
+		if (threadIdx.x == 0) { L->done = 1; L->count = 0; }
+		return;
+	}
+	unsigned long long bound = ~0ull;
+	for (int round = 0; round < k; round++) {
+		unsigned long long best = 0;
+		for (int i = threadIdx.x; i < count; i += blockDim.x) {
+			const unsigned long long c = cand[i];
+			if (c < bound && c > best) best = c;
+		}
+		best = wave_max(best);
+		if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = best;
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			for (int w = 1; w < (int)(blockDim.x / kWave); w++) best = wmax[w] > best ? wmax[w] : best;
+			wmax[0] = best;
+			picked[round] = key_slot<false>(best);
+		}
+		__syncthreads();
+		bound = wmax[0];
+		__syncthreads();
+	}
+	if (threadIdx.x < k) make_source<F>(R, picked[threadIdx.x], L->src[threadIdx.x]);
+	__syncthreads();
+	if (threadIdx.x < k) R.dB[picked[threadIdx.x]] = make_float4(0.f, 0.f, 0.f, 0.f);
+	if (threadIdx.x == 0) {
+		L->nsrc = k;
+		L->shots++;
+		L->count = 0;
+	}
+}
+
+template <bool F, bool FUSED_SELECT, bool LOWEST>
+__global__ __launch_bounds__(kBlock) void k_shoot(DScene S, DRecv R, Loop *L, float converge, long long *bcnt, int nblocks)
+{
+	if (L->done) return;
+	__shared__ unsigned long long wmax[kWavesPerBlock];
+	__shared__ int wcnt[kWavesPerBlock][8];
+	// XCD-aware order: consecutive hardware blocks go to different XCDs, so
+	// give each XCD a contiguous run of receivers (and of their rays' cells)
+	const int xcd = blockIdx.x & 7, per = (nblocks + 7) >> 3;
+	const int vb = xcd * per + (blockIdx.x >> 3);
+	const int lane = threadIdx.x & 63;
+	const int rl = lane / 3, v = lane - 3 * rl, base = lane - v;
+	const int r = (vb * kWavesPerBlock + (threadIdx.x >> 6)) * kRecvPerWave + rl;
+	const bool slot = lane < 63 && vb < nblocks && r < R.nslots;
+	const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+	const float4 nr = slot ? R.nrm[r] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+	const int parent = __float_as_int(nr.w);
+	float4 acc = slot ? R.acc[3 * r + v] : zero;
+	// saturated: all nine accB >= 255 (rad.cpp:424-428)
+	const bool vfull = !(acc.x < 255.0f || acc.y < 255.0f || acc.z < 255.0f);
+	const bool full = __shfl(vfull, base) && __shfl(vfull, base + 1) && __shfl(vfull, base + 2);
+	const int nsrc = L->nsrc;
+	bool chosen = false;
+	for (int k = 0; k < nsrc; k++) chosen |= (L->src[k].phys == r);
+	const bool live = slot && parent >= 0 && !chosen && !full;
+	float4 db = slot ? R.dB[r] : zero;
+	Cnt c = {0, 0, 0, 0, 0, 0, 0};
+	if (live) {
+		const float px = R.vx[3 * r + v], py = R.vy[3 * r + v], pz = R.vz[3 * r + v];
+		const float4 rho = R.rho[r];
+		const float nlen = __builtin_sqrtf(dot3<F>(nr.x, nr.y, nr.z, nr.x, nr.y, nr.z));
+		float4 dacc = zero;
+		for (int k = 0; k < nsrc; k++) {
+			const SrcDesc &s = L->src[k];
+			float ff;
+			if (s.split) {
+				point_ff<F>(S, s.whole, s.log, nr.x, nr.y, nr.z, nlen, parent, px, py, pz, false, c);
+				const float f1 = point_ff<F>(S, s.half[0], s.log, nr.x, nr.y, nr.z, nlen, parent, px, py, pz, true, c);
+				const float f2 = point_ff<F>(S, s.half[1], s.log, nr.x, nr.y, nr.z, nlen, parent, px, py, pz, true, c);
+				ff = f1 + f2;
+			} else {
+				ff = point_ff<F>(S, s.whole, s.log, nr.x, nr.y, nr.z, nlen, parent, px, py, pz, true, c);
+			}
+			const float kr = (float)(0.5 / 255.0);
+			const float dr = rho.x * kr * s.dB[0] / 3 * ff;
+			const float dg = rho.y * kr * s.dB[1] / 3 * ff;
+			const float dbb = rho.z * kr * s.dB[2] / 3 * ff;
+			// the receiver's three lanes are all live or all not
+			const float r1 = __shfl(dr, base + 1), g1 = __shfl(dg, base + 1), b1 = __shfl(dbb, base + 1);
+			const float r2 = __shfl(dr, base + 2), g2 = __shfl(dg, base + 2), b2 = __shfl(dbb, base + 2);
+			acc.x += dr;
+			acc.y += dg;
+			acc.z += dbb;
+			dacc = make_float4(dr, dg, dbb, 0.f);
+			if (v == 0) {
+				db.x += dr; db.x += r1; db.x += r2;
+				db.y += dg; db.y += g1; db.y += g2;
+				db.z += dbb; db.z += b1; db.z += b2;
+			}
+		}
+		R.acc[3 * r + v] = acc;
+		R.dacc[3 * r + v] = dacc;
+		if (v == 0) R.dB[r] = db;
+	}
+	if (FUSED_SELECT) {
+		unsigned long long key = 0;
+		if (slot && v == 0 && parent >= 0) {
+			const float sum = db.x + db.y + db.z;
+			if (sum > converge) key = cand_key<LOWEST>(sum, r);
+		}
+		key = wave_max(key);
+		if (lane == 0) wmax[threadIdx.x >> 6] = key;
+	}
+	const int recv = (live && v == 0) ? nsrc : 0;
+	int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, recv};
+	for (int k = 0; k < 8; k++) vals[k] = wave_sum(vals[k]);
+	if (lane == 0)
+		for (int k = 0; k < 8; k++) wcnt[threadIdx.x >> 6][k] = vals[k];
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		if (FUSED_SELECT) {
+			unsigned long long key = wmax[0];
+			for (int w = 1; w < kWavesPerBlock; w++) key = wmax[w] > key ? wmax[w] : key;
+			if (key) atomicMax(&L->key, key);
+		}
+		for (int k = 0; k < 8; k++) {
+			long long s = 0;
+			for (int w = 0; w < kWavesPerBlock; w++) s += wcnt[w][k];
+			bcnt[(size_t)blockIdx.x * 8 + k] += s;
+		}
+	}
+}
+
+template <bool F>
+__global__ __launch_bounds__(kBlock) void k_cast(DScene S, const RayIn *rays, int n, int *hit, int *cells, int *tests)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) return;
+	const RayIn r = rays[i];
+	Cnt c = {0, 0, 0, 0, 0, 0, 0};
+	const int h = walk<F>(S, r.p[0], r.p[1], r.p[2], r.v[0], r.v[1], r.v[2], r.ban, c);
+	hit[i] = h;
+	cells[i] = c.cells;
+	tests[i] = c.tests;
+}
+
+template <bool F>
+__global__ __launch_bounds__(kBlock) void k_ff_pairs(DScene S, DRecv R, const int *pairs, int n, float *ff3)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) return;
+	SrcDesc s;
+	make_source<F>(R, pairs[2 * i], s);
+	const int d = pairs[2 * i + 1];
+	const float4 nr = R.nrm[d];
+	const float nlen = __builtin_sqrtf(dot3<F>(nr.x, nr.y, nr.z, nr.x, nr.y, nr.z));
+	Cnt c = {0, 0, 0, 0, 0, 0, 0};
+	for (int v = 0; v < 3; v++) {
+		const float px = R.vx[3 * d + v], py = R.vy[3 * d + v], pz = R.vz[3 * d + v];
+		const int logdst = __float_as_int(nr.w);
+		float ff;
+		if (s.split)
+			ff = point_ff<F>(S, s.half[0], s.log, nr.x, nr.y, nr.z, nlen, logdst, px, py, pz, true, c) +
+			     point_ff<F>(S, s.half[1], s.log, nr.x, nr.y, nr.z, nlen, logdst, px, py, pz, true, c);
+		else
+			ff = point_ff<F>(S, s.whole, s.log, nr.x, nr.y, nr.z, nlen, logdst, px, py, pz, true, c);
+		ff3[3 * i + v] = ff;
+	}
+}
+
+// ---- host side --------------------------------------------------------------
+#define HIP_OK(expr, what)                                                        \
+	do {                                                                          \
+		hipError_t e_ = (expr);                                                   \
+		if (e_ != hipSuccess) {                                                   \
+			err = std::string(what) + ": " + hipGetErrorString(e_);               \
+			return -1;                                                            \
+		}                                                                         \
+	} while (0)
+
+template <typename T> struct DevBuf {
+	T *p = nullptr;
+	size_t n = 0;
+	~DevBuf() { if (p) (void)hipFree(p); }
+	int alloc(size_t count, std::string &err)
+	{
+		if (p) (void)hipFree(p);
+		p = nullptr;
+		n = count;
+		if (count == 0) return 0;
+		HIP_OK(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+		return 0;
+	}
+	int put(const T *src, size_t count, hipStream_t s, std::string &err)
+	{
+		if (alloc(count, err)) return -1;
+		if (count) HIP_OK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+		return 0;
+	}
+};
+
+}  // namespace
+
+struct DeviceEngine::Impl {
+	hipStream_t stream = nullptr;
+	DevBuf<float4> tri_nc, tri_se, leaf_box, nrm, cA, rho, acc, dacc, dB;
+	DevBuf<int> cell_leaf, items;
+	DevBuf<float> vx, vy, vz;
+	DevBuf<Loop> loop;
+	DevBuf<long long> bcnt;
+	DevBuf<unsigned long long> cand;
+	DScene S;
+	DRecv R;
+	Loop *host_loop = nullptr;  // pinned mirror of the loop header
+	int nblocks = 0, n_orig = 0;
+	bool fma = true, lowest = false;
+	int parallel_src = 1, batch = 32;
+	float converge = 1200.f;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	static constexpr int kEv = 16;
+	hipEvent_t kev[2 * kEv] = {};
+};
+
+DeviceEngine::~DeviceEngine()
+{
+	if (!d_) return;
+	if (d_->stream) (void)hipStreamSynchronize(d_->stream);
+	if (d_->host_loop) (void)hipHostFree(d_->host_loop);
+	if (d_->ev0) (void)hipEventDestroy(d_->ev0);
+	if (d_->ev1) (void)hipEventDestroy(d_->ev1);
+	for (hipEvent_t &e : d_->kev)
+		if (e) (void)hipEventDestroy(e);
+	Impl *d = d_;
+	d_ = nullptr;
+	hipStream_t s = d->stream;
+	delete d;  // frees the device buffers
+	if (s) (void)hipStreamDestroy(s);
+}
+
+DeviceEngine *DeviceEngine::create(std::string &err)
+{
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+		err = "no HIP device available: the engine runs on the GPU only";
+		return nullptr;
+	}
+	DeviceEngine *e = new DeviceEngine();
+	e->d_ = new Impl();
+	if (hipStreamCreateWithFlags(&e->d_->stream, hipStreamNonBlocking) != hipSuccess ||
+	    hipHostMalloc((void **)&e->d_->host_loop, sizeof(Loop), hipHostMallocDefault) != hipSuccess ||
+	    hipEventCreate(&e->d_->ev0) != hipSuccess || hipEventCreate(&e->d_->ev1) != hipSuccess) {
+		err = "HIP stream/event/pinned allocation failed";
+		delete e;
+		return nullptr;
+	}
+	for (hipEvent_t &ev : e->d_->kev)
+		if (hipEventCreate(&ev) != hipSuccess) {
+		err = "HIP stream/event/pinned allocation failed";
+		delete e;
+		return nullptr;
+	}
+	return e;
+}
+
+int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &err)
+{
+	Impl &d = *d_;
+	const int n0 = s.n_orig, n = s.trinum;
+	d.n_orig = n0;
+	d.fma = s.o.fma != 0;
+	d.lowest = s.o.lowest != 0;
+	d.parallel_src = s.o.parallel_src;
+	d.batch = s.o.batch > 0 ? s.o.batch : 32;
+	d.converge = s.o.converge;
+	if (!d.lowest && d.parallel_src > kMaxSrc) {
+		err = "-parallel_src above " + std::to_string(kMaxSrc) + " is not supported";
+		return -1;
+	}
+	if ((size_t)t.gt * t.gt * t.gt > (size_t)1 << 30) {
+		err = "octree cell table too large";
+		return -1;
+	}
+	// originals as the octree sees them: post-pre-partition slot contents
+	std::vector<float4> nc((size_t)n0 * 2), se((size_t)n0 * 3);
+	for (int i = 0; i < n0; i++) {
+		const Patch &p = s.pool[i];
+		nc[2 * i] = make_float4(p.n[0], p.n[1], p.n[2], p.c[0]);
+		nc[2 * i + 1] = make_float4(p.c[1], p.c[2], 0.f, 0.f);
+		for (int k = 0; k < 3; k++) se[3 * i + k] = make_float4(p.se[k][0], p.se[k][1], p.se[k][2], p.se[k][3]);
+	}
+	std::vector<float> vx((size_t)n * 3), vy((size_t)n * 3), vz((size_t)n * 3);
+	std::vector<float4> nrm(n), cA(n), rho(n), acc((size_t)n * 3), dacc((size_t)n * 3), dB(n);
+	for (int i = 0; i < n; i++) {
+		const Patch &p = s.pool[i];
+		for (int v = 0; v < 3; v++) {
+			vx[3 * i + v] = p.p[v][0];
+			vy[3 * i + v] = p.p[v][1];
+			vz[3 * i + v] = p.p[v][2];
+			acc[3 * i + v] = make_float4(p.accB[v][0], p.accB[v][1], p.accB[v][2], 0.f);
+			dacc[3 * i + v] = make_float4(p.deltaaccB[v][0], p.deltaaccB[v][1], p.deltaaccB[v][2], 0.f);
+		}
+		int par = p.parent;
+		float parf;
+		std::memcpy(&parf, &par, 4);
+		nrm[i] = make_float4(p.n[0], p.n[1], p.n[2], parf);
+		cA[i] = make_float4(p.c[0], p.c[1], p.c[2], p.area);
+		rho[i] = make_float4((float)p.Frgb[0], (float)p.Frgb[1], (float)p.Frgb[2], 0.f);
+		dB[i] = make_float4(p.deltaB[0], p.deltaB[1], p.deltaB[2], 0.f);
+	}
+	hipStream_t st = d.stream;
+	if (d.tri_nc.put(nc.data(), nc.size(), st, err) || d.tri_se.put(se.data(), se.size(), st, err) ||
+	    d.cell_leaf.put(t.cell_leaf.data(), t.cell_leaf.size(), st, err) ||
+	    d.leaf_box.put((const float4 *)t.leaf_box.data(), t.leaf_box.size() / 4, st, err) ||
+	    d.items.put(t.items.data(), t.items.size(), st, err) || d.vx.put(vx.data(), vx.size(), st, err) ||
+	    d.vy.put(vy.data(), vy.size(), st, err) || d.vz.put(vz.data(), vz.size(), st, err) ||
+	    d.nrm.put(nrm.data(), nrm.size(), st, err) || d.cA.put(cA.data(), cA.size(), st, err) ||
+	    d.rho.put(rho.data(), rho.size(), st, err) || d.acc.put(acc.data(), acc.size(), st, err) ||
+	    d.dacc.put(dacc.data(), dacc.size(), st, err) || d.dB.put(dB.data(), dB.size(), st, err))
+		return -1;
+	const int waves = (n + kRecvPerWave - 1) / kRecvPerWave;
+	d.nblocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+	const int grid = ((d.nblocks + 7) / 8) * 8;  // a whole number of blocks per XCD
+	std::vector<long long> zeros((size_t)grid * 8, 0);
+	if (d.bcnt.put(zeros.data(), zeros.size(), st, err)) return -1;
+	if (!d.lowest && d.parallel_src > 1 && d.cand.alloc((size_t)n + 1, err)) return -1;
+	std::memset(d.host_loop, 0, sizeof(Loop));
+	if (d.loop.put(d.host_loop, 1, st, err)) return -1;
+	d.S = DScene{d.tri_nc.p, d.tri_se.p, d.cell_leaf.p, d.leaf_box.p, d.items.p, s.G0[0], s.G0[1], s.G0[2],
+	             s.G1[0], s.G1[1], s.G1[2], (double)t.min_grid, t.gt};
+	d.R = DRecv{d.vx.p, d.vy.p, d.vz.p, d.nrm.p, d.cA.p, d.rho.p, d.acc.p, d.dacc.p, d.dB.p, n};
+	if (d.lowest || d.parallel_src <= 1) {
+		const int sb = (n + kBlock - 1) / kBlock;
+		if (sb > 0) {
+			if (d.lowest) hipLaunchKernelGGL(k_select<true>, dim3(sb), dim3(kBlock), 0, st, d.R, d.loop.p, d.converge);
+			else hipLaunchKernelGGL(k_select<false>, dim3(sb), dim3(kBlock), 0, st, d.R, d.loop.p, d.converge);
+		}
+	}
+	HIP_OK(hipGetLastError(), "select launch");
+	HIP_OK(hipStreamSynchronize(st), "upload");
+	shoot_ms_ = kernel_ms_ = 0;
+	kernel_launches_ = kernel_timed_ = 0;
+	return 0;
+}
+
+int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &err)
+{
+	Impl &d = *d_;
+	hipStream_t st = d.stream;
+	*shots = 0;
+	*converged = 0;
+	HIP_OK(hipMemcpyAsync(d.host_loop, d.loop.p, 64, hipMemcpyDeviceToHost, st), "loop read");
+	HIP_OK(hipStreamSynchronize(st), "loop read");
+	if (d.host_loop->done) {
+		*converged = 1;
+		return 0;
+	}
+	const int start = d.host_loop->shots;
+	const int grid = ((d.nblocks + 7) / 8) * 8;
+	const bool multi = !d.lowest && d.parallel_src > 1;
+	const int cb = (d.R.nslots + kBlock - 1) / kBlock;
+	int queued = 0;
+	while (queued < max_shots) {
+		const int b = std::min(d.batch, max_shots - queued);
+		HIP_OK(hipEventRecord(d.ev0, st), "event");
+		for (int i = 0; i < b; i++) {
+			if (multi) {
+				hipLaunchKernelGGL(k_collect, dim3(cb > 0 ? cb : 1), dim3(kBlock), 0, st, d.R, d.loop.p, d.cand.p, d.converge);
+				if (d.fma) hipLaunchKernelGGL(k_topk<true>, dim3(1), dim3(1024), 0, st, d.R, d.loop.p, d.cand.p, d.parallel_src);
+				else hipLaunchKernelGGL(k_topk<false>, dim3(1), dim3(1024), 0, st, d.R, d.loop.p, d.cand.p, d.parallel_src);
+			} else if (d.fma) {
+				if (d.lowest) hipLaunchKernelGGL((k_prep<true, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+				else hipLaunchKernelGGL((k_prep<true, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+			} else {
+				if (d.lowest) hipLaunchKernelGGL((k_prep<false, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+				else hipLaunchKernelGGL((k_prep<false, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+			}
+			if (grid > 0) {
+				if (timing) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv)], st), "event");
+#define SHOOT(F, FS, LO) hipLaunchKernelGGL((k_shoot<F, FS, LO>), dim3(grid), dim3(kBlock), 0, st, d.S, d.R, d.loop.p, d.converge, d.bcnt.p, d.nblocks)
+				if (multi) { if (d.fma) SHOOT(true, false, false); else SHOOT(false, false, false); }
+				else if (d.lowest) { if (d.fma) SHOOT(true, true, true); else SHOOT(false, true, true); }
+				else { if (d.fma) SHOOT(true, true, false); else SHOOT(false, true, false); }
+#undef SHOOT
+				if (timing) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv) + 1], st), "event");
+			}
+			if (timing && (i % Impl::kEv == Impl::kEv - 1 || i == b - 1)) {
+				// collect this group of kernel timings before the events are reused
+				HIP_OK(hipEventSynchronize(d.kev[2 * (i % Impl::kEv) + 1]), "event");
+				for (int j = 0; j <= i % Impl::kEv; j++) {
+					float ms = 0;
+					if (hipEventElapsedTime(&ms, d.kev[2 * j], d.kev[2 * j + 1]) == hipSuccess) {
+						kernel_ms_ += ms;
+						kernel_timed_++;
+					}
+				}
+			}
+		}
+		HIP_OK(hipGetLastError(), "shoot launch");
+		HIP_OK(hipEventRecord(d.ev1, st), "event");
+		HIP_OK(hipMemcpyAsync(d.host_loop, d.loop.p, 64, hipMemcpyDeviceToHost, st), "loop read");
+		HIP_OK(hipStreamSynchronize(st), "shoot");
+		float ms = 0;
+		(void)hipEventElapsedTime(&ms, d.ev0, d.ev1);
+		shoot_ms_ += ms;
+		queued += b;
+		kernel_launches_ += b;
+		if (d.host_loop->done) break;
+	}
+	*shots = d.host_loop->shots - start;
+	*converged = d.host_loop->done;
+	return 0;
+}
+
+int DeviceEngine::download(HostScene &s, std::string &err)
+{
+	Impl &d = *d_;
+	const int n = d.R.nslots;
+	std::vector<float4> acc((size_t)n * 3), dacc((size_t)n * 3), dB(n);
+	hipStream_t st = d.stream;
+	if (n) {
+		HIP_OK(hipMemcpyAsync(acc.data(), d.acc.p, acc.size() * sizeof(float4), hipMemcpyDeviceToHost, st), "download");
+		HIP_OK(hipMemcpyAsync(dacc.data(), d.dacc.p, dacc.size() * sizeof(float4), hipMemcpyDeviceToHost, st), "download");
+		HIP_OK(hipMemcpyAsync(dB.data(), d.dB.p, dB.size() * sizeof(float4), hipMemcpyDeviceToHost, st), "download");
+	}
+	HIP_OK(hipStreamSynchronize(st), "download");
+	for (int i = 0; i < n; i++) {
+		Patch &p = s.pool[i];
+		for (int v = 0; v < 3; v++) {
+			const float4 a = acc[3 * i + v], b = dacc[3 * i + v];
+			p.accB[v][0] = a.x; p.accB[v][1] = a.y; p.accB[v][2] = a.z;
+			p.deltaaccB[v][0] = b.x; p.deltaaccB[v][1] = b.y; p.deltaaccB[v][2] = b.z;
+		}
+		p.deltaB[0] = dB[i].x;
+		p.deltaB[1] = dB[i].y;
+		p.deltaB[2] = dB[i].z;
+	}
+	return 0;
+}
+
+int DeviceEngine::cast(int n, const RayIn *rays, int *hit, int *cells, int *tests, std::string &err)
+{
+	Impl &d = *d_;
+	if (n <= 0) return 0;
+	DevBuf<RayIn> dr;
+	DevBuf<int> dh, dc, dt;
+	hipStream_t st = d.stream;
+	if (dr.put(rays, n, st, err) || dh.alloc(n, err) || dc.alloc(n, err) || dt.alloc(n, err)) return -1;
+	const int blocks = (n + kBlock - 1) / kBlock;
+	if (d.fma) hipLaunchKernelGGL(k_cast<true>, dim3(blocks), dim3(kBlock), 0, st, d.S, dr.p, n, dh.p, dc.p, dt.p);
+	else hipLaunchKernelGGL(k_cast<false>, dim3(blocks), dim3(kBlock), 0, st, d.S, dr.p, n, dh.p, dc.p, dt.p);
+	HIP_OK(hipGetLastError(), "cast launch");
+	HIP_OK(hipMemcpyAsync(hit, dh.p, n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
+	if (cells) HIP_OK(hipMemcpyAsync(cells, dc.p, n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
+	if (tests) HIP_OK(hipMemcpyAsync(tests, dt.p, n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
+	HIP_OK(hipStreamSynchronize(st), "cast");
+	return 0;
+}
+
+int DeviceEngine::ff_pairs(int n, const int *pairs, float *ff3, std::string &err)
+{
+	Impl &d = *d_;
+	if (n <= 0) return 0;
+	for (int i = 0; i < 2 * n; i++)
+		if (pairs[i] < 0 || pairs[i] >= d.R.nslots) {
+			err = "pair index out of range";
+			return -1;
+		}
+	DevBuf<int> dp;
+	DevBuf<float> df;
+	hipStream_t st = d.stream;
+	if (dp.put(pairs, (size_t)2 * n, st, err) || df.alloc((size_t)3 * n, err)) return -1;
+	const int blocks = (n + kBlock - 1) / kBlock;
+	if (d.fma) hipLaunchKernelGGL(k_ff_pairs<true>, dim3(blocks), dim3(kBlock), 0, st, d.S, d.R, dp.p, n, df.p);
+	else hipLaunchKernelGGL(k_ff_pairs<false>, dim3(blocks), dim3(kBlock), 0, st, d.S, d.R, dp.p, n, df.p);
+	HIP_OK(hipGetLastError(), "ff launch");
+	HIP_OK(hipMemcpyAsync(ff3, df.p, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost, st), "ff");
+	HIP_OK(hipStreamSynchronize(st), "ff");
+	return 0;
+}
+
+void DeviceEngine::counters(long long *out)
+{
+	Impl &d = *d_;
+	for (int k = 0; k < CNT_N; k++) out[k] = 0;
+	if (!d.bcnt.p) return;
+	std::vector<long long> h(d.bcnt.n);
+	if (hipMemcpy(h.data(), d.bcnt.p, h.size() * sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess) return;
+	for (size_t i = 0; i < h.size(); i++) out[i % 8] += h[i];
+}
+
+}  // namespace rgpu

@@ -1,0 +1,243 @@
+// Octree build (raycast.cpp:24-233) and its flattening for the device.
+//
+// The tree is the reference's: a node is a leaf when it holds at most 10
+// triangles or sits at depth 6; otherwise it has 8 children, child bit j set
+// meaning the upper half in dimension j, split at the float midpoint; a
+// triangle is listed in every child it overlaps by the CrossOver/Clip test.
+// Subtrees are built in parallel (OpenMP tasks) and stitched; the node
+// numbering does not matter to any result.
+//
+// The device never walks the tree.  The reference's TreeNodeNum turns a
+// point into integer cell coordinates gnum = (int)((q - G0) / MinGridLen)
+// and descends with gsize = GridNum >> level; the leaf it reaches depends on
+// the three integers only, so the descent is tabulated once for every
+// (gx, gy, gz) in [0, GridNum + 1]^3 and a device lookup is a single load.
+#include "octree.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+namespace rgpu {
+namespace {
+
+// Clip (raycast.cpp:24-60): does segment p0-p1 meet the (x, y) rectangle?
+bool segment_meets_rect(const float *p0, const float *p1, const float *g0, const float *g1, int x, int y)
+{
+	const int out0 = (p0[x] < g0[x]) | (p0[x] > g1[x]) << 1 | (p0[y] < g0[y]) << 2 | (p0[y] > g1[y]) << 3;
+	const int out1 = (p1[x] < g0[x]) | (p1[x] > g1[x]) << 1 | (p1[y] < g0[y]) << 2 | (p1[y] > g1[y]) << 3;
+	if (out0 & out1) return false;
+	if (!(out0 | out1)) return true;
+	const float a = p1[y] - p0[y], b = p1[x] - p0[x];
+	const float side00 = a * (g0[x] - p0[x]) - b * (g0[y] - p0[y]);
+	if (side00 == 0) return true;
+	const bool neg = side00 < 0;
+	if (((a * (g0[x] - p0[x]) - b * (g1[y] - p0[y])) < 0) != neg) return true;
+	if (((a * (g1[x] - p0[x]) - b * (g0[y] - p0[y])) < 0) != neg) return true;
+	if (((a * (g1[x] - p0[x]) - b * (g1[y] - p0[y])) < 0) != neg) return true;
+	return false;
+}
+
+// InterpVector (raycast.cpp:91-100): the point of segment lo-hi where
+// coordinate d equals g.
+template <bool F> void point_at(const float *hi, const float *lo, float g, int d, float *out)
+{
+	const float span = hi[d] - lo[d];
+	if (span == 0.0f) {
+		std::memcpy(out, hi, 12);
+		return;
+	}
+	const float wa = (g - lo[d]) / span, wb = (hi[d] - g) / span;
+	for (int k = 0; k < 3; k++) out[k] = Arith<F>::lerp2(wa, hi[k], wb, lo[k]);
+}
+
+// CrossOver (raycast.cpp:109-155).  The three vertex pointers stay sorted
+// across dimensions exactly as the reference's in-place sort leaves them.
+template <bool F> bool overlaps(const Patch &t, const float *g0, const float *g1)
+{
+	const float *hi = t.p[0], *mid = t.p[1], *lo = t.p[2];
+	float a[3], b[3];
+	for (int x = 0; x < 3; x++) {
+		if (hi[x] < mid[x]) std::swap(hi, mid);
+		if (hi[x] < lo[x]) std::swap(hi, lo);
+		if (mid[x] < lo[x]) std::swap(mid, lo);
+		const int u = (x + 1) % 3, w = (x + 2) % 3;
+		if (g1[x] < lo[x]) return false;
+		point_at<F>(hi, lo, g1[x], x, a);
+		if (g1[x] < mid[x]) {
+			point_at<F>(mid, lo, g1[x], x, b);
+			return segment_meets_rect(a, b, g0, g1, u, w);
+		}
+		if (g1[x] <= hi[x]) {
+			point_at<F>(hi, mid, g1[x], x, b);
+			return segment_meets_rect(a, b, g0, g1, u, w);
+		}
+		if (g0[x] > hi[x]) return false;
+		point_at<F>(hi, lo, g0[x], x, a);
+		if (g0[x] > mid[x]) {
+			point_at<F>(hi, mid, g0[x], x, b);
+			return segment_meets_rect(a, b, g0, g1, u, w);
+		}
+		if (g0[x] >= lo[x]) {
+			point_at<F>(mid, lo, g0[x], x, b);
+			return segment_meets_rect(a, b, g0, g1, u, w);
+		}
+	}
+	return true;
+}
+
+struct BNode {
+	float g0[3], g1[3];
+	int sub[8];
+	int leaf;   // leaf number within its subtree, -1 for an inner node
+	int depth;
+};
+
+struct Subtree {
+	std::vector<BNode> nodes;
+	std::vector<int> item_begin, item_count;  // per leaf
+	std::vector<int> items;
+	float min_grid;
+};
+
+template <bool F> struct Builder {
+	const Patch *pool;
+	int task_depth;
+
+	int build(Subtree &st, const std::vector<int> &idx, int level, const float *g0, const float *g1)
+	{
+		const int id = (int)st.nodes.size();
+		st.nodes.push_back(BNode());
+		BNode nd;
+		std::memcpy(nd.g0, g0, 12);
+		std::memcpy(nd.g1, g1, 12);
+		nd.depth = level;
+		nd.leaf = -1;
+		if ((int)idx.size() <= kMaxTriPerNode || level >= kMaxLevel) {
+			if (g1[0] - g0[0] < st.min_grid) st.min_grid = g1[0] - g0[0];
+			for (int i = 0; i < 8; i++) nd.sub[i] = -1;
+			nd.leaf = (int)st.item_begin.size();
+			st.item_begin.push_back((int)st.items.size());
+			st.item_count.push_back((int)idx.size());
+			st.items.insert(st.items.end(), idx.begin(), idx.end());
+			st.nodes[id] = nd;
+			return id;
+		}
+		float mid[3];
+		for (int k = 0; k < 3; k++) mid[k] = (float)(0.5 * (double)g0[k] + 0.5 * (double)g1[k]);
+		float q0[8][3], q1[8][3];
+		std::vector<int> part[8];
+		for (int c = 0; c < 8; c++) {
+			for (int k = 0; k < 3; k++) {
+				const bool upper = (c >> k) & 1;
+				q0[c][k] = upper ? mid[k] : g0[k];
+				q1[c][k] = upper ? g1[k] : mid[k];
+			}
+		}
+		if (level < task_depth) {
+			// children in parallel, each into its own subtree, then stitched
+			std::unique_ptr<Subtree> kid[8];
+			for (int c = 0; c < 8; c++) {
+				kid[c].reset(new Subtree());
+				kid[c]->min_grid = st.min_grid;
+			}
+			for (int c = 0; c < 8; c++) {
+#pragma omp task shared(kid, q0, q1, idx) firstprivate(c)
+				{
+					std::vector<int> sel;
+					sel.reserve(idx.size());
+					for (int t : idx)
+						if (overlaps<F>(pool[t], q0[c], q1[c])) sel.push_back(t);
+					build(*kid[c], sel, level + 1, q0[c], q1[c]);
+				}
+			}
+#pragma omp taskwait
+			for (int c = 0; c < 8; c++) {
+				const int off = (int)st.nodes.size(), loff = (int)st.item_begin.size(), ioff = (int)st.items.size();
+				for (BNode b : kid[c]->nodes) {
+					for (int s = 0; s < 8; s++)
+						if (b.sub[s] >= 0) b.sub[s] += off;
+					if (b.leaf >= 0) b.leaf += loff;
+					st.nodes.push_back(b);
+				}
+				for (size_t l = 0; l < kid[c]->item_begin.size(); l++) {
+					st.item_begin.push_back(kid[c]->item_begin[l] + ioff);
+					st.item_count.push_back(kid[c]->item_count[l]);
+				}
+				st.items.insert(st.items.end(), kid[c]->items.begin(), kid[c]->items.end());
+				st.min_grid = std::min(st.min_grid, kid[c]->min_grid);
+				nd.sub[c] = off;
+			}
+		} else {
+			for (int c = 0; c < 8; c++) {
+				std::vector<int> sel;
+				sel.reserve(idx.size());
+				for (int t : idx)
+					if (overlaps<F>(pool[t], q0[c], q1[c])) sel.push_back(t);
+				nd.sub[c] = build(st, sel, level + 1, q0[c], q1[c]);
+			}
+		}
+		st.nodes[id] = nd;
+		return id;
+	}
+};
+
+}  // namespace
+
+FlatOctree build_octree(const Patch *pool, int n, const float *G0, const float *G1, bool fma, int threads)
+{
+	Subtree st;
+	st.min_grid = G1[0] - G0[0];
+	std::vector<int> all(n);
+	for (int i = 0; i < n; i++) all[i] = i;
+	const int nthreads = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#pragma omp single
+	{
+		if (fma) Builder<true>{pool, 2}.build(st, all, 0, G0, G1);
+		else Builder<false>{pool, 2}.build(st, all, 0, G0, G1);
+	}
+	FlatOctree T;
+	T.n_nodes = (int)st.nodes.size();
+	T.n_leaves = (int)st.item_begin.size();
+	T.n_list = (int)st.items.size() + T.n_leaves;
+	T.min_grid = st.min_grid;
+	T.grid_num = (int)((G1[0] - G0[0]) / st.min_grid);
+	T.items = std::move(st.items);
+	T.leaf_box.assign((size_t)T.n_leaves * 8, 0.0f);
+	for (const BNode &b : st.nodes) {
+		if (b.leaf < 0) continue;
+		float *o = &T.leaf_box[(size_t)b.leaf * 8];
+		const int begin = st.item_begin[b.leaf], cd = st.item_count[b.leaf] | (b.depth << 24);
+		o[0] = b.g0[0]; o[1] = b.g0[1]; o[2] = b.g0[2];
+		std::memcpy(&o[3], &begin, 4);
+		o[4] = b.g1[0]; o[5] = b.g1[1]; o[6] = b.g1[2];
+		std::memcpy(&o[7], &cd, 4);
+	}
+	// tabulate TreeNodeNum's descent (raycast.cpp:262-280)
+	T.gt = T.grid_num + 2;
+	const int gt = T.gt;
+	T.cell_leaf.assign((size_t)gt * gt * gt, -1);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+	for (int gx = 0; gx < gt; gx++)
+		for (int gy = 0; gy < gt; gy++)
+			for (int gz = 0; gz < gt; gz++) {
+				int g[3] = {gx, gy, gz}, node = 0, gsize = T.grid_num;
+				while (st.nodes[node].leaf < 0) {
+					gsize >>= 1;
+					int child = 0;
+					for (int k = 0; k < 3; k++)
+						if (g[k] >= gsize) {
+							child |= 1 << k;
+							g[k] -= gsize;
+						}
+					node = st.nodes[node].sub[child];
+				}
+				T.cell_leaf[((size_t)gx * gt + gy) * gt + gz] = st.nodes[node].leaf;
+			}
+	return T;
+}
+
+}  // namespace rgpu

@@ -1,0 +1,198 @@
+"""Python binding of the MI355X radiosity engine (libradgpu.so, C ABI in
+include/radiosity_gpu.h).
+
+Mirrors the reference's path (morris821028/hw-radiosity): load a ``.tri``
+scene (InitModelFromFile, rad.cpp:244-252), prepare (InitRadiosity +
+PrePartitionTriangles), shoot on the device (parallelRadiosity loop,
+rad.cpp:406-451), write ``.fin`` (PrintOut, rad.cpp:117-242).  The compute
+runs in the HIP library only; if the library is missing this raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "_bin", "libradgpu.so")
+RAD_BIN = os.path.join(PKG_DIR, "_bin", "rad")
+
+RAD_OK, RAD_EINVAL, RAD_EIO, RAD_ESTATE, RAD_EDEVICE = range(5)
+(CNT_RAYS, CNT_REF_RAYS, CNT_CELLS, CNT_TESTS, CNT_LEVELS, CNT_PRECISION,
+ CNT_STALLED, CNT_RECEIVERS) = range(8)
+COUNTER_NAMES = ("rays", "ref_rays", "cells", "tests", "levels", "precision", "stalled", "receivers")
+
+# The reference Triangle record (rad.h:25-39), 236 bytes.
+RECORD_DTYPE = np.dtype([
+    ("p", "<f4", (3, 3)), ("Frgb", "<i4", 3), ("Brgb", "<i4", 3),
+    ("accB", "<f4", (3, 3)), ("deltaaccB", "<f4", (3, 3)), ("deltaB", "<f4", 3),
+    ("se", "<f4", (3, 4)), ("c", "<f4", 3), ("n", "<f4", 3), ("area", "<f4"),
+    ("parent", "<i4"), ("neighbor", "<i4", 3)])
+assert RECORD_DTYPE.itemsize == 236
+# RayHitted's arguments (raycast.cpp:335).
+RAY_DTYPE = np.dtype([("p", "<f4", 3), ("v", "<f4", 3), ("ban", "<i4"), ("target", "<i4")])
+
+EXPORTS = ("rad_create", "rad_destroy", "rad_last_error", "rad_load_tri", "rad_set_triangles",
+           "rad_prepare_host", "rad_upload", "rad_shoot", "rad_run", "rad_write_fin", "rad_records",
+           "rad_tree_info", "rad_cast", "rad_ff_pairs", "rad_counters", "rad_timings")
+
+
+class RadError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("status %d: %s" % (status, msg))
+        self.status = status
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Load libradgpu.so (build it with ``make -C hw-radiosity_amd``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libradgpu.so is not built: run `make -C hw-radiosity_amd` (%s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p
+        sig = {
+            "rad_create": [ctypes.POINTER(vp), ip, ctypes.POINTER(cp)],
+            "rad_destroy": [vp],
+            "rad_load_tri": [vp, cp],
+            "rad_set_triangles": [vp, ip, vp, vp, vp],
+            "rad_prepare_host": [vp],
+            "rad_upload": [vp],
+            "rad_shoot": [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip)],
+            "rad_run": [vp, cp, ctypes.POINTER(ip)],
+            "rad_write_fin": [vp, cp],
+            "rad_records": [vp, ctypes.POINTER(ip), vp, vp],
+            "rad_tree_info": [vp, vp, ctypes.POINTER(ctypes.c_float)],
+            "rad_cast": [vp, ip, vp, vp, vp, vp],
+            "rad_ff_pairs": [vp, ip, vp, vp],
+            "rad_counters": [vp, vp],
+            "rad_timings": [vp, vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = None if name == "rad_destroy" else ip
+        L.rad_last_error.argtypes = [vp]
+        L.rad_last_error.restype = cp
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Engine:
+    """One engine context.  ``opts`` are rad options, e.g.
+    ``Engine("-triangle", "20000", "-converge", "1200")``."""
+
+    def __init__(self, *opts: str):
+        L = library()
+        argv = (ctypes.c_char_p * max(1, len(opts)))(*[o.encode() for o in opts])
+        h = ctypes.c_void_p()
+        st = L.rad_create(ctypes.byref(h), len(opts), argv)
+        if st != RAD_OK:
+            raise RadError(st, "bad options %r" % (opts,))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().rad_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _ok(self, st):
+        if st != RAD_OK:
+            raise RadError(st, library().rad_last_error(self.h).decode())
+
+    # ---- stages -------------------------------------------------------
+    def load_tri(self, path: str):
+        self._ok(library().rad_load_tri(self.h, path.encode()))
+        return self
+
+    def set_scene(self, scene):
+        p = np.ascontiguousarray(scene.p, np.float32)
+        f = np.ascontiguousarray(scene.frgb, np.int32)
+        b = np.ascontiguousarray(scene.brgb, np.int32)
+        self._ok(library().rad_set_triangles(self.h, int(p.shape[0]), _p(p), _p(f), _p(b)))
+        return self
+
+    def prepare_host(self):
+        self._ok(library().rad_prepare_host(self.h))
+        return self
+
+    def upload(self):
+        self._ok(library().rad_upload(self.h))
+        return self
+
+    def shoot(self, max_shots: int = 1 << 30):
+        shots, conv = ctypes.c_int(0), ctypes.c_int(0)
+        self._ok(library().rad_shoot(self.h, int(max_shots), ctypes.byref(shots), ctypes.byref(conv)))
+        return shots.value, bool(conv.value)
+
+    def run(self, prefix: str) -> int:
+        it = ctypes.c_int(0)
+        self._ok(library().rad_run(self.h, prefix.encode(), ctypes.byref(it)))
+        return it.value
+
+    def write_fin(self, path: str):
+        self._ok(library().rad_write_fin(self.h, path.encode()))
+
+    # ---- state / checks ------------------------------------------------
+    def records(self):
+        n = ctypes.c_int(0)
+        self._ok(library().rad_records(self.h, ctypes.byref(n), None, None))
+        rec = np.zeros(n.value, RECORD_DTYPE)
+        g = np.zeros(6, np.float32)
+        self._ok(library().rad_records(self.h, ctypes.byref(n), _p(rec), _p(g)))
+        return rec, g
+
+    def tree_info(self):
+        out = np.zeros(3, np.int32)
+        mgl = ctypes.c_float(0)
+        self._ok(library().rad_tree_info(self.h, _p(out), ctypes.byref(mgl)))
+        return int(out[0]), int(out[1]), int(out[2]), mgl.value
+
+    def cast(self, rays):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        n = len(rays)
+        hit, cells, tests = (np.zeros(n, np.int32) for _ in range(3))
+        self._ok(library().rad_cast(self.h, n, _p(rays), _p(hit), _p(cells), _p(tests)))
+        return hit, cells, tests
+
+    def ff_pairs(self, pairs):
+        pairs = np.ascontiguousarray(pairs, np.int32)
+        out = np.zeros((len(pairs), 3), np.float32)
+        self._ok(library().rad_ff_pairs(self.h, len(pairs), _p(pairs), _p(out)))
+        return out
+
+    def counters(self) -> dict:
+        out = np.zeros(8, np.int64)
+        self._ok(library().rad_counters(self.h, _p(out)))
+        return dict(zip(COUNTER_NAMES, (int(x) for x in out)))
+
+    def timings(self) -> dict:
+        out = np.zeros(6, np.float64)
+        self._ok(library().rad_timings(self.h, _p(out)))
+        return {"setup_ms": out[0], "upload_ms": out[1], "shoot_ms": out[2], "iterations_launched": int(out[3]),
+                "kernel_ms": out[4], "kernels_timed": int(out[5])}
+
+
+def run_cli(args, cwd=None, timeout=None):
+    """Run the drop-in ``rad`` binary; returns the CompletedProcess."""
+    if not os.path.exists(RAD_BIN):
+        raise ImportError("rad is not built: run `make -C hw-radiosity_amd`")
+    return subprocess.run([RAD_BIN] + list(args), cwd=cwd, capture_output=True, text=True, timeout=timeout)

@@ -1,0 +1,119 @@
+/* radiosity_gpu.h -- C ABI of the MI355X radiosity engine (libradgpu.so).
+ *
+ * Drop-in for the form-factor and shooting path of morris821028/hw-radiosity
+ * ("the reference").  The reference is one C++ program with no library API;
+ * its seams are the `rad` command line (src/rad.cpp:525-661) and the
+ * functions over its global TriStore (include/raycast.h:4-5,
+ * include/shade.h:4-5, src/rad.cpp).  Each entry point below names the
+ * reference code it replaces.  Plain pointers and sizes only; every call
+ * returns a status, and the library never exits the process.
+ *
+ * Threading: one context per host thread.  A context owns one HIP stream
+ * and its device buffers; host arrays belong to the caller.
+ */
+#ifndef RADIOSITY_GPU_H
+#define RADIOSITY_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rad_ctx rad_ctx;
+
+typedef enum {
+	RAD_OK = 0,
+	RAD_EINVAL = 1,   /* bad option or argument */
+	RAD_EIO = 2,      /* file open/read/write failure */
+	RAD_ESTATE = 3,   /* call out of order (e.g. shoot before prepare) */
+	RAD_EDEVICE = 4   /* HIP runtime error, or no GPU present */
+} rad_status;
+
+/* Counter indices for rad_counters(). */
+enum {
+	RAD_CNT_RAYS = 0,      /* rays traced on the device */
+	RAD_CNT_REF_RAYS = 1,  /* rays the reference casts for the same work (CalFF calls reaching RayHitted) */
+	RAD_CNT_CELLS = 2,     /* octree leaves visited */
+	RAD_CNT_TESTS = 3,     /* ray/triangle tests */
+	RAD_CNT_LEVELS = 4,    /* octree levels the reference descends for those leaves */
+	RAD_CNT_PRECISION = 5, /* "Precision Exceeded" steps (raycast.cpp:344-350) */
+	RAD_CNT_STALLED = 6,   /* rays the reference never finishes; their hit is -2 */
+	RAD_CNT_RECEIVERS = 7, /* receiver updates (Shade calls, shade.cpp:435) */
+	RAD_CNT_N = 8
+};
+
+/* Options: the reference's (rad.cpp:527-528, defaults rad.cpp:17-31):
+ *   -debug -adapt_area -sample_area -converge -delta_ff -write_cycle -zip
+ *   -triangle -light -o -interactive -jobs -parallel_src
+ * plus engine switches:
+ *   -fma 0|1          reproduce the reference compiler's FMA contractions (default 1)
+ *   -select high|low  tie-break on equal unshot energy: highest index (the
+ *                     shipped build, default) or lowest (MaxDeltaRad, rad.cpp:359-386)
+ *   -max_tri N        the reference pool size MaxTri (rad.h:10), default 1000000
+ *   -old_format 0|1   per-vertex output of the OLD_FORMAT build (rad.cpp:150-159)
+ *   -batch N          shooting iterations queued between host checks (default 32)
+ *   -kernel_timing 0|1  time every shoot kernel with HIP events (rad_timings)
+ *   -quiet 0|1        no log lines
+ * Replaces ProcessOption (rad.cpp:525-631) for everything but the input name. */
+rad_status rad_create(rad_ctx **out, int argc, const char *const *argv);
+void rad_destroy(rad_ctx *ctx);
+const char *rad_last_error(const rad_ctx *ctx);
+
+/* Input.  ReadTriangle + BuildTree (InitModelFromFile, rad.cpp:244-252,
+ * rad.cpp:47-111, raycast.cpp:216-233).  rad_set_triangles takes n
+ * triangles as p9 (x,y,z of 3 vertices), Frgb and Brgb triples. */
+rad_status rad_load_tri(rad_ctx *ctx, const char *path);
+rad_status rad_set_triangles(rad_ctx *ctx, int n, const float *p9, const int *frgb3, const int *brgb3);
+
+/* InitRadiosity + PrePartitionTriangles on the host (rad.cpp:259-356,
+ * shade.cpp:556-598).  Needs no GPU. */
+rad_status rad_prepare_host(rad_ctx *ctx);
+/* Copies the prepared scene and octree to the device and selects the first
+ * source.  Needs a GPU. */
+rad_status rad_upload(rad_ctx *ctx);
+
+/* Up to max_shots iterations of the shooting loop on the device
+ * (parallelRadiosity, rad.cpp:406-451, iterated by DoRadiosity
+ * rad.cpp:499-514).  *shots = iterations done, *converged = 1 when no
+ * candidate is left. */
+rad_status rad_shoot(rad_ctx *ctx, int max_shots, int *shots, int *converged);
+
+/* Whole solve as DoRadiosity (rad.cpp:491-523): prepare, upload, shoot,
+ * -write_cycle snapshots <prefix>.<loop>, final <prefix>.fin, log lines.
+ * *iterations = the logged "Complete in #Iterations" value (shots + 1). */
+rad_status rad_run(rad_ctx *ctx, const char *prefix, int *iterations);
+
+/* PrintOut (rad.cpp:117-242) of the current state to `path`. */
+rad_status rad_write_fin(rad_ctx *ctx, const char *path);
+
+/* State in the reference's 236-byte Triangle layout (rad.h:25-39), slots
+ * [0, *trinum); records may be NULL to query *trinum.  g6 (may be NULL)
+ * receives G0 and G1 (rad.cpp:105-110). */
+rad_status rad_records(rad_ctx *ctx, int *trinum, void *records, float *g6);
+
+/* Octree facts: out3 = {#nodes, #list entries, GridNum}, *min_grid_len =
+ * MinGridLen (raycast.cpp:175-232). */
+rad_status rad_tree_info(rad_ctx *ctx, int *out3, float *min_grid_len);
+
+/* RayHitted (raycast.cpp:335-360) for n rays, each {float p[3], float v[3],
+ * int ban, int target} (32 bytes), against the uploaded scene.  hit is the
+ * returned triangle, -1 for none, -2 where the reference never returns;
+ * cells/tests (may be NULL) count leaves visited and triangle tests. */
+rad_status rad_cast(rad_ctx *ctx, int n, const void *rays, int *hit, int *cells, int *tests);
+
+/* The per-vertex form factors Shade computes for receiver slot pairs[2i+1]
+ * from source slot pairs[2i] (AdaptCalFF(CalFF(...)), shade.cpp:442-450). */
+rad_status rad_ff_pairs(rad_ctx *ctx, int n, const int *pairs, float *ff3);
+
+/* Device counters since rad_upload (RAD_CNT_*). */
+rad_status rad_counters(rad_ctx *ctx, long long *out);
+
+/* Timings: out[0] host setup ms (load to prepare), out[1] upload ms,
+ * out[2] device ms of the shooting loop, out[3] shooting iterations
+ * launched, out[4] device ms of the fused shoot kernel alone and out[5] the
+ * number of such kernels timed (both 0 unless -kernel_timing 1). */
+rad_status rad_timings(rad_ctx *ctx, double *out6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -32,6 +32,7 @@
 namespace {
 
 const double kPi = 3.1415926535897932;  // rad.h:5
+const int kMaxWalk = 1 << 24;            // walk steps before a ray is reported stalled
 
 // One pool slot, byte-compatible with the reference Triangle (rad.h:25-39).
 struct Rec {
@@ -412,12 +413,18 @@ struct rs_ctx {
 	template <bool F> int ray_hit(const float *p, const float *v, int ban, long long *c) const
 	{
 		float t = 0.f, q[3] = {p[0], p[1], p[2]};
-		int prev = -1, cur;
+		int prev = -1, cur, steps = 0;
 		c[RS_CNT_RAYS]++;
 		while ((cur = cell_of(q, &c[RS_CNT_LEVELS])) >= 0) {
+			// The reference has no bound on this walk.  Report a ray instead
+			// of looping where the reference never returns: when t += 1.0
+			// no longer changes t, q repeats forever; kMaxWalk is a backstop.
+			if (++steps > kMaxWalk) { c[RS_CNT_CAPPED]++; return -2; }
 			if (prev == cur) {
 				c[RS_CNT_PRECISION]++;
-				t = (float)((double)t + 1.0);
+				float nt = (float)((double)t + 1.0);
+				if (nt == t) { c[RS_CNT_CAPPED]++; return -2; }
+				t = nt;
 				for (int k = 0; k < 3; k++) q[k] = axpy<F>(t, v[k], p[k]);
 				continue;
 			}

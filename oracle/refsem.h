@@ -27,6 +27,7 @@ enum {
 	RS_CNT_PRECISION = 4,   /* "Precision Exceeded" events (raycast.cpp:344-350) */
 	RS_CNT_SHADES = 5,      /* Shade calls */
 	RS_CNT_ITERS = 6,       /* shots done */
+	RS_CNT_CAPPED = 7,      /* rays the reference never finishes (hit = -2) */
 	RS_CNT_N = 8
 };
 

@@ -1,0 +1,171 @@
+"""Write the golden fixtures from the reference itself.
+
+Runs only where the reference is present (this needs /root/reference and the
+oracle/_ref/ builds from ``make -f oracle/ref.mk``).  The fixtures it writes
+are data: the inputs, hashes of the reference's full TriStore contents after
+each stage, .fin md5s, iteration counts, and per-ray / per-pair vectors.
+
+    python tests/golden/make_goldens.py
+
+Reference builds used: oracle/_ref/refdump (REF-FMA, shipped flags) and
+oracle/_ref/refdump_nofma (-ffp-contract=off); see oracle/ref.mk.
+"""
+from __future__ import annotations
+
+import gzip
+import hashlib
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF_MODELS = "/root/reference/model"
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "hw-radiosity_amd"))
+import oracle_lib as ol  # noqa: E402
+from radgpu import scenes  # noqa: E402
+
+C1_OPTS = ["-converge", "1200", "-delta_ff", "5e-6", "-adapt_area", "5", "-triangle", "20000"]
+# (case, input, options); every case runs under REF-FMA and REF-NOFMA
+CASES = [
+    ("box_t2000", "box.tri", ["-triangle", "2000"]),
+    ("box_t1000_l05_ps4", "box.tri", ["-triangle", "1000", "-light", "0.5", "-parallel_src", "4"]),
+    ("room_t5000", "room.tri", ["-triangle", "5000"]),
+    ("room_t20000", "room.tri", ["-triangle", "20000"]),
+    ("blocks_t3000", "blocks.tri", ["-triangle", "3000"]),
+    ("blocks_t3000_c100_ps3", "blocks.tri", ["-triangle", "3000", "-converge", "100", "-parallel_src", "3"]),
+    ("c1", "c1_seed1.tri", C1_OPTS),
+    ("c1_c100", "c1_seed1.tri", ["-converge", "100", "-triangle", "20000"]),
+]
+RAY_CASES = {"box_t2000", "room_t5000", "blocks_t3000", "c1"}
+N_RAYS = 6000
+N_PAIRS = 400
+
+
+def sha(*arrays) -> str:
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def refdump(variant, stage, out, opts, model, prefix):
+    exe = os.path.join(ROOT, "oracle", "_ref", "refdump" + ("" if variant == "fma" else "_nofma"))
+    jobs = ["-jobs", "1"]
+    r = subprocess.run([exe, stage, out] + opts + jobs + ["-write_cycle", "1000000", "-o", prefix, model],
+                       capture_output=True, text=True, check=True, timeout=600)
+    return r.stderr
+
+
+def make_rays(rec, g, rng):
+    """Rays as CalFF casts them -- receiver vertex -> source (half) centre,
+    ban = receiver's original, target = source's original -- plus random
+    rays through the grid, including near-axis and tiny directions."""
+    live = np.nonzero(rec["parent"] >= 0)[0]
+    n_real = N_RAYS * 2 // 3
+    src = rng.choice(live, n_real)
+    dst = rng.choice(live, n_real)
+    v = rng.integers(0, 3, n_real)
+    p = rec["p"][dst, v]
+    sp = rec["p"][src]
+    # PartitionSource's first half: midpoint of edge 0, then p2, p0
+    m = ((sp[:, 0] + sp[:, 1]) / np.float32(2)).astype(np.float32)
+    hc = (((m + sp[:, 2]) + sp[:, 0]) / np.float32(3)).astype(np.float32)
+    use_half = rng.random(n_real) < 0.5
+    c = np.where(use_half[:, None], hc, rec["c"][src])
+    rays = np.zeros(N_RAYS, ol.RAY_DTYPE)
+    rays["p"][:n_real] = p
+    rays["v"][:n_real] = (c - p).astype(np.float32)
+    rays["ban"][:n_real] = rec["parent"][dst]
+    rays["target"][:n_real] = rec["parent"][src]
+    k = N_RAYS - n_real
+    g0, g1 = g[:3].astype(np.float64), g[3:].astype(np.float64)
+    org = (g0 + (g1 - g0) * (0.1 + 0.8 * rng.random((k, 3)))).astype(np.float32)
+    # directions keep a length of at least 0.1: RayHitted's "Precision
+    # Exceeded" branch advances q by one |v| per step (raycast.cpp:344-350),
+    # so a tiny direction walks for millions of steps
+    d = rng.normal(size=(k, 3))
+    axis = rng.random(k) < 0.2
+    d[axis, :2] = 0.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d *= rng.choice([0.1, 1.0, 50.0], size=(k, 1))
+    rays["p"][n_real:] = org
+    rays["v"][n_real:] = d.astype(np.float32)
+    rays["ban"][n_real:] = rng.integers(-1, len(rec), k)
+    rays["target"][n_real:] = rng.integers(0, len(rec), k)
+    return rays
+
+
+def main():
+    if not os.path.isdir(REF_MODELS):
+        sys.exit("the reference is not present here")
+    os.makedirs(os.path.join(HERE, "models"), exist_ok=True)
+    inputs = {}
+    for m in ("box.tri", "room.tri", "blocks.tri"):
+        dst = os.path.join(HERE, "models", m + ".gz")
+        with open(os.path.join(REF_MODELS, m), "rb") as fi, gzip.GzipFile(dst, "wb", mtime=0) as fo:
+            shutil.copyfileobj(fi, fo)
+        inputs[m] = os.path.join(REF_MODELS, m)
+    tmp = tempfile.mkdtemp(prefix="goldens_")
+    c1 = os.path.join(tmp, "c1_seed1.tri")
+    scenes.write_tri(c1, scenes.room_scene(1))
+    inputs["c1_seed1.tri"] = c1
+    c1_sha = hashlib.sha256(open(c1, "rb").read()).hexdigest()
+
+    out = {"generator": {"c1_seed1.tri": c1_sha}, "cases": {}}
+    rng = np.random.default_rng(12345)
+    for case, model, opts in CASES:
+        for variant in ("fma", "nofma"):
+            key = "%s/%s" % (case, variant)
+            entry = {"input": model, "opts": opts, "variant": variant}
+            prefix = os.path.join(tmp, key.replace("/", "_"))
+            for stage in ("read", "init", "prepart", "final"):
+                dump = os.path.join(tmp, "dump.bin")
+                err = refdump(variant, stage, dump, opts, inputs[model], prefix)
+                rec, g = ol.read_dump(dump)
+                entry[stage] = {"trinum": int(len(rec)), "sha256": sha(rec, g)}
+                if stage == "read":
+                    entry["g"] = [float(x) for x in g]
+                    mt = re.search(r"#TreeNode = (\d+), #Trilist = (\d+)", err)
+                    entry["tree_nodes"], entry["tree_lists"] = int(mt.group(1)), int(mt.group(2))
+                if stage == "final":
+                    entry["iterations"] = int(re.search(r"Complete in #Iterations (\d+)", err).group(1))
+                    fin = open(prefix + ".fin", "rb").read()
+                    entry["fin_md5"] = hashlib.md5(fin).hexdigest()
+                    entry["fin_records"] = fin.count(b"Triangle")
+                    entry["live"] = int((rec["parent"] >= 0).sum())
+                    np.savez_compressed(os.path.join(HERE, "final_%s_%s.npz" % (case, variant)),
+                                        parent=rec["parent"], accB=rec["accB"], deltaB=rec["deltaB"])
+                if stage == "prepart" and case in RAY_CASES:
+                    rays = make_rays(rec, g, rng)
+                    rf = os.path.join(tmp, "rays.bin")
+                    rays.tofile(rf)
+                    hits_f = os.path.join(tmp, "hits.bin")
+                    refdump(variant, "rays:" + rf, hits_f, opts, inputs[model], prefix)
+                    hits = np.fromfile(hits_f, np.int32)
+                    live = np.nonzero(rec["parent"] >= 0)[0]
+                    pairs = np.stack([rng.choice(live, N_PAIRS), rng.choice(live, N_PAIRS)], 1).astype(np.int32)
+                    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+                    pf = os.path.join(tmp, "pairs.bin")
+                    pairs.tofile(pf)
+                    sf = os.path.join(tmp, "shade.bin")
+                    refdump(variant, "shade:" + pf, sf, opts, inputs[model], prefix)
+                    shade = np.fromfile(sf, np.float32).reshape(-1, 9)
+                    np.savez_compressed(os.path.join(HERE, "vectors_%s_%s.npz" % (case, variant)),
+                                        rays=rays.view(np.uint8), hits=hits, pairs=pairs, shade=shade)
+            out["cases"][key] = entry
+            print(key, entry["final"]["trinum"], entry["iterations"], entry["fin_md5"], flush=True)
+    with open(os.path.join(HERE, "goldens.json"), "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+    shutil.rmtree(tmp)
+
+
+if __name__ == "__main__":
+    main()

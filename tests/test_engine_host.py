@@ -1,0 +1,142 @@
+"""The engine's host stages (libradgpu.so, no GPU needed) against the
+reference goldens and the oracle: ReadTriangle-derived records, octree facts,
+InitRadiosity's neighbour links and pre-partition's slot contents must be
+byte-identical; the .fin writer's text must match the oracle's."""
+import hashlib
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import radgpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def sha(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def case_keys():
+    import json
+    with open(os.path.join(GOLDEN, "goldens.json")) as fh:
+        return sorted(json.load(fh)["cases"])
+
+
+def engine_for(entry, *extra):
+    return radgpu.Engine(*(entry["opts"] + ["-fma", "1" if entry["variant"] == "fma" else "0", "-quiet", "1"]
+                           + list(extra)))
+
+
+@pytest.mark.parametrize("key", case_keys())
+def test_host_stages_match_reference(key, goldens, inputs):
+    e = goldens["cases"][key]
+    with engine_for(e) as eng:
+        eng.load_tri(inputs[e["input"]])
+        nn, nl, _, _ = eng.tree_info()
+        assert (nn, nl) == (e["tree_nodes"], e["tree_lists"])
+        assert sha(*eng.records()) == e["read"]["sha256"]
+        eng.prepare_host()
+        rec, g = eng.records()
+        assert len(rec) == e["prepart"]["trinum"]
+        assert sha(rec, g) == e["prepart"]["sha256"]
+
+
+def test_set_triangles_equals_file(inputs):
+    from radgpu import scenes
+    s = scenes.read_tri(inputs["c1_seed1.tri"])
+    with radgpu.Engine("-quiet", "1") as a, radgpu.Engine("-quiet", "1") as b:
+        a.load_tri(inputs["c1_seed1.tri"])
+        b.set_scene(s)
+        assert a.records()[0].tobytes() == b.records()[0].tobytes()
+
+
+def test_tree_matches_oracle(inputs):
+    for m in ("room.tri", "blocks.tri", "c1_seed1.tri"):
+        o = ol.Oracle("-quiet", "1")
+        o.load_file(inputs[m])
+        o.build_tree()
+        with radgpu.Engine("-quiet", "1") as eng:
+            eng.load_tri(inputs[m])
+            a, b = eng.tree_info(), o.tree_info()
+            assert a[:3] == b[:3] and struct.pack("f", a[3]) == struct.pack("f", b[3]), m
+
+
+def test_fin_text_matches_oracle(inputs, tmp_path):
+    """After pre-partition (lights carry accB), both writers print the same bytes."""
+    for m, opts in (("room.tri", ["-triangle", "4000"]), ("c1_seed1.tri", ["-triangle", "20000"])):
+        o = ol.Oracle(*(opts + ["-quiet", "1"]))
+        o.load_file(inputs[m])
+        o.build_tree()
+        o.init_radiosity()
+        o.prepartition()
+        o.write_fin(str(tmp_path / "o.fin"))
+        with radgpu.Engine(*(opts + ["-quiet", "1"])) as eng:
+            eng.load_tri(inputs[m]).prepare_host()
+            eng.write_fin(str(tmp_path / "e.fin"))
+        assert open(tmp_path / "o.fin", "rb").read() == open(tmp_path / "e.fin", "rb").read(), m
+
+
+def _ref_format(x):
+    """fp::tostring<float> (float2string.h:103-271) restated from its definition:
+    the exact value of x truncated to 8 fraction digits, trailing zeros removed."""
+    bits = struct.unpack("<I", struct.pack("<f", x))[0]
+    e = (bits >> 23) & 0xFF
+    if e == 0xFF or (e == 0 and bits & 0x7FFFFF):
+        return None
+    sign = "-" if bits >> 31 else ""
+    if e == 0:
+        return sign + "0.0"
+    ex = e - 150
+    if ex >= 41:
+        return None
+    from fractions import Fraction
+    v = Fraction((bits & 0x7FFFFF) | 0x800000) * (Fraction(2) ** ex)
+    ip = int(v)
+    if ex >= 0 or ex < -60:
+        frac = ""
+    else:
+        digits = int((v - ip) * 10 ** 8)
+        frac = ("%08d" % digits).rstrip("0")
+    return "%s%d.%s" % (sign, ip, frac or "0")
+
+
+def test_number_format_known_answers(tmp_path):
+    """The engine's formatter, through a written .fin, against the definition
+    and hand-checked values."""
+    from radgpu import scenes
+    assert _ref_format(1.0) == "1.0"
+    assert _ref_format(-0.0) == "-0.0"
+    assert _ref_format(0.1) == "0.1"            # 0.100000001490116...
+    assert _ref_format(1.0 / 3.0) == "0.33333334"  # 0.3333333432674408
+    assert _ref_format(2.0 ** -30) == "0.0"      # below 1e-8 after truncation
+    assert _ref_format(123456.789) == "123456.78906250"[:-1].rstrip("0")
+    rng = np.random.default_rng(3)
+    vals = np.concatenate([rng.normal(size=300) * 10.0 ** rng.integers(-9, 12, 300),
+                           [0.0, -0.0, 1e-45, 3e38, 1e-7, 5e-9, 16777217.0, -2.5]]).astype(np.float32)
+    n = len(vals) // 9 * 9
+    p = vals[:n].reshape(-1, 3, 3)
+    s = scenes.Scene(p, np.full((len(p), 3), 100, np.int32), np.zeros((len(p), 3), np.int32))
+    path = str(tmp_path / "v.tri")
+    scenes.write_tri(path, s)
+    with radgpu.Engine("-quiet", "1") as eng:
+        eng.load_tri(path)
+        eng.write_fin(str(tmp_path / "v.fin"))
+    lines = open(tmp_path / "v.fin").read().split("\n")
+    got = []
+    for k in range(len(p)):
+        for v in range(3):
+            got.extend(lines[5 * k + 2 + v].split()[:3])
+    last = ""
+    for x, txt in zip(p.reshape(-1), got):
+        want = _ref_format(float(x))
+        if want is None:  # the reference reprints the previous text
+            want = last
+        assert txt == want, (float(x), txt, want)
+        last = want

@@ -1,0 +1,217 @@
+"""GPU parity: the HIP path through the C ABI against the reference's goldens
+and the oracle.  Integer/index results (hits, iteration counts, topology) are
+bit-exact; floating-point state (accB, deltaB, deltaaccB) is also held to
+bit-exactness because the engine reproduces the reference's operation order
+and contraction sites -- relative L2 is reported as a diagnostic only."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import radgpu
+from radgpu import scenes
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def sha(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def case_keys():
+    import json
+    with open(os.path.join(GOLDEN, "goldens.json")) as fh:
+        return sorted(json.load(fh)["cases"])
+
+
+def opts_of(e, *extra):
+    return e["opts"] + ["-fma", "1" if e["variant"] == "fma" else "0", "-quiet", "1"] + list(extra)
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def describe_diff(rec, ref):
+    out = []
+    for f in ref.dtype.names:
+        if rec[f].tobytes() != ref[f].tobytes():
+            bad = np.nonzero((rec[f] != ref[f]).reshape(len(ref), -1).any(1))[0]
+            out.append("%s: %d slots (first %s)" % (f, len(bad), bad[:4]))
+    return "; ".join(out) + " | accB rel-L2 %.3g" % rel_l2(rec["accB"], ref["accB"])
+
+
+@pytest.mark.parametrize("key", [k for k in case_keys()
+                                 if os.path.exists(os.path.join(GOLDEN, "vectors_%s.npz" % k.replace("/", "_")))])
+def test_cast_and_ff_match_reference(key, goldens, inputs):
+    print("case", key, flush=True)
+    e = goldens["cases"][key]
+    v = np.load(os.path.join(GOLDEN, "vectors_%s.npz" % key.replace("/", "_")))
+    rays = v["rays"].view(radgpu.RAY_DTYPE)
+    with radgpu.Engine(*opts_of(e)) as eng:
+        eng.load_tri(inputs[e["input"]]).prepare_host().upload()
+        hit, cells, tests = eng.cast(rays)
+        ff = eng.ff_pairs(v["pairs"])
+    assert np.array_equal(hit, v["hits"]), "hits differ on %d rays" % int((hit != v["hits"]).sum())
+    o = ol.Oracle(*opts_of(e))
+    o.load_file(inputs[e["input"]])
+    o.build_tree()
+    o.init_radiosity()
+    o.prepartition()
+    ohit, ocells, otests = o.raycast(rays.view(ol.RAY_DTYPE))
+    assert np.array_equal(cells, ocells) and np.array_equal(tests, otests)
+    assert ff.tobytes() == o.ff_pairs(v["pairs"]).tobytes()
+    # Shade's increments from the reference: lo = 255*k*dB/3, d = lo*ff
+    k = np.float32(0.5 / 255.0)
+    lo = (np.float32(255.0) * k * np.array([1000, 2000, 3000], np.float32)) / np.float32(3)
+    d = (lo[None, None, :] * ff[:, :, None]).astype(np.float32).reshape(-1, 9)
+    assert d.tobytes() == v["shade"].astype(np.float32).tobytes()
+
+
+@pytest.mark.parametrize("key", case_keys())
+def test_full_solve_matches_reference(key, goldens, inputs, tmp_path):
+    print("case", key, flush=True)
+    e = goldens["cases"][key]
+    with radgpu.Engine(*opts_of(e, "-write_cycle", "1000000")) as eng:
+        eng.load_tri(inputs[e["input"]])
+        it = eng.run(str(tmp_path / "gpu"))
+        rec, g = eng.records()
+    ref = np.load(os.path.join(GOLDEN, "final_%s.npz" % key.replace("/", "_")))
+    assert it == e["iterations"]
+    assert len(rec) == e["final"]["trinum"]
+    assert np.array_equal(rec["parent"], ref["parent"])
+    assert rec["accB"].tobytes() == ref["accB"].tobytes(), "accB rel-L2 %.3g" % rel_l2(rec["accB"], ref["accB"])
+    assert rec["deltaB"].tobytes() == ref["deltaB"].tobytes()
+    assert sha(rec, g) == e["final"]["sha256"]
+    assert hashlib.md5(open(tmp_path / "gpu.fin", "rb").read()).hexdigest() == e["fin_md5"]
+
+
+def test_cli_drop_in(goldens, inputs, tmp_path):
+    e = goldens["cases"]["c1/fma"]
+    r = radgpu.run_cli(e["opts"] + ["-write_cycle", "10", inputs["c1_seed1.tri"], "-o", str(tmp_path / "c1")],
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "Complete in #Iterations %d" % e["iterations"] in r.stderr
+    assert hashlib.md5(open(tmp_path / "c1.fin", "rb").read()).hexdigest() == e["fin_md5"]
+    # snapshots every 10 iterations, as -write_cycle asks
+    assert sorted(p.name for p in tmp_path.iterdir() if p.name.startswith("c1.") and p.name != "c1.fin") == \
+        ["c1.10", "c1.20"]
+
+
+@pytest.mark.parametrize("select", ["high", "low"])
+def test_hall_subset_matches_oracle(select):
+    """The hall generator (yawed boxes, pre-partition at scale) with 40 boxes:
+    the full solve is bit-identical to the oracle, with either tie-break."""
+    s = scenes.hall_scene(2, n_boxes=40)
+    opts = ["-converge", "1200", "-triangle", "1000000", "-select", select, "-quiet", "1"]
+    print("hall40 select", select, s.n, flush=True)
+    with radgpu.Engine(*opts) as eng:
+        eng.set_scene(s).prepare_host().upload()
+        shots, conv = eng.shoot()
+        rec, _ = eng.records()
+    o = ol.Oracle(*(opts + ["-jobs", "16"]))
+    o.load(s)
+    o.build_tree()
+    o.init_radiosity()
+    o.prepartition()
+    oshots, oconv = o.iterate(1 << 30)
+    orec, _ = o.dump()
+    assert conv and oconv and shots == oshots
+    assert rec.tobytes() == orec.tobytes(), describe_diff(rec, orec)
+
+
+def test_hall_full_first_shots_match_oracle():
+    """The 200k-triangle hall: the first 3 shots bit-identical to the oracle,
+    and random rays through the full octree identical."""
+    s = scenes.hall_scene(2)
+    opts = ["-converge", "1200", "-triangle", "1000000", "-quiet", "1"]
+    print("hall full", s.n, flush=True)
+    with radgpu.Engine(*opts) as eng:
+        eng.set_scene(s).prepare_host().upload()
+        rec0, g = eng.records()
+        rng = np.random.default_rng(5)
+        live = np.nonzero(rec0["parent"] >= 0)[0]
+        n = 200000
+        src, dst, v = rng.choice(live, n), rng.choice(live, n), rng.integers(0, 3, n)
+        rays = np.zeros(n, radgpu.RAY_DTYPE)
+        rays["p"] = rec0["p"][dst, v]
+        rays["v"] = rec0["c"][src] - rec0["p"][dst, v]
+        rays["ban"] = rec0["parent"][dst]
+        rays["target"] = rec0["parent"][src]
+        hit, cells, tests = eng.cast(rays)
+        shots, _ = eng.shoot(3)
+        rec, _ = eng.records()
+    o = ol.Oracle(*(opts + ["-jobs", "16"]))
+    o.load(s)
+    o.build_tree()
+    o.init_radiosity()
+    o.prepartition()
+    orec0, _ = o.dump()
+    assert rec0.tobytes() == orec0.tobytes()
+    ohit, ocells, otests = o.raycast(rays.view(ol.RAY_DTYPE))
+    assert np.array_equal(hit, ohit) and np.array_equal(cells, ocells) and np.array_equal(tests, otests)
+    o.iterate(3)
+    orec, _ = o.dump()
+    assert shots == 3
+    assert rec.tobytes() == orec.tobytes(), describe_diff(rec, orec)
+
+
+def test_repeat_runs_are_bitwise_equal():
+    s = scenes.room_scene(3)
+    outs = []
+    for _ in range(2):
+        with radgpu.Engine("-triangle", "20000", "-converge", "300", "-quiet", "1") as eng:
+            eng.set_scene(s).prepare_host().upload()
+            eng.shoot()
+            outs.append(eng.records()[0].tobytes())
+    assert outs[0] == outs[1]
+
+
+def test_shoot_in_pieces_equals_one_call():
+    s = scenes.room_scene(4)
+    opts = ("-triangle", "20000", "-converge", "400", "-quiet", "1")
+    with radgpu.Engine(*opts) as a, radgpu.Engine(*(opts + ("-batch", "3"))) as b:
+        a.set_scene(s).prepare_host().upload()
+        b.set_scene(s).prepare_host().upload()
+        na, _ = a.shoot()
+        nb = 0
+        while True:
+            k, conv = b.shoot(7)
+            nb += k
+            if conv:
+                break
+        assert na == nb
+        assert a.records()[0].tobytes() == b.records()[0].tobytes()
+
+
+def test_device_float_ops_are_ieee():
+    """Correctly rounded division and square root and IEEE NaN handling in
+    the walk: rays built to give a1 = 0 (NaN and inf hit parameters)."""
+    s = scenes.room_scene(1)
+    with radgpu.Engine("-quiet", "1", "-triangle", "20000") as eng:
+        eng.set_scene(s).prepare_host().upload()
+        rec, g = eng.records()
+        rng = np.random.default_rng(9)
+        n = 50000
+        rays = np.zeros(n, radgpu.RAY_DTYPE)
+        rays["p"] = (g[:3] + (g[3:] - g[:3]) * rng.random((n, 3))).astype(np.float32)
+        d = rng.normal(size=(n, 3)).astype(np.float32)
+        d[: n // 4, 2] = 0.0   # parallel to floor/ceiling planes
+        d[n // 4: n // 2, :2] = 0.0
+        rays["v"] = d
+        rays["ban"] = -1
+        hit, cells, tests = eng.cast(rays)
+    o = ol.Oracle("-quiet", "1", "-triangle", "20000")
+    o.load(s)
+    o.build_tree()
+    o.init_radiosity()
+    o.prepartition()
+    ohit, ocells, otests = o.raycast(rays.view(ol.RAY_DTYPE))
+    assert np.array_equal(hit, ohit) and np.array_equal(cells, ocells) and np.array_equal(tests, otests)
