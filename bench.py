@@ -35,14 +35,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(cnt, slots, launches):
-    """Bytes the shoot kernel must read/write per the path's definition:
-    per ray 40 (origin, source patch) + 32 per leaf (box, list offset) + 8 per
-    descent level + 76 per triangle test (n, c, 3 side planes, list entry);
-    196 per receiver update (read 112, write 84); 16 per slot per launch for
-    the candidate reduction."""
-    return (40 * cnt["rays"] + 32 * cnt["cells"] + 8 * cnt["levels"] + 76 * cnt["tests"]
-            + 196 * cnt["receivers"] + 16 * slots * launches)
+def algorithmic_bytes(cnt):
+    """Bytes the ray-tracing kernel must read per the path's definition: per
+    ray 40 (origin, source patch) + 32 per leaf (box, list offset) + 8 per
+    descent level + 76 per triangle test (n, c, 3 side planes, list entry)."""
+    return 40 * cnt["rays"] + 32 * cnt["cells"] + 8 * cnt["levels"] + 76 * cnt["tests"]
 
 
 def cpu_baseline(scene, seconds, cores):
@@ -122,7 +119,7 @@ def main():
         log("step %d: %d shots, %.3f s, %d rays" % (k, shots, dt, c["rays"]))
     rays, ref_rays = cnt_sum["rays"], cnt_sum["ref_rays"]
     kernel_avg_ms = kms / max(kn, 1)
-    bytes_per_launch = algorithmic_bytes(cnt_sum, slots, kn) / max(kn, 1)
+    bytes_per_launch = algorithmic_bytes(cnt_sum) / max(kn, 1)
     achieved = bytes_per_launch / (kernel_avg_ms * 1e-3) / 1e9 if kernel_avg_ms > 0 else 0.0
     value = rays / total_s
     out = {
@@ -144,7 +141,7 @@ def main():
         "ref_equivalent_rays_per_s": ref_rays / total_s,
         "time_to_converge_s": {"device_shooting": total_s / args.steps, "end_to_end_init_to_fin": e2e_s,
                                "host_setup_ms": e2e_t["setup_ms"], "upload_ms": e2e_t["upload_ms"]},
-        "roofline": {"kernel": "k_shoot (fused visibility + form factor + update + next-source reduction)",
+        "roofline": {"kernel": "k_trace (octree-DDA visibility of the iteration's rays)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_kernel_ms": kernel_avg_ms, "launches": kn,

@@ -256,15 +256,16 @@ rad_status rad_counters(rad_ctx *c, long long *out)
 	return RAD_OK;
 }
 
-rad_status rad_timings(rad_ctx *c, double *out6)
+rad_status rad_timings(rad_ctx *c, double *out7)
 {
-	if (!c || !out6) return RAD_EINVAL;
-	out6[0] = c->ms_setup;
-	out6[1] = c->ms_upload;
-	out6[2] = c->dev ? c->dev->shoot_ms() : 0;
-	out6[3] = c->dev ? (double)c->dev->kernel_launches() : 0;
-	out6[4] = c->dev ? c->dev->kernel_ms() : 0;
-	out6[5] = c->dev ? (double)c->dev->kernel_timed() : 0;
+	if (!c || !out7) return RAD_EINVAL;
+	out7[6] = c->dev ? c->dev->cast_ms() : 0;
+	out7[0] = c->ms_setup;
+	out7[1] = c->ms_upload;
+	out7[2] = c->dev ? c->dev->shoot_ms() : 0;
+	out7[3] = c->dev ? (double)c->dev->kernel_launches() : 0;
+	out7[4] = c->dev ? c->dev->kernel_ms() : 0;
+	out7[5] = c->dev ? (double)c->dev->kernel_timed() : 0;
 	return RAD_OK;
 }
 

@@ -47,13 +47,14 @@ public:
 	double kernel_ms() const { return kernel_ms_; }  // device time of the fused shoot kernel alone
 	long long kernel_launches() const { return kernel_launches_; }
 	long long kernel_timed() const { return kernel_timed_; }
+	double cast_ms() const { return cast_ms_; }  // device time of the last cast kernel
 	bool timing = false;  // time each shoot kernel with events (for the roofline)
 
 private:
 	DeviceEngine() = default;
 	struct Impl;
 	Impl *d_ = nullptr;
-	double shoot_ms_ = 0, kernel_ms_ = 0;
+	double shoot_ms_ = 0, kernel_ms_ = 0, cast_ms_ = 0;
 	long long kernel_launches_ = 0, kernel_timed_ = 0;
 };
 

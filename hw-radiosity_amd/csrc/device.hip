@@ -1,33 +1,49 @@
 // Device kernels and buffers of the shooting loop (gfx950).
 //
 // Layout in HBM
-//   originals (octree hit tests), AoS float4:   tri_nc[2*i] = {n.xyz, c.x},
-//       tri_nc[2*i+1] = {c.y, c.z, -, -}, tri_se[3*i+s] = side plane s
 //   octree: cell_leaf[gt^3] (TreeNodeNum tabulated), leaf_box[2*l] =
-//       {g0.xyz, first item}, {g1.xyz, count | depth<<24}, items[]
+//       {g0.xyz, first item}, {g1.xyz, count | depth<<24}
+//   leaf lists, leaf-major with the triangle inline (the list order is the
+//       reference's ascending original index): lnc[2*k] = {n.xyz, c.x},
+//       lnc[2*k+1] = {c.y, c.z, original id, -}, lse[3*k+s] = side plane s.
+//       The contents are the originals' slots after pre-partition, which is
+//       what RayHitted tests (raycast.cpp:329).
 //   receivers (every pool slot), SoA by vertex: vx/vy/vz[3*r+v],
 //       acc/dacc[3*r+v] = accB/deltaaccB of vertex v (float4),
 //       nrm[r] = {n.xyz, parent}, cA[r] = {c.xyz, area}, rho[r] = Frgb as
 //       floats, dB[r] = deltaB
 //
-// One shooting iteration (rad.cpp:406-451) is two launches: k_prep (one
-// lane) turns the best candidate key into a source descriptor (whole
-// source and its two halves, AdaptCalFF/PartitionSource shade.cpp:141-168,
-// 76-99) and clears that source's deltaB; k_shoot gives each (receiver,
-// vertex) a lane, 21 receivers per 64-lane wave, casts the vertex's rays
-// (CalFF shade.cpp:107-127, RayHitted raycast.cpp:335-360), applies
-// CalRadiosity (shade.cpp:244-259) with the three vertices combined by
-// lane shuffles in the reference's order, and reduces the next iteration's
-// candidate key (listCanditate rad.cpp:387-404) by wave and block maxima
-// and one atomicMax per block.  The host only polls a done flag every few
-// dozen iterations.
+// One shooting iteration (rad.cpp:406-451) is four launches on one stream:
+//   k_prep   (one lane) turns the best candidate key into a source descriptor
+//            -- the whole source and the halves AdaptCalFF shoots from
+//            (shade.cpp:141-168, 76-99) -- and clears the source's deltaB;
+//   k_gen    one lane per (receiver, vertex), 168 receivers per block:
+//            CalFF up to the ray (shade.cpp:107-123) for each half, into a
+//            per-(receiver, vertex, half) form-factor buffer;
+//   k_trace  one lane per (receiver, vertex, half) in slot order: the rays
+//            to cast (RayHitted, raycast.cpp:335-360); a ray that misses the
+//            source zeroes its form factor;
+//   k_update CalRadiosity (shade.cpp:244-259), the three vertex lanes of a
+//            receiver combined by shuffles in the reference's order, and the
+//            next iteration's candidate key (listCanditate, rad.cpp:387-404)
+//            by wave, block and one 64-bit atomicMax per block.
+// With k sources per iteration (-parallel_src) gen/trace/update repeat per
+// source in the reference's order.  The host polls a done flag every -batch
+// iterations.
 //
 // Every float operation is IEEE single or double in source order
 // (-ffp-contract=off, correctly rounded division and square root); with
 // F=true an explicit fma sits at each of the reference compiler's
-// contraction sites, so results are bit-identical to the reference.
+// contraction sites, so results are bit-identical to the reference.  Three
+// shortcuts are exact by construction: a triangle test is rejected from
+// a2 * rcp(a1) only when the exact quotient is certainly on the rejected
+// side (a 2^-19 relative guard band around a <= 2^-21 error); TreeNodeNum's
+// double divide is a multiply by 1/MinGridLen unless the product is within
+// 1e-9 of a cell edge; and the double compares against 1e-4 and 1e-6 are
+// float compares against the nearest floats on the right side.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -41,19 +57,26 @@ constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
+constexpr int kGroups = 2;        // receiver groups per block
+constexpr int kRecvPerBlock = kGroups * kWavesPerBlock * kRecvPerWave;
 constexpr int kMaxSrc = 256;
 constexpr int kMaxWalk = 1 << 24;
 constexpr double kPiD = 3.1415926535897932;
+constexpr float kGuardUp = 1.0f + 0x1p-19f;
+constexpr float kGuardDown = 1.0f - 0x1p-19f;
+
 
 struct DScene {
-	const float4 *__restrict__ tri_nc;
-	const float4 *__restrict__ tri_se;
+	const float4 *__restrict__ lnc;
+	const float4 *__restrict__ lse;
 	const int *__restrict__ cell_leaf;
 	const float4 *__restrict__ leaf_box;
-	const int *__restrict__ items;
 	float g0x, g0y, g0z, g1x, g1y, g1z;
-	double mgl;
+	double mgl, inv_mgl;
 	int gt;
+	float thr_t;     // smallest float f with (double)f >= 1e-4
+	float thr_t_lo;  // thr_t * (1 - 2^-19)
+	float thr_v;     // largest float f with (double)f <= 1e-6
 };
 
 struct DRecv {
@@ -136,22 +159,32 @@ __device__ __forceinline__ int wave_sum(int x)
 	for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
 	return x;
 }
+// (int)(((double)q - G0) / MinGridLen) of TreeNodeNum (raycast.cpp:263)
+__device__ __forceinline__ int cell_coord(float q, float g0, const DScene &S)
+{
+	const double x = (double)q - (double)g0;
+	const double qd = x * S.inv_mgl;
+	const double fl = floor(qd);
+	const double fr = qd - fl;
+	if (fr > 1e-9 && fr < 1.0 - 1e-9) return (int)fl;
+	return (int)(x / S.mgl);
+}
 
 // ---- RayHitted ------------------------------------------------------------
 // The walk of raycast.cpp:335-360 over the tabulated octree: find the leaf
-// of q, test its list (TriListHitted/TriHitted, raycast.cpp:283-333) and
-// return the nearest hit of the first leaf with any hit; else step q to
-// just past the leaf's nearest exit plane (NextPoint, raycast.cpp:235-251).
+// of q (TreeNodeNum), test its list (TriListHitted/TriHitted,
+// raycast.cpp:283-333) and return the nearest hit of the first leaf with any
+// hit; else step q to just past the leaf's nearest exit plane (NextPoint,
+// raycast.cpp:235-251).
 template <bool F>
 __device__ int walk(const DScene &S, float px, float py, float pz, float vx, float vy, float vz, int ban, Cnt &c)
 {
 	float t = 0.f, qx = px, qy = py, qz = pz;
 	int prev = -1;
+	const bool use_x = fabsf(vx) > S.thr_v, use_y = fabsf(vy) > S.thr_v, use_z = fabsf(vz) > S.thr_v;
 	for (int steps = 0;;) {
 		if (qx < S.g0x || qx > S.g1x || qy < S.g0y || qy > S.g1y || qz < S.g0z || qz > S.g1z) return -1;
-		const int gx = (int)(((double)qx - (double)S.g0x) / S.mgl);
-		const int gy = (int)(((double)qy - (double)S.g0y) / S.mgl);
-		const int gz = (int)(((double)qz - (double)S.g0z) / S.mgl);
+		const int gx = cell_coord(qx, S.g0x, S), gy = cell_coord(qy, S.g0y, S), gz = cell_coord(qz, S.g0z, S);
 		if ((unsigned)gx >= (unsigned)S.gt || (unsigned)gy >= (unsigned)S.gt || (unsigned)gz >= (unsigned)S.gt) {
 			c.stalled++;  // outside the tabulated descent: cannot happen for q inside [G0, G1]
 			return -2;
@@ -172,40 +205,67 @@ __device__ int walk(const DScene &S, float px, float py, float pz, float vx, flo
 		const int begin = __float_as_int(b0.w), cd = __float_as_int(b1.w), cnt = cd & 0xffffff;
 		c.cells++;
 		c.levels += cd >> 24;
-		float tmin = 1000000.0f;
+		float tmin = 1000000.0f, tmin_hi = 1000000.0f * kGuardUp;
 		int best = -1;
+		const float4 *nc = S.lnc + 2 * (size_t)begin;
+		float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+		if (cnt > 0) {
+			a = nc[0];
+			b = nc[1];
+		}
 		for (int e = 0; e < cnt; e++) {
-			const int id = S.items[begin + e];
-			if (id == ban) continue;
-			c.tests++;
-			const float4 a = S.tri_nc[2 * id], b = S.tri_nc[2 * id + 1];
-			const float a1 = dot3<F>(vx, vy, vz, a.x, a.y, a.z);
-			const float a2 = dot3<F>(a.w - px, b.x - py, b.y - pz, a.x, a.y, a.z);
-			const float tt = a2 / a1;
-			if ((double)tt < 0.0001) continue;
-			if (tt >= tmin) continue;
-			const float hx = axpy<F>(tt, vx, px), hy = axpy<F>(tt, vy, py), hz = axpy<F>(tt, vz, pz);
-			const float4 s0 = S.tri_se[3 * id];
-			if (dot3<F>(hx, hy, hz, s0.x, s0.y, s0.z) + s0.w < 0.0f) continue;
-			const float4 s1 = S.tri_se[3 * id + 1];
-			if (dot3<F>(hx, hy, hz, s1.x, s1.y, s1.z) + s1.w < 0.0f) continue;
-			const float4 s2 = S.tri_se[3 * id + 2];
-			if (dot3<F>(hx, hy, hz, s2.x, s2.y, s2.z) + s2.w < 0.0f) continue;
-			tmin = tt;
-			best = id;
+			float4 an = a, bn = b;
+			if (e + 1 < cnt) {  // prefetch the next list entry
+				an = nc[2 * e + 2];
+				bn = nc[2 * e + 3];
+			}
+			const int id = __float_as_int(b.z);
+			if (id != ban) {
+				c.tests++;
+				const float a1 = dot3<F>(vx, vy, vz, a.x, a.y, a.z);
+				const float a2 = dot3<F>(a.w - px, b.x - py, b.y - pz, a.x, a.y, a.z);
+				bool undecided = true;
+				const float fa1 = fabsf(a1);
+				if (fa1 >= 0x1p-100f && fa1 <= 0x1p100f && fabsf(a2) <= 0x1p100f) {
+					const float ta = a2 * __builtin_amdgcn_rcpf(a1);
+					if (ta < S.thr_t_lo || ta > tmin_hi) undecided = false;
+				}
+				if (undecided) {
+					const float tt = a2 / a1;
+					// TriHitted: t < 0.0001 (in double) or t >= t_min rejects
+					if (!(tt < S.thr_t) && !(tt >= tmin)) {
+						const float hx = axpy<F>(tt, vx, px), hy = axpy<F>(tt, vy, py), hz = axpy<F>(tt, vz, pz);
+						const float4 *se = S.lse + 3 * ((size_t)begin + e);
+						const float4 s0 = se[0];
+						if (!(dot3<F>(hx, hy, hz, s0.x, s0.y, s0.z) + s0.w < 0.0f)) {
+							const float4 s1 = se[1];
+							if (!(dot3<F>(hx, hy, hz, s1.x, s1.y, s1.z) + s1.w < 0.0f)) {
+								const float4 s2 = se[2];
+								if (!(dot3<F>(hx, hy, hz, s2.x, s2.y, s2.z) + s2.w < 0.0f)) {
+									tmin = tt;
+									tmin_hi = tt * kGuardUp;
+									best = id;
+								}
+							}
+						}
+					}
+				}
+			}
+			a = an;
+			b = bn;
 		}
 		if (best >= 0) return best;
 		prev = leaf;
 		float tm = 1000000.0f;
-		if ((double)fabsf(vx) > 1e-6) {
+		if (use_x) {
 			const float s = vx > 0 ? (b1.x - px) / vx : (b0.x - px) / vx;
 			tm = s < tm ? s : tm;
 		}
-		if ((double)fabsf(vy) > 1e-6) {
+		if (use_y) {
 			const float s = vy > 0 ? (b1.y - py) / vy : (b0.y - py) / vy;
 			tm = s < tm ? s : tm;
 		}
-		if ((double)fabsf(vz) > 1e-6) {
+		if (use_z) {
 			const float s = vz > 0 ? (b1.z - pz) / vz : (b0.z - pz) / vz;
 			tm = s < tm ? s : tm;
 		}
@@ -217,12 +277,11 @@ __device__ int walk(const DScene &S, float px, float py, float pz, float vx, flo
 	}
 }
 
-// CalFF (shade.cpp:107-127): point-to-patch form factor times visibility.
-// cast=false evaluates the prefix only and reports whether the reference
-// would cast a ray (for the reference-equivalent ray count).
+// CalFF (shade.cpp:107-123) up to the ray: the point-to-patch form factor,
+// or 0 where the reference returns 0 before casting.
 template <bool F>
-__device__ float point_ff(const DScene &S, const SrcPatch &s, int logsrc, float nx, float ny, float nz, float nlen,
-                          int logdst, float px, float py, float pz, bool cast, Cnt &c)
+__device__ __forceinline__ float ff_prefix(const SrcPatch &s, float nx, float ny, float nz, float nlen, float px,
+                                           float py, float pz)
 {
 	const float dx = s.c[0] - px, dy = s.c[1] - py, dz = s.c[2] - pz;
 	const float ds = dot3<F>(dx, dy, dz, s.n[0], s.n[1], s.n[2]);
@@ -235,11 +294,19 @@ __device__ float point_ff(const DScene &S, const SrcPatch &s, int logsrc, float 
 	float ff = c1 * c2;
 	if (ff <= 0.0f) return 0.0f;
 	ff = (float)((double)ff * ((double)s.area / ((double)r2 * kPiD)));
-	if (ff <= 0.0f) return 0.0f;
-	c.ref_rays++;
-	if (!cast) return ff;
+	return ff <= 0.0f ? 0.0f : ff;
+}
+
+// Whole CalFF: prefix, then the ray (shade.cpp:124).
+template <bool F>
+__device__ float point_ff(const DScene &S, const SrcPatch &s, int logsrc, float nx, float ny, float nz, float nlen,
+                          int logdst, float px, float py, float pz, Cnt &c)
+{
+	const float ff = ff_prefix<F>(s, nx, ny, nz, nlen, px, py, pz);
+	if (ff == 0.0f) return 0.0f;
 	c.rays++;
-	return walk<F>(S, px, py, pz, dx, dy, dz, logdst, c) == logsrc ? ff : 0.0f;
+	c.ref_rays++;
+	return walk<F>(S, px, py, pz, s.c[0] - px, s.c[1] - py, s.c[2] - pz, logdst, c) == logsrc ? ff : 0.0f;
 }
 
 // Source descriptor of slot `idx`: the whole patch and, when its longest
@@ -378,95 +445,202 @@ template <bool F> __global__ __launch_bounds__(1024) void k_topk(DRecv R, Loop *
 	}
 }
 
-template <bool F, bool FUSED_SELECT, bool LOWEST>
-__global__ __launch_bounds__(kBlock) void k_shoot(DScene S, DRecv R, Loop *L, float converge, long long *bcnt, int nblocks)
+// Receiver-side mapping shared by k_gen and k_update: 168 receivers per
+// block, 3 lanes (vertices) per receiver, 21 receivers per wave.  Blocks
+// are XCD-aware: the hardware deals consecutive blocks to different XCDs,
+// so each XCD is given a contiguous run of receivers.
+struct RecvMap {
+	int vb, lane, wid, v, base;
+	__device__ RecvMap(int nblocks)
+	{
+		const int xcd = blockIdx.x & 7, per = (nblocks + 7) >> 3;
+		vb = xcd * per + (blockIdx.x >> 3);
+		lane = threadIdx.x & 63;
+		wid = threadIdx.x >> 6;
+		const int rl = lane / 3;
+		v = lane - 3 * rl;
+		base = lane - v;
+	}
+	__device__ int slot(int g) const
+	{
+		return ((vb * kGroups + g) * kWavesPerBlock + wid) * kRecvPerWave + lane / 3;
+	}
+	__device__ bool valid(int g, int nslots, int nblocks) const
+	{
+		return vb < nblocks && lane < 63 && slot(g) < nslots;
+	}
+};
+
+__device__ void add_block_counters(const int (&vals_in)[8], long long *bcnt)
 {
-	if (L->done) return;
-	__shared__ unsigned long long wmax[kWavesPerBlock];
 	__shared__ int wcnt[kWavesPerBlock][8];
-	// XCD-aware order: consecutive hardware blocks go to different XCDs, so
-	// give each XCD a contiguous run of receivers (and of their rays' cells)
+	int vals[8];
+	for (int k = 0; k < 8; k++) vals[k] = wave_sum(vals_in[k]);
+	if ((threadIdx.x & 63) == 0)
+		for (int k = 0; k < 8; k++) wcnt[threadIdx.x >> 6][k] = vals[k];
+	__syncthreads();
+	if (threadIdx.x == 0)
+		for (int k = 0; k < 8; k++) {
+			long long sum = 0;
+			for (int w = 0; w < kWavesPerBlock; w++) sum += wcnt[w][k];
+			if (sum) bcnt[(size_t)blockIdx.x * 8 + k] += sum;
+		}
+}
+
+// Source k's rays, part 1: which receivers take part (fixed at the first
+// source: parent >= 0, not a source, not all nine accB >= 255,
+// rad.cpp:420-428) and CalFF up to the ray (shade.cpp:107-123) for each
+// (receiver, vertex, half) into ffbuf[2 * (3 * slot + vertex) + half]; 0
+// where no ray is cast.
+template <bool F>
+__global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigned char *live, float *ffbuf,
+                                                long long *bcnt, int nblocks)
+{
+	if (L->done || k >= L->nsrc) return;
+	const RecvMap m(nblocks);
+	const SrcDesc &s = L->src[k];
+	int ref_rays = 0;
+#pragma unroll 1
+	for (int g = 0; g < kGroups; g++) {
+		const int r = m.slot(g);
+		const bool valid = m.valid(g, R.nslots, nblocks);
+		bool lv;
+		if (k == 0) {
+			int parent = -1;
+			float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+			if (valid) {
+				parent = __float_as_int(R.nrm[r].w);
+				acc = R.acc[3 * r + m.v];
+			}
+			const int vfull = !(acc.x < 255.0f || acc.y < 255.0f || acc.z < 255.0f);
+			const bool full = __shfl(vfull, m.base) && __shfl(vfull, m.base + 1) && __shfl(vfull, m.base + 2);
+			bool chosen = false;
+			for (int j = 0; j < L->nsrc; j++) chosen |= (L->src[j].phys == r);
+			lv = valid && parent >= 0 && !chosen && !full;
+			if (valid && m.v == 0) live[r] = lv;
+		} else {
+			lv = valid && live[r];
+		}
+		if (!valid) continue;
+		float f0 = 0.f, f1 = 0.f;
+		const int sv = 3 * r + m.v;
+		if (lv) {
+			const float4 nr = R.nrm[r];
+			const float nlen = __builtin_sqrtf(dot3<F>(nr.x, nr.y, nr.z, nr.x, nr.y, nr.z));
+			const float px = R.vx[sv], py = R.vy[sv], pz = R.vz[sv];
+			if (s.split) {
+				// the full-source ray AdaptCalFF computes and then discards
+				if (ff_prefix<F>(s.whole, nr.x, nr.y, nr.z, nlen, px, py, pz) != 0.0f) ref_rays++;
+				f0 = ff_prefix<F>(s.half[0], nr.x, nr.y, nr.z, nlen, px, py, pz);
+				f1 = ff_prefix<F>(s.half[1], nr.x, nr.y, nr.z, nlen, px, py, pz);
+			} else {
+				f0 = ff_prefix<F>(s.whole, nr.x, nr.y, nr.z, nlen, px, py, pz);
+			}
+		}
+		ffbuf[2 * sv] = f0;
+		ffbuf[2 * sv + 1] = f1;
+	}
+	const int vals[8] = {0, ref_rays, 0, 0, 0, 0, 0, 0};
+	add_block_counters(vals, bcnt);
+}
+
+// Source k's rays, part 2: one lane per (receiver, vertex, half) in slot
+// order, so a wave's rays start from ~11 neighbouring receivers toward the
+// same point; lanes without a ray exit at once (facing and non-facing
+// receivers come in spatial runs, so waves are mostly all busy or all idle).
+// RayHitted (raycast.cpp:335-360); a ray that does not reach the source
+// zeroes its form factor (shade.cpp:124-126).  XCD-contiguous block order.
+template <bool F>
+__global__ __launch_bounds__(kBlock) void k_trace(DScene S, DRecv R, Loop *L, int k, float *ffbuf, long long *bcnt,
+                                                  int nblocks)
+{
+	if (L->done || k >= L->nsrc) return;
 	const int xcd = blockIdx.x & 7, per = (nblocks + 7) >> 3;
 	const int vb = xcd * per + (blockIdx.x >> 3);
-	const int lane = threadIdx.x & 63;
-	const int rl = lane / 3, v = lane - 3 * rl, base = lane - v;
-	const int r = (vb * kWavesPerBlock + (threadIdx.x >> 6)) * kRecvPerWave + rl;
-	const bool slot = lane < 63 && vb < nblocks && r < R.nslots;
-	const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-	const float4 nr = slot ? R.nrm[r] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-	const int parent = __float_as_int(nr.w);
-	float4 acc = slot ? R.acc[3 * r + v] : zero;
-	// saturated: all nine accB >= 255 (rad.cpp:424-428)
-	const bool vfull = !(acc.x < 255.0f || acc.y < 255.0f || acc.z < 255.0f);
-	const bool full = __shfl(vfull, base) && __shfl(vfull, base + 1) && __shfl(vfull, base + 2);
-	const int nsrc = L->nsrc;
-	bool chosen = false;
-	for (int k = 0; k < nsrc; k++) chosen |= (L->src[k].phys == r);
-	const bool live = slot && parent >= 0 && !chosen && !full;
-	float4 db = slot ? R.dB[r] : zero;
+	const int i = vb * kBlock + threadIdx.x;
 	Cnt c = {0, 0, 0, 0, 0, 0, 0};
-	if (live) {
-		const float px = R.vx[3 * r + v], py = R.vy[3 * r + v], pz = R.vz[3 * r + v];
-		const float4 rho = R.rho[r];
-		const float nlen = __builtin_sqrtf(dot3<F>(nr.x, nr.y, nr.z, nr.x, nr.y, nr.z));
-		float4 dacc = zero;
-		for (int k = 0; k < nsrc; k++) {
-			const SrcDesc &s = L->src[k];
-			float ff;
-			if (s.split) {
-				point_ff<F>(S, s.whole, s.log, nr.x, nr.y, nr.z, nlen, parent, px, py, pz, false, c);
-				const float f1 = point_ff<F>(S, s.half[0], s.log, nr.x, nr.y, nr.z, nlen, parent, px, py, pz, true, c);
-				const float f2 = point_ff<F>(S, s.half[1], s.log, nr.x, nr.y, nr.z, nlen, parent, px, py, pz, true, c);
-				ff = f1 + f2;
-			} else {
-				ff = point_ff<F>(S, s.whole, s.log, nr.x, nr.y, nr.z, nlen, parent, px, py, pz, true, c);
-			}
-			const float kr = (float)(0.5 / 255.0);
-			const float dr = rho.x * kr * s.dB[0] / 3 * ff;
-			const float dg = rho.y * kr * s.dB[1] / 3 * ff;
-			const float dbb = rho.z * kr * s.dB[2] / 3 * ff;
-			// the receiver's three lanes are all live or all not
-			const float r1 = __shfl(dr, base + 1), g1 = __shfl(dg, base + 1), b1 = __shfl(dbb, base + 1);
-			const float r2 = __shfl(dr, base + 2), g2 = __shfl(dg, base + 2), b2 = __shfl(dbb, base + 2);
+	if (vb < nblocks && i < 6 * R.nslots && ffbuf[i] != 0.0f) {
+		const SrcDesc &s = L->src[k];
+		const int sv = i >> 1, h = i & 1, r = sv / 3;
+		const float px = R.vx[sv], py = R.vy[sv], pz = R.vz[sv];
+		const int ban = __float_as_int(R.nrm[r].w);
+		const SrcPatch &sp = s.split ? s.half[h] : s.whole;
+		c.rays++;
+		c.ref_rays++;
+		if (walk<F>(S, px, py, pz, sp.c[0] - px, sp.c[1] - py, sp.c[2] - pz, ban, c) != s.log) ffbuf[i] = 0.0f;
+	}
+	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
+	add_block_counters(vals, bcnt);
+}
+
+// Source k, part 3: CalRadiosity (shade.cpp:244-259) per receiver, its
+// three vertex lanes combined by shuffles in the reference's order; after
+// the last source the next iteration's candidate key (listCanditate,
+// rad.cpp:387-404) by wave, block and one 64-bit atomicMax per block.
+template <bool F, bool FUSED_SELECT, bool LOWEST>
+__global__ __launch_bounds__(kBlock) void k_update(DRecv R, Loop *L, int k, const unsigned char *live,
+                                                   const float *ffbuf, float converge, long long *bcnt, int nblocks)
+{
+	if (L->done || k >= L->nsrc) return;
+	__shared__ unsigned long long wmax[kWavesPerBlock];
+	const RecvMap m(nblocks);
+	const SrcDesc &s = L->src[k];
+	const bool last = k == L->nsrc - 1;
+	unsigned long long key = 0;
+	int recv = 0;
+#pragma unroll 1
+	for (int g = 0; g < kGroups; g++) {
+		const int r = m.slot(g);
+		const bool valid = m.valid(g, R.nslots, nblocks);
+		const bool lv = valid && live[r];
+		const int sv = 3 * r + m.v;
+		float ff = 0.f;
+		float4 rho = make_float4(0.f, 0.f, 0.f, 0.f);
+		if (lv) {
+			ff = s.split ? ffbuf[2 * sv] + ffbuf[2 * sv + 1] : ffbuf[2 * sv];
+			rho = R.rho[r];
+		}
+		const float kr = (float)(0.5 / 255.0);
+		const float dr = rho.x * kr * s.dB[0] / 3 * ff;
+		const float dg = rho.y * kr * s.dB[1] / 3 * ff;
+		const float dbb = rho.z * kr * s.dB[2] / 3 * ff;
+		const float r1 = __shfl(dr, m.base + 1), g1 = __shfl(dg, m.base + 1), b1 = __shfl(dbb, m.base + 1);
+		const float r2 = __shfl(dr, m.base + 2), g2 = __shfl(dg, m.base + 2), b2 = __shfl(dbb, m.base + 2);
+		float4 db = make_float4(0.f, 0.f, 0.f, 0.f);
+		if (valid && m.v == 0) db = R.dB[r];
+		if (lv) {
+			float4 acc = R.acc[sv];
 			acc.x += dr;
 			acc.y += dg;
 			acc.z += dbb;
-			dacc = make_float4(dr, dg, dbb, 0.f);
-			if (v == 0) {
+			R.acc[sv] = acc;
+			R.dacc[sv] = make_float4(dr, dg, dbb, 0.f);
+			if (m.v == 0) {
 				db.x += dr; db.x += r1; db.x += r2;
 				db.y += dg; db.y += g1; db.y += g2;
 				db.z += dbb; db.z += b1; db.z += b2;
+				R.dB[r] = db;
+				recv++;
 			}
 		}
-		R.acc[3 * r + v] = acc;
-		R.dacc[3 * r + v] = dacc;
-		if (v == 0) R.dB[r] = db;
+		if (FUSED_SELECT && last && valid && m.v == 0 && __float_as_int(R.nrm[r].w) >= 0) {
+			const float sum = db.x + db.y + db.z;
+			if (sum > converge) {
+				const unsigned long long kk = cand_key<LOWEST>(sum, r);
+				key = kk > key ? kk : key;
+			}
+		}
 	}
 	if (FUSED_SELECT) {
-		unsigned long long key = 0;
-		if (slot && v == 0 && parent >= 0) {
-			const float sum = db.x + db.y + db.z;
-			if (sum > converge) key = cand_key<LOWEST>(sum, r);
-		}
 		key = wave_max(key);
-		if (lane == 0) wmax[threadIdx.x >> 6] = key;
+		if (m.lane == 0) wmax[m.wid] = key;
 	}
-	const int recv = (live && v == 0) ? nsrc : 0;
-	int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, recv};
-	for (int k = 0; k < 8; k++) vals[k] = wave_sum(vals[k]);
-	if (lane == 0)
-		for (int k = 0; k < 8; k++) wcnt[threadIdx.x >> 6][k] = vals[k];
-	__syncthreads();
+	const int vals[8] = {0, 0, 0, 0, 0, 0, 0, recv};
+	add_block_counters(vals, bcnt);  // synchronises the block
 	if (threadIdx.x == 0) {
 		if (FUSED_SELECT) {
-			unsigned long long key = wmax[0];
 			for (int w = 1; w < kWavesPerBlock; w++) key = wmax[w] > key ? wmax[w] : key;
 			if (key) atomicMax(&L->key, key);
-		}
-		for (int k = 0; k < 8; k++) {
-			long long s = 0;
-			for (int w = 0; w < kWavesPerBlock; w++) s += wcnt[w][k];
-			bcnt[(size_t)blockIdx.x * 8 + k] += s;
 		}
 	}
 }
@@ -500,10 +674,10 @@ __global__ __launch_bounds__(kBlock) void k_ff_pairs(DScene S, DRecv R, const in
 		const int logdst = __float_as_int(nr.w);
 		float ff;
 		if (s.split)
-			ff = point_ff<F>(S, s.half[0], s.log, nr.x, nr.y, nr.z, nlen, logdst, px, py, pz, true, c) +
-			     point_ff<F>(S, s.half[1], s.log, nr.x, nr.y, nr.z, nlen, logdst, px, py, pz, true, c);
+			ff = point_ff<F>(S, s.half[0], s.log, nr.x, nr.y, nr.z, nlen, logdst, px, py, pz, c) +
+			     point_ff<F>(S, s.half[1], s.log, nr.x, nr.y, nr.z, nlen, logdst, px, py, pz, c);
 		else
-			ff = point_ff<F>(S, s.whole, s.log, nr.x, nr.y, nr.z, nlen, logdst, px, py, pz, true, c);
+			ff = point_ff<F>(S, s.whole, s.log, nr.x, nr.y, nr.z, nlen, logdst, px, py, pz, c);
 		ff3[3 * i + v] = ff;
 	}
 }
@@ -539,20 +713,37 @@ template <typename T> struct DevBuf {
 	}
 };
 
+float smallest_float_at_least(double x)
+{
+	float f = (float)x;
+	while ((double)f < x) f = std::nextafter(f, INFINITY);
+	while ((double)std::nextafter(f, -INFINITY) >= x) f = std::nextafter(f, -INFINITY);
+	return f;
+}
+float largest_float_at_most(double x)
+{
+	float f = (float)x;
+	while ((double)f > x) f = std::nextafter(f, -INFINITY);
+	while ((double)std::nextafter(f, INFINITY) <= x) f = std::nextafter(f, INFINITY);
+	return f;
+}
+
 }  // namespace
 
 struct DeviceEngine::Impl {
 	hipStream_t stream = nullptr;
-	DevBuf<float4> tri_nc, tri_se, leaf_box, nrm, cA, rho, acc, dacc, dB;
-	DevBuf<int> cell_leaf, items;
+	DevBuf<float4> lnc, lse, leaf_box, nrm, cA, rho, acc, dacc, dB;
+	DevBuf<int> cell_leaf;
 	DevBuf<float> vx, vy, vz;
 	DevBuf<Loop> loop;
 	DevBuf<long long> bcnt;
 	DevBuf<unsigned long long> cand;
+	DevBuf<unsigned char> live;
+	DevBuf<float> ffbuf;
 	DScene S;
 	DRecv R;
 	Loop *host_loop = nullptr;  // pinned mirror of the loop header
-	int nblocks = 0, n_orig = 0;
+	int nblocks = 0, n_orig = 0, trace_nblocks = 0;
 	bool fma = true, lowest = false;
 	int parallel_src = 1, batch = 32;
 	float converge = 1200.f;
@@ -595,10 +786,10 @@ DeviceEngine *DeviceEngine::create(std::string &err)
 	}
 	for (hipEvent_t &ev : e->d_->kev)
 		if (hipEventCreate(&ev) != hipSuccess) {
-		err = "HIP stream/event/pinned allocation failed";
-		delete e;
-		return nullptr;
-	}
+			err = "HIP event creation failed";
+			delete e;
+			return nullptr;
+		}
 	return e;
 }
 
@@ -620,14 +811,37 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 		err = "octree cell table too large";
 		return -1;
 	}
-	// originals as the octree sees them: post-pre-partition slot contents
-	std::vector<float4> nc((size_t)n0 * 2), se((size_t)n0 * 3);
-	for (int i = 0; i < n0; i++) {
-		const Patch &p = s.pool[i];
-		nc[2 * i] = make_float4(p.n[0], p.n[1], p.n[2], p.c[0]);
-		nc[2 * i + 1] = make_float4(p.c[1], p.c[2], 0.f, 0.f);
-		for (int k = 0; k < 3; k++) se[3 * i + k] = make_float4(p.se[k][0], p.se[k][1], p.se[k][2], p.se[k][3]);
+	// leaf lists with the originals inline, as the octree sees them: the
+	// post-pre-partition contents of slots [0, n_orig)
+	const size_t ni = t.items.size();
+	std::vector<float4> lnc(ni * 2), lse(ni * 3);
+	for (size_t k = 0; k < ni; k++) {
+		const int id = t.items[k];
+		if (id < 0 || id >= n0) {
+			err = "octree item out of range";
+			return -1;
+		}
+		const Patch &p = s.pool[id];
+		float idf;
+		std::memcpy(&idf, &id, 4);
+		lnc[2 * k] = make_float4(p.n[0], p.n[1], p.n[2], p.c[0]);
+		lnc[2 * k + 1] = make_float4(p.c[1], p.c[2], idf, 0.f);
+		for (int q = 0; q < 3; q++) lse[3 * k + q] = make_float4(p.se[q][0], p.se[q][1], p.se[q][2], p.se[q][3]);
 	}
+	for (int l = 0; l < t.n_leaves; l++) {
+		int begin, cd;
+		std::memcpy(&begin, &t.leaf_box[(size_t)l * 8 + 3], 4);
+		std::memcpy(&cd, &t.leaf_box[(size_t)l * 8 + 7], 4);
+		if (begin < 0 || (size_t)begin + (size_t)(cd & 0xffffff) > ni) {
+			err = "octree leaf list out of range";
+			return -1;
+		}
+	}
+	for (int leaf : t.cell_leaf)
+		if (leaf < 0 || leaf >= t.n_leaves) {
+			err = "octree cell table entry out of range";
+			return -1;
+		}
 	std::vector<float> vx((size_t)n * 3), vy((size_t)n * 3), vz((size_t)n * 3);
 	std::vector<float4> nrm(n), cA(n), rho(n), acc((size_t)n * 3), dacc((size_t)n * 3), dB(n);
 	for (int i = 0; i < n; i++) {
@@ -648,25 +862,31 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 		dB[i] = make_float4(p.deltaB[0], p.deltaB[1], p.deltaB[2], 0.f);
 	}
 	hipStream_t st = d.stream;
-	if (d.tri_nc.put(nc.data(), nc.size(), st, err) || d.tri_se.put(se.data(), se.size(), st, err) ||
+	if (d.lnc.put(lnc.data(), lnc.size(), st, err) || d.lse.put(lse.data(), lse.size(), st, err) ||
 	    d.cell_leaf.put(t.cell_leaf.data(), t.cell_leaf.size(), st, err) ||
 	    d.leaf_box.put((const float4 *)t.leaf_box.data(), t.leaf_box.size() / 4, st, err) ||
-	    d.items.put(t.items.data(), t.items.size(), st, err) || d.vx.put(vx.data(), vx.size(), st, err) ||
-	    d.vy.put(vy.data(), vy.size(), st, err) || d.vz.put(vz.data(), vz.size(), st, err) ||
-	    d.nrm.put(nrm.data(), nrm.size(), st, err) || d.cA.put(cA.data(), cA.size(), st, err) ||
-	    d.rho.put(rho.data(), rho.size(), st, err) || d.acc.put(acc.data(), acc.size(), st, err) ||
-	    d.dacc.put(dacc.data(), dacc.size(), st, err) || d.dB.put(dB.data(), dB.size(), st, err))
+	    d.vx.put(vx.data(), vx.size(), st, err) || d.vy.put(vy.data(), vy.size(), st, err) ||
+	    d.vz.put(vz.data(), vz.size(), st, err) || d.nrm.put(nrm.data(), nrm.size(), st, err) ||
+	    d.cA.put(cA.data(), cA.size(), st, err) || d.rho.put(rho.data(), rho.size(), st, err) ||
+	    d.acc.put(acc.data(), acc.size(), st, err) || d.dacc.put(dacc.data(), dacc.size(), st, err) ||
+	    d.dB.put(dB.data(), dB.size(), st, err))
 		return -1;
-	const int waves = (n + kRecvPerWave - 1) / kRecvPerWave;
-	d.nblocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+	// grid sizes first: every kernel that adds into bcnt indexes it by its
+	// hardware block index, so bcnt must cover the largest grid
+	d.nblocks = (n + kRecvPerBlock - 1) / kRecvPerBlock;
+	d.trace_nblocks = (int)(((size_t)n * 6 + kBlock - 1) / kBlock);
 	const int grid = ((d.nblocks + 7) / 8) * 8;  // a whole number of blocks per XCD
-	std::vector<long long> zeros((size_t)grid * 8, 0);
+	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
+	std::vector<long long> zeros((size_t)std::max(grid, tgrid) * 8, 0);
 	if (d.bcnt.put(zeros.data(), zeros.size(), st, err)) return -1;
 	if (!d.lowest && d.parallel_src > 1 && d.cand.alloc((size_t)n + 1, err)) return -1;
+	if (d.live.alloc((size_t)n + 1, err) || d.ffbuf.alloc((size_t)n * 6 + 2, err)) return -1;
 	std::memset(d.host_loop, 0, sizeof(Loop));
 	if (d.loop.put(d.host_loop, 1, st, err)) return -1;
-	d.S = DScene{d.tri_nc.p, d.tri_se.p, d.cell_leaf.p, d.leaf_box.p, d.items.p, s.G0[0], s.G0[1], s.G0[2],
-	             s.G1[0], s.G1[1], s.G1[2], (double)t.min_grid, t.gt};
+	const float thr_t = smallest_float_at_least(0.0001);
+	d.S = DScene{d.lnc.p, d.lse.p, d.cell_leaf.p, d.leaf_box.p, s.G0[0], s.G0[1], s.G0[2], s.G1[0], s.G1[1], s.G1[2],
+	             (double)t.min_grid, 1.0 / (double)t.min_grid, t.gt, thr_t, thr_t * kGuardDown,
+	             largest_float_at_most(1e-6)};
 	d.R = DRecv{d.vx.p, d.vy.p, d.vz.p, d.nrm.p, d.cA.p, d.rho.p, d.acc.p, d.dacc.p, d.dB.p, n};
 	if (d.lowest || d.parallel_src <= 1) {
 		const int sb = (n + kBlock - 1) / kBlock;
@@ -696,6 +916,11 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 	}
 	const int start = d.host_loop->shots;
 	const int grid = ((d.nblocks + 7) / 8) * 8;
+	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
+	if (d.bcnt.n < (size_t)std::max(grid, tgrid) * 8 || d.ffbuf.n < (size_t)d.R.nslots * 6 || d.live.n < (size_t)d.R.nslots) {
+		err = "device buffers smaller than the launch grids";
+		return -1;
+	}
 	const bool multi = !d.lowest && d.parallel_src > 1;
 	const int cb = (d.R.nslots + kBlock - 1) / kBlock;
 	int queued = 0;
@@ -714,14 +939,20 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 				if (d.lowest) hipLaunchKernelGGL((k_prep<false, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
 				else hipLaunchKernelGGL((k_prep<false, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
 			}
-			if (grid > 0) {
-				if (timing) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv)], st), "event");
-#define SHOOT(F, FS, LO) hipLaunchKernelGGL((k_shoot<F, FS, LO>), dim3(grid), dim3(kBlock), 0, st, d.S, d.R, d.loop.p, d.converge, d.bcnt.p, d.nblocks)
-				if (multi) { if (d.fma) SHOOT(true, false, false); else SHOOT(false, false, false); }
-				else if (d.lowest) { if (d.fma) SHOOT(true, true, true); else SHOOT(false, true, true); }
-				else { if (d.fma) SHOOT(true, true, false); else SHOOT(false, true, false); }
-#undef SHOOT
-				if (timing) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv) + 1], st), "event");
+			const int nk = multi ? d.parallel_src : 1;
+			for (int k = 0; grid > 0 && k < nk; k++) {
+				const dim3 g(grid), blk(kBlock), tg(tgrid);
+				if (d.fma) hipLaunchKernelGGL(k_gen<true>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks);
+				else hipLaunchKernelGGL(k_gen<false>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks);
+				if (timing && k == 0) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv)], st), "event");
+				if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, blk, 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.trace_nblocks);
+				else hipLaunchKernelGGL(k_trace<false>, tg, blk, 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.trace_nblocks);
+				if (timing && k == 0) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv) + 1], st), "event");
+#define UPD(F, FS, LO) hipLaunchKernelGGL((k_update<F, FS, LO>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.converge, d.bcnt.p, d.nblocks)
+				if (multi) { if (d.fma) UPD(true, false, false); else UPD(false, false, false); }
+				else if (d.lowest) { if (d.fma) UPD(true, true, true); else UPD(false, true, true); }
+				else { if (d.fma) UPD(true, true, false); else UPD(false, true, false); }
+#undef UPD
 			}
 			if (timing && (i % Impl::kEv == Impl::kEv - 1 || i == b - 1)) {
 				// collect this group of kernel timings before the events are reused
@@ -786,13 +1017,17 @@ int DeviceEngine::cast(int n, const RayIn *rays, int *hit, int *cells, int *test
 	hipStream_t st = d.stream;
 	if (dr.put(rays, n, st, err) || dh.alloc(n, err) || dc.alloc(n, err) || dt.alloc(n, err)) return -1;
 	const int blocks = (n + kBlock - 1) / kBlock;
+	HIP_OK(hipEventRecord(d.ev0, st), "event");
 	if (d.fma) hipLaunchKernelGGL(k_cast<true>, dim3(blocks), dim3(kBlock), 0, st, d.S, dr.p, n, dh.p, dc.p, dt.p);
 	else hipLaunchKernelGGL(k_cast<false>, dim3(blocks), dim3(kBlock), 0, st, d.S, dr.p, n, dh.p, dc.p, dt.p);
 	HIP_OK(hipGetLastError(), "cast launch");
+	HIP_OK(hipEventRecord(d.ev1, st), "event");
 	HIP_OK(hipMemcpyAsync(hit, dh.p, n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
 	if (cells) HIP_OK(hipMemcpyAsync(cells, dc.p, n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
 	if (tests) HIP_OK(hipMemcpyAsync(tests, dt.p, n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
 	HIP_OK(hipStreamSynchronize(st), "cast");
+	float ms = 0;
+	if (hipEventElapsedTime(&ms, d.ev0, d.ev1) == hipSuccess) cast_ms_ = ms;
 	return 0;
 }
 

@@ -85,7 +85,7 @@ int main(int argc, char **argv)
 	}
 	std::fprintf(stderr, "[INFO] Radiosity Method took %f sec. time.\n", std::chrono::duration<double>(t2 - t1).count());
 	long long cnt[RAD_CNT_N];
-	double tm[6];
+	double tm[7];
 	rad_counters(ctx, cnt);
 	rad_timings(ctx, tm);
 	std::fprintf(stderr, "[STAT] rays %lld (reference-equivalent %lld) cells %lld tests %lld precision %lld stalled %lld; "

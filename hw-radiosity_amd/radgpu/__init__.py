@@ -16,7 +16,7 @@ import subprocess
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "_bin", "libradgpu.so")
+LIB_PATH = os.environ.get("RADGPU_LIB") or os.path.join(PKG_DIR, "_bin", "libradgpu.so")
 RAD_BIN = os.path.join(PKG_DIR, "_bin", "rad")
 
 RAD_OK, RAD_EINVAL, RAD_EIO, RAD_ESTATE, RAD_EDEVICE = range(5)
@@ -185,10 +185,10 @@ class Engine:
         return dict(zip(COUNTER_NAMES, (int(x) for x in out)))
 
     def timings(self) -> dict:
-        out = np.zeros(6, np.float64)
+        out = np.zeros(7, np.float64)
         self._ok(library().rad_timings(self.h, _p(out)))
         return {"setup_ms": out[0], "upload_ms": out[1], "shoot_ms": out[2], "iterations_launched": int(out[3]),
-                "kernel_ms": out[4], "kernels_timed": int(out[5])}
+                "kernel_ms": out[4], "kernels_timed": int(out[5]), "cast_ms": out[6]}
 
 
 def run_cli(args, cwd=None, timeout=None):

@@ -109,9 +109,10 @@ rad_status rad_counters(rad_ctx *ctx, long long *out);
 
 /* Timings: out[0] host setup ms (load to prepare), out[1] upload ms,
  * out[2] device ms of the shooting loop, out[3] shooting iterations
- * launched, out[4] device ms of the fused shoot kernel alone and out[5] the
- * number of such kernels timed (both 0 unless -kernel_timing 1). */
-rad_status rad_timings(rad_ctx *ctx, double *out6);
+ * launched, out[4] device ms of the ray-tracing kernel alone and out[5] the
+ * number of such kernels timed (both 0 unless -kernel_timing 1), out[6]
+ * device ms of the last rad_cast kernel. */
+rad_status rad_timings(rad_ctx *ctx, double *out7);
 
 #ifdef __cplusplus
 }
