@@ -29,6 +29,9 @@ METRIC = "rays/s + time-to-converge at 1e-3 rel-L2 vs FP64; mixed/batched vs pla
 OPTS = ["-converge", "1200", "-delta_ff", "5e-6", "-adapt_area", "5", "-triangle", "1000000",
         "-write_cycle", "1000000"]
 HBM_PEAK_GBS = 8000.0
+# measured HBM bytes per k_trace dispatch on this workload (rocprofv3 --pmc
+# FETCH_SIZE + WRITE_SIZE, copy-calibrated); see profiles/
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_trace_hall200k_pmc.json")
 
 
 def log(*a):
@@ -122,6 +125,10 @@ def main():
     bytes_per_launch = algorithmic_bytes(cnt_sum) / max(kn, 1)
     achieved = bytes_per_launch / (kernel_avg_ms * 1e-3) / 1e9 if kernel_avg_ms > 0 else 0.0
     value = rays / total_s
+    traffic = None
+    if os.path.exists(PMC_SUMMARY):
+        with open(PMC_SUMMARY) as fh:
+            traffic = json.load(fh).get("traffic_bytes_per_dispatch")
     out = {
         "metric": METRIC,
         "value": value,
@@ -143,9 +150,12 @@ def main():
                                "host_setup_ms": e2e_t["setup_ms"], "upload_ms": e2e_t["upload_ms"]},
         "roofline": {"kernel": "k_trace (octree-DDA visibility of the iteration's rays)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_kernel_ms": kernel_avg_ms, "launches": kn,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "note": "algorithmic bytes count every octree and triangle byte a ray reads; most "
+                             "are L2/MALL hits (traffic = measured HBM bytes per launch), so frac above "
+                             "1 means the kernel is latency/divergence-bound, not HBM-bound"},
         "counters": cnt_sum,
     }
     if not args.no_cpu:

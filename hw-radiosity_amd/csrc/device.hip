@@ -108,7 +108,8 @@ struct Loop {
 	int done;
 	int shots;
 	int nsrc;
-	int pad[2];
+	int ncand;  // candidates after the last update: a lower bound on the shots left
+	int pad;
 	SrcDesc src[kMaxSrc];
 };
 
@@ -359,19 +360,32 @@ template <bool LOWEST>
 __global__ __launch_bounds__(kBlock) void k_select(DRecv R, Loop *L, float converge)
 {
 	__shared__ unsigned long long wmax[kWavesPerBlock];
+	__shared__ int wn[kWavesPerBlock];
 	const int r = blockIdx.x * kBlock + threadIdx.x;
 	unsigned long long key = 0;
+	int n = 0;
 	if (r < R.nslots && __float_as_int(R.nrm[r].w) >= 0) {
 		const float4 db = R.dB[r];
 		const float sum = db.x + db.y + db.z;
-		if (sum > converge) key = cand_key<LOWEST>(sum, r);
+		if (sum > converge) {
+			key = cand_key<LOWEST>(sum, r);
+			n = 1;
+		}
 	}
 	key = wave_max(key);
-	if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = key;
+	n = wave_sum(n);
+	if ((threadIdx.x & 63) == 0) {
+		wmax[threadIdx.x >> 6] = key;
+		wn[threadIdx.x >> 6] = n;
+	}
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		for (int w = 1; w < kWavesPerBlock; w++) key = wmax[w] > key ? wmax[w] : key;
+		for (int w = 1; w < kWavesPerBlock; w++) {
+			key = wmax[w] > key ? wmax[w] : key;
+			n += wn[w];
+		}
 		if (key) atomicMax(&L->key, key);
+		if (n) atomicAdd(&L->ncand, n);
 	}
 }
 
@@ -389,6 +403,7 @@ template <bool F, bool LOWEST> __global__ void k_prep(DRecv R, Loop *L)
 	R.dB[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
 	L->nsrc = 1;
 	L->shots++;
+	L->ncand = 0;
 }
 
 // Multi-source selection (-parallel_src k > 1): collect candidate keys, then
@@ -442,6 +457,7 @@ template <bool F> __global__ __launch_bounds__(1024) void k_topk(DRecv R, Loop *
 		L->nsrc = k;
 		L->shots++;
 		L->count = 0;
+		L->ncand = 0;
 	}
 }
 
@@ -583,11 +599,12 @@ __global__ __launch_bounds__(kBlock) void k_update(DRecv R, Loop *L, int k, cons
 {
 	if (L->done || k >= L->nsrc) return;
 	__shared__ unsigned long long wmax[kWavesPerBlock];
+	__shared__ int wn[kWavesPerBlock];
 	const RecvMap m(nblocks);
 	const SrcDesc &s = L->src[k];
 	const bool last = k == L->nsrc - 1;
 	unsigned long long key = 0;
-	int recv = 0;
+	int recv = 0, ncand = 0;
 #pragma unroll 1
 	for (int g = 0; g < kGroups; g++) {
 		const int r = m.slot(g);
@@ -623,25 +640,32 @@ __global__ __launch_bounds__(kBlock) void k_update(DRecv R, Loop *L, int k, cons
 				recv++;
 			}
 		}
-		if (FUSED_SELECT && last && valid && m.v == 0 && __float_as_int(R.nrm[r].w) >= 0) {
+		if (last && valid && m.v == 0 && __float_as_int(R.nrm[r].w) >= 0) {
 			const float sum = db.x + db.y + db.z;
 			if (sum > converge) {
-				const unsigned long long kk = cand_key<LOWEST>(sum, r);
-				key = kk > key ? kk : key;
+				ncand++;
+				if (FUSED_SELECT) {
+					const unsigned long long kk = cand_key<LOWEST>(sum, r);
+					key = kk > key ? kk : key;
+				}
 			}
 		}
 	}
-	if (FUSED_SELECT) {
-		key = wave_max(key);
-		if (m.lane == 0) wmax[m.wid] = key;
+	if (FUSED_SELECT) key = wave_max(key);
+	ncand = wave_sum(ncand);
+	if (m.lane == 0) {
+		wmax[m.wid] = key;
+		wn[m.wid] = ncand;
 	}
 	const int vals[8] = {0, 0, 0, 0, 0, 0, 0, recv};
 	add_block_counters(vals, bcnt);  // synchronises the block
 	if (threadIdx.x == 0) {
-		if (FUSED_SELECT) {
-			for (int w = 1; w < kWavesPerBlock; w++) key = wmax[w] > key ? wmax[w] : key;
-			if (key) atomicMax(&L->key, key);
+		for (int w = 1; w < kWavesPerBlock; w++) {
+			key = wmax[w] > key ? wmax[w] : key;
+			ncand += wn[w];
 		}
+		if (FUSED_SELECT && key) atomicMax(&L->key, key);
+		if (ncand) atomicAdd(&L->ncand, ncand);
 	}
 }
 
@@ -925,7 +949,12 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 	const int cb = (d.R.nslots + kBlock - 1) / kBlock;
 	int queued = 0;
 	while (queued < max_shots) {
-		const int b = std::min(d.batch, max_shots - queued);
+		// A candidate's unshot energy only grows until it is shot, so the
+		// candidate count bounds the shots left from below: queue no more
+		// iterations than that, and none run idle after convergence.
+		const int per_iter = multi ? d.parallel_src : 1;
+		const int left = std::max(1, (d.host_loop->ncand + per_iter - 1) / per_iter);
+		const int b = std::min(std::min(d.batch, left), max_shots - queued);
 		HIP_OK(hipEventRecord(d.ev0, st), "event");
 		for (int i = 0; i < b; i++) {
 			if (multi) {

@@ -1,12 +1,23 @@
-"""Per-kernel summary of a rocprofv3 --kernel-trace results database.
+"""Summaries of rocprofv3 results databases.
 
-    python -m radgpu.profsum gpurun_out/prof/hall_results.db
+    python -m radgpu.profsum trace <results.db>            per-kernel durations
+    python -m radgpu.profsum pmc <fetch.db> <write.db> <kernel-substring> <copy_bytes>
+
+The pmc form reports FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the
+kernels whose name contains the substring, each scaled by the factor that
+makes the same counter read `copy_bytes` for the plain device copy
+(__amd_rocclr_copyBuffer) profiled in the same run.
 """
+import json
 import sqlite3
 import sys
 
 
-def summarize(db_path):
+def _short(name):
+    return name.replace("rgpu::(anonymous namespace)::", "").replace("void ", "").split("(")[0]
+
+
+def trace(db_path, min_us=0.0):
     con = sqlite3.connect(db_path)
     rows = con.execute(
         "select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
@@ -14,11 +25,36 @@ def summarize(db_path):
     total = sum(r[2] for r in rows) or 1
     out = ["| kernel | calls | total us | avg us | min us | max us | share |", "|---|---|---|---|---|---|---|"]
     for name, n, tot, avg, mn, mx in rows:
-        short = name.replace("rgpu::(anonymous namespace)::", "").replace("void ", "").split("(")[0]
         out.append("| `%s` | %d | %.1f | %.2f | %.2f | %.2f | %.1f%% |" % (
-            short, n, tot / 1e3, avg / 1e3, mn / 1e3, mx / 1e3, 100.0 * tot / total))
+            _short(name), n, tot / 1e3, avg / 1e3, mn / 1e3, mx / 1e3, 100.0 * tot / total))
     return "\n".join(out)
 
 
+def _counter(db_path, counter):
+    con = sqlite3.connect(db_path)
+    return con.execute("select kernel_name, dispatch_id, sum(value) from counters_collection "
+                       "where counter_name = ? group by kernel_name, dispatch_id", (counter,)).fetchall()
+
+
+def pmc(fetch_db, write_db, kernel, copy_bytes):
+    res = {}
+    for counter, db in (("FETCH_SIZE", fetch_db), ("WRITE_SIZE", write_db)):
+        rows = _counter(db, counter)
+        copy_kib = sum(v for n, _, v in rows if "copyBuffer" in n)
+        scale = (copy_bytes / 1024.0) / copy_kib if copy_kib else None
+        ker = [v for n, _, v in rows if kernel in n]
+        busy = [v for v in ker if v > 0]
+        res[counter] = {"copy_kib": copy_kib, "scale": scale, "dispatches": len(ker),
+                        "kib_per_dispatch": sum(ker) / max(len(ker), 1),
+                        "bytes_per_dispatch_scaled": (sum(ker) / max(len(ker), 1)) * 1024.0 * (scale or 0.0),
+                        "nonzero_dispatches": len(busy)}
+    res["traffic_bytes_per_dispatch"] = (res["FETCH_SIZE"]["bytes_per_dispatch_scaled"]
+                                         + res["WRITE_SIZE"]["bytes_per_dispatch_scaled"])
+    return res
+
+
 if __name__ == "__main__":
-    print(summarize(sys.argv[1]))
+    if sys.argv[1] == "trace":
+        print(trace(sys.argv[2]))
+    else:
+        print(json.dumps(pmc(sys.argv[2], sys.argv[3], sys.argv[4], float(sys.argv[5])), indent=1))
