@@ -5,10 +5,12 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/radiosity_gpu.h"
 #include "device.h"
 #include "fin.h"
+#include "linear.h"
 #include "octree.h"
 #include "scene.h"
 
@@ -24,6 +26,11 @@ struct rad_ctx {
 	double ms_setup = 0, ms_upload = 0;
 };
 
+struct rad_lin {
+	std::unique_ptr<rgpu::LinearSystem> sys;
+	std::string err;
+};
+
 namespace {
 
 using Clock = std::chrono::steady_clock;
@@ -35,6 +42,12 @@ double ms_since(Clock::time_point t0)
 rad_status fail(rad_ctx *c, rad_status s, const std::string &msg)
 {
 	c->err = msg;
+	return s;
+}
+
+rad_status lfail(rad_lin *l, rad_status s, const std::string &msg)
+{
+	l->err = msg;
 	return s;
 }
 
@@ -267,6 +280,125 @@ rad_status rad_timings(rad_ctx *c, double *out7)
 	out7[4] = c->dev ? c->dev->kernel_ms() : 0;
 	out7[5] = c->dev ? (double)c->dev->kernel_timed() : 0;
 	return RAD_OK;
+}
+
+/* ---- explicit form-factor matrix ------------------------------------------ */
+
+rad_status rad_lin_create(rad_lin **out, int n)
+{
+	if (!out) return RAD_EINVAL;
+	*out = nullptr;
+	std::unique_ptr<rad_lin> l(new rad_lin());
+	if (n <= 0) return RAD_EINVAL;
+	std::string err;
+	LinearSystem *L = LinearSystem::create(n, err);
+	if (!L) {
+		std::fprintf(stderr, "rad_lin_create: %s\n", err.c_str());
+		return RAD_EDEVICE;
+	}
+	l->sys.reset(L);
+	*out = l.release();
+	return RAD_OK;
+}
+
+void rad_lin_destroy(rad_lin *l) { delete l; }
+
+const char *rad_lin_last_error(const rad_lin *l) { return l ? l->err.c_str() : "null handle"; }
+
+rad_status rad_lin_from_scene(rad_lin *l, rad_ctx *c, int visibility, double *ms)
+{
+	if (!l || !c) return RAD_EINVAL;
+	if (visibility != RAD_VIS_SEGMENT && visibility != RAD_VIS_REFERENCE)
+		return lfail(l, RAD_EINVAL, "unknown visibility mode");
+	if (!c->uploaded) return lfail(l, RAD_ESTATE, "rad_upload the scene first");
+	const HostScene &s = c->scene;
+	if (s.trinum != l->sys->n())
+		return lfail(l, RAD_EINVAL, "matrix size " + std::to_string(l->sys->n()) + " != scene slots " +
+		                                std::to_string(s.trinum));
+	if (c->dev->form_factors(l->sys->matrix32(), s.trinum, l->sys->ld(), visibility, c->tree, s, l->err))
+		return RAD_EDEVICE;
+	if (ms) *ms = c->dev->cast_ms();
+	std::vector<double> area(s.trinum);
+	for (int i = 0; i < s.trinum; i++) area[i] = s.pool[i].area;
+	if (l->sys->set_areas(area.data(), l->err)) return RAD_EDEVICE;
+	return RAD_OK;
+}
+
+rad_status rad_lin_set_matrix(rad_lin *l, const float *F)
+{
+	if (!l || !F) return RAD_EINVAL;
+	return l->sys->set_matrix(F, l->err) ? RAD_EDEVICE : RAD_OK;
+}
+
+rad_status rad_lin_get_matrix(rad_lin *l, float *F)
+{
+	if (!l || !F) return RAD_EINVAL;
+	return l->sys->get_matrix(F, l->err) ? RAD_EDEVICE : RAD_OK;
+}
+
+rad_status rad_lin_set_areas(rad_lin *l, const double *area)
+{
+	if (!l || !area) return RAD_EINVAL;
+	return l->sys->set_areas(area, l->err) ? RAD_EDEVICE : RAD_OK;
+}
+
+rad_status rad_lin_autotune(rad_lin *l, int nrhs, int prec, int *rt, int *splitk)
+{
+	if (!l || nrhs <= 0 || nrhs > 192 || (prec != RAD_PREC_F16 && prec != RAD_PREC_BF16))
+		return l ? lfail(l, RAD_EINVAL, "autotune: nrhs 1..192, precision FP16 or BF16") : RAD_EINVAL;
+	if (l->sys->autotune(nrhs, prec, l->err)) return RAD_EDEVICE;
+	int a, b;
+	l->sys->tile_config(nrhs, prec, &a, &b);
+	if (rt) *rt = a;
+	if (splitk) *splitk = b;
+	return RAD_OK;
+}
+
+rad_status rad_lin_set_tile(rad_lin *l, int nrhs, int prec, int rt, int splitk)
+{
+	if (!l) return RAD_EINVAL;
+	return l->sys->set_tile(nrhs, prec, rt, splitk, l->err) ? RAD_EINVAL : RAD_OK;
+}
+
+rad_status rad_lin_gemm_ms(rad_lin *l, int nrhs, int prec, int reps, double *ms)
+{
+	if (!l || !ms || nrhs <= 0 || nrhs > 192 || reps <= 0 || (prec != RAD_PREC_F16 && prec != RAD_PREC_BF16))
+		return l ? lfail(l, RAD_EINVAL, "gemm_ms: bad arguments") : RAD_EINVAL;
+	return l->sys->gemm_ms(nrhs, prec, reps, ms, l->err) ? RAD_EDEVICE : RAD_OK;
+}
+
+rad_status rad_lin_multiply(rad_lin *l, int prec, int nrhs, const double *X, double *Y)
+{
+	if (!l || !X || !Y || nrhs <= 0 || nrhs > 192 || prec < RAD_PREC_F32 || prec > RAD_PREC_BF16)
+		return l ? lfail(l, RAD_EINVAL, "multiply: bad arguments") : RAD_EINVAL;
+	return l->sys->multiply(prec, nrhs, X, Y, l->err) ? RAD_EDEVICE : RAD_OK;
+}
+
+rad_status rad_lin_gather(rad_lin *l, int prec, int nrhs, const double *rho, const double *E, double tol,
+                          int max_outer, int sweeps, const double *Bref, double target, double *B, double *info)
+{
+	if (!l || !rho || !E || !B || nrhs <= 0 || nrhs > 192 || sweeps <= 0 || max_outer < 0 ||
+	    (prec != RAD_PREC_F16 && prec != RAD_PREC_BF16))
+		return l ? lfail(l, RAD_EINVAL, "gather: bad arguments") : RAD_EINVAL;
+	return l->sys->gather(prec, nrhs, rho, E, tol, max_outer, sweeps, Bref, target, B, info, l->err) ? RAD_EDEVICE
+	                                                                                                    : RAD_OK;
+}
+
+rad_status rad_lin_shoot(rad_lin *l, int nrhs, const double *rho, const double *E, const double *Bref, double target,
+                         int max_shots, double *B, double *info)
+{
+	if (!l || !rho || !E || !B || nrhs <= 0 || nrhs > 16 || max_shots < 0)
+		return l ? lfail(l, RAD_EINVAL, "shoot: bad arguments") : RAD_EINVAL;
+	return l->sys->shoot(nrhs, rho, E, Bref, target, max_shots, B, info, l->err) ? RAD_EDEVICE : RAD_OK;
+}
+
+rad_status rad_lin_adjoint(rad_lin *l, int nrhs, const double *rho, const double *B, const double *dLdB, double tol,
+                           int max_outer, int sweeps, double *dLdE, double *dLdrho, double *info)
+{
+	if (!l || !rho || !B || !dLdB || !dLdE || !dLdrho || nrhs <= 0 || nrhs > 4 || sweeps <= 0 || max_outer < 0)
+		return l ? lfail(l, RAD_EINVAL, "adjoint: bad arguments") : RAD_EINVAL;
+	return l->sys->adjoint(nrhs, rho, B, dLdB, tol, max_outer, sweeps, dLdE, dLdrho, info, l->err) ? RAD_EDEVICE
+	                                                                                                : RAD_OK;
 }
 
 }  // extern "C"

@@ -23,6 +23,8 @@ enum {
 
 // One ray for the cast entry point: origin, direction, banned and target
 // original (RayHitted's arguments, raycast.cpp:335).
+enum { VIS_SEGMENT = 0, VIS_REFERENCE = 1 };
+
 struct RayIn {
 	float p[3], v[3];
 	int ban, target;
@@ -43,6 +45,13 @@ public:
 	int cast(int n, const RayIn *rays, int *hit, int *cells, int *tests, std::string &err);
 	int ff_pairs(int n, const int *pairs, float *ff3, std::string &err);
 	void counters(long long *out);
+	// Explicit form factors of slots [0, n) into the device matrix dF
+	// (row-major, leading dimension ld): CalFF's value from each centroid to
+	// each whole patch, times the visibility vis (VIS_SEGMENT: the segment
+	// between the centroids is clear, k_ff_seg; VIS_REFERENCE: RayHitted from
+	// the centroid returns the patch, k_ff_rows); rows scaled to sum <= 1.
+	// VIS_SEGMENT builds t's segment index on first use.
+	int form_factors(float *dF, int n, int ld, int vis, FlatOctree &t, const HostScene &s, std::string &err);
 	double shoot_ms() const { return shoot_ms_; }  // device time of the shooting kernels so far
 	double kernel_ms() const { return kernel_ms_; }  // device time of the fused shoot kernel alone
 	long long kernel_launches() const { return kernel_launches_; }

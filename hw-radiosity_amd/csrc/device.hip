@@ -182,6 +182,11 @@ __device__ int walk(const DScene &S, float px, float py, float pz, float vx, flo
 {
 	float t = 0.f, qx = px, qy = py, qz = pz;
 	int prev = -1;
+	// Brent's cycle detection over the leaves searched: the walk's next leaf
+	// is a function of the leaf it leaves (the hit test and the exit point
+	// use only the ray and that leaf's box), so a leaf searched twice starts
+	// a cycle the reference never leaves.
+	int tort = -1, power = 1, lam = 0;
 	const bool use_x = fabsf(vx) > S.thr_v, use_y = fabsf(vy) > S.thr_v, use_z = fabsf(vz) > S.thr_v;
 	for (int steps = 0;;) {
 		if (qx < S.g0x || qx > S.g1x || qy < S.g0y || qy > S.g1y || qz < S.g0z || qz > S.g1z) return -1;
@@ -256,6 +261,8 @@ __device__ int walk(const DScene &S, float px, float py, float pz, float vx, flo
 			b = bn;
 		}
 		if (best >= 0) return best;
+		if (leaf == tort) { c.stalled++; return -2; }
+		if (++lam == power) { tort = leaf; power <<= 1; lam = 0; }
 		prev = leaf;
 		float tm = 1000000.0f;
 		if (use_x) {
@@ -669,6 +676,177 @@ __global__ __launch_bounds__(kBlock) void k_update(DRecv R, Loop *L, int k, cons
 	}
 }
 
+
+// Explicit form factors (the linear system of the gathering solvers):
+// F[i][j] = CalFF(source = whole patch j, receiver = patch i, point = the
+// centroid of i) with its visibility ray (shade.cpp:107-127), 0 on the
+// diagonal and for logical slots; a row whose sum exceeds 1 is scaled to 1.
+// One block per row: the row's rays share their origin.  F is row-major
+// with leading dimension ld (>= n); columns [n, ld) are left untouched.
+template <bool F>
+__global__ __launch_bounds__(kBlock) void k_ff_rows(DScene S, DRecv R, int n, int ld, float *Fm, long long *bcnt)
+{
+	__shared__ double wsum[kWavesPerBlock];
+	const int i = blockIdx.x;
+	if (i >= n) return;
+	const float4 ni = R.nrm[i], ci = R.cA[i];
+	const int pi = __float_as_int(ni.w);
+	const float nlen_i = __builtin_sqrtf(dot3<F>(ni.x, ni.y, ni.z, ni.x, ni.y, ni.z));
+	float *row = Fm + (size_t)i * ld;
+	Cnt c = {0, 0, 0, 0, 0, 0, 0};
+	double sum = 0.0;
+	for (int j = threadIdx.x; j < n; j += kBlock) {
+		float ff = 0.0f;
+		const float4 nj = R.nrm[j], cj = R.cA[j];
+		const int pj = __float_as_int(nj.w);
+		if (j != i && pi >= 0 && pj >= 0) {
+			SrcPatch sp;
+			sp.c[0] = cj.x; sp.c[1] = cj.y; sp.c[2] = cj.z;
+			sp.n[0] = nj.x; sp.n[1] = nj.y; sp.n[2] = nj.z;
+			sp.area = cj.w;
+			sp.nlen = __builtin_sqrtf(dot3<F>(nj.x, nj.y, nj.z, nj.x, nj.y, nj.z));
+			ff = point_ff<F>(S, sp, pj, ni.x, ni.y, ni.z, nlen_i, pi, ci.x, ci.y, ci.z, c);
+		}
+		row[j] = ff;
+		sum += (double)ff;
+	}
+	for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+	if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = sum;
+	__syncthreads();
+	const double total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+	if (total > 1.0) {
+		const float scale = (float)(1.0 / total);
+		for (int j = threadIdx.x; j < n; j += kBlock) row[j] *= scale;
+	}
+	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
+	add_block_counters(vals, bcnt);
+}
+
+// ---- explicit matrix with segment visibility -------------------------------
+// The gathering matrix asks whether anything blocks the open segment between
+// two centroids.  Point location is geometric (the 2^D grid of the deepest
+// octree level, each cell mapped to its leaf), the walk is Amanatides-Woo over
+// that grid (at most 3 * 2^D steps), and each leaf's complete list is tested
+// once (a leaf is a box, so its cells are consecutive along a segment).  The
+// test of one original is TriHitted's plane and side-plane arithmetic with the
+// segment's parameter range (1e-4, 1 - 1e-4).
+constexpr float kSegLo = 0.0001f, kSegHi = 0.9999f;
+
+struct DFine {
+	const int *__restrict__ leaf;        // R^3 cell -> leaf
+	const int *__restrict__ begin;       // leaf -> first record, n_leaves + 1
+	const float4 *__restrict__ nc;       // 2 per record: {n, c.x}, {c.y, c.z, id, -}
+	const float4 *__restrict__ se;       // 3 per record: side planes
+	int r;
+	float g0[3], h[3], inv[3];
+};
+
+template <bool F>
+__device__ bool seg_occluded(const DFine &G, float px, float py, float pz, float vx, float vy, float vz, int ban0,
+                             int ban1, Cnt &c)
+{
+	const int R = G.r;
+	const float p[3] = {px, py, pz}, v[3] = {vx, vy, vz};
+	int cell[3], step[3];
+	float tmax[3], tdelta[3];
+#pragma unroll
+	for (int k = 0; k < 3; k++) {
+		const float rel = p[k] - G.g0[k];
+		int ck = (int)floorf(rel * G.inv[k]);
+		ck = ck < 0 ? 0 : (ck >= R ? R - 1 : ck);
+		cell[k] = ck;
+		if (v[k] > 0.0f) {
+			step[k] = 1;
+			tmax[k] = ((float)(ck + 1) * G.h[k] - rel) / v[k];
+			tdelta[k] = G.h[k] / v[k];
+		} else if (v[k] < 0.0f) {
+			step[k] = -1;
+			tmax[k] = ((float)ck * G.h[k] - rel) / v[k];
+			tdelta[k] = -G.h[k] / v[k];
+		} else {
+			step[k] = 0;
+			tmax[k] = tdelta[k] = __builtin_inff();
+		}
+	}
+	c.rays++;
+	int last = -1;
+	for (;;) {
+		const int leaf = G.leaf[((size_t)cell[0] * R + cell[1]) * R + cell[2]];
+		if (leaf != last) {
+			last = leaf;
+			c.cells++;
+			const int e1 = G.begin[leaf + 1];
+			for (int e = G.begin[leaf]; e < e1; e++) {
+				const float4 a = G.nc[2 * (size_t)e], b = G.nc[2 * (size_t)e + 1];
+				const int id = __float_as_int(b.z);
+				if (id == ban0 || id == ban1) continue;
+				c.tests++;
+				const float a1 = dot3<F>(vx, vy, vz, a.x, a.y, a.z);
+				const float a2 = dot3<F>(a.w - px, b.x - py, b.y - pz, a.x, a.y, a.z);
+				const float tt = a2 / a1;
+				if (!(tt > kSegLo && tt < kSegHi)) continue;
+				const float hx = axpy<F>(tt, vx, px), hy = axpy<F>(tt, vy, py), hz = axpy<F>(tt, vz, pz);
+				const float4 *se = G.se + 3 * (size_t)e;
+				const float4 s0 = se[0], s1 = se[1], s2 = se[2];
+				if (!(dot3<F>(hx, hy, hz, s0.x, s0.y, s0.z) + s0.w < 0.0f) &&
+				    !(dot3<F>(hx, hy, hz, s1.x, s1.y, s1.z) + s1.w < 0.0f) &&
+				    !(dot3<F>(hx, hy, hz, s2.x, s2.y, s2.z) + s2.w < 0.0f))
+					return true;
+			}
+		}
+		const int ax = tmax[0] <= tmax[1] ? (tmax[0] <= tmax[2] ? 0 : 2) : (tmax[1] <= tmax[2] ? 1 : 2);
+		if (tmax[ax] >= 1.0f) return false;
+		cell[ax] += step[ax];
+		if ((unsigned)cell[ax] >= (unsigned)R) return false;
+		tmax[ax] += tdelta[ax];
+	}
+}
+
+// Row i (one block): F[i][j] = CalFF's value from the centroid of i to the
+// whole patch j (shade.cpp:107-123) when the segment between the centroids is
+// clear; 0 on the diagonal and for logical slots; the row scaled by
+// (float)(1 / sum) when its double sum (fixed reduction order) exceeds 1.
+template <bool F>
+__global__ __launch_bounds__(kBlock) void k_ff_seg(DFine G, DRecv R, int n, int ld, float *Fm, long long *bcnt)
+{
+	__shared__ double wsum[kWavesPerBlock];
+	const int i = blockIdx.x;
+	if (i >= n) return;
+	const float4 ni = R.nrm[i], ci = R.cA[i];
+	const int pi = __float_as_int(ni.w);
+	const float nlen_i = __builtin_sqrtf(dot3<F>(ni.x, ni.y, ni.z, ni.x, ni.y, ni.z));
+	float *row = Fm + (size_t)i * ld;
+	Cnt c = {0, 0, 0, 0, 0, 0, 0};
+	double sum = 0.0;
+	for (int j = threadIdx.x; j < n; j += kBlock) {
+		float ff = 0.0f;
+		const float4 nj = R.nrm[j], cj = R.cA[j];
+		const int pj = __float_as_int(nj.w);
+		if (j != i && pi >= 0 && pj >= 0) {
+			SrcPatch sp;
+			sp.c[0] = cj.x; sp.c[1] = cj.y; sp.c[2] = cj.z;
+			sp.n[0] = nj.x; sp.n[1] = nj.y; sp.n[2] = nj.z;
+			sp.area = cj.w;
+			sp.nlen = __builtin_sqrtf(dot3<F>(nj.x, nj.y, nj.z, nj.x, nj.y, nj.z));
+			ff = ff_prefix<F>(sp, ni.x, ni.y, ni.z, nlen_i, ci.x, ci.y, ci.z);
+			if (ff != 0.0f && seg_occluded<F>(G, ci.x, ci.y, ci.z, cj.x - ci.x, cj.y - ci.y, cj.z - ci.z, pi, pj, c))
+				ff = 0.0f;
+		}
+		row[j] = ff;
+		sum += (double)ff;
+	}
+	for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+	if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = sum;
+	__syncthreads();
+	const double total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+	if (total > 1.0) {
+		const float scale = (float)(1.0 / total);
+		for (int j = threadIdx.x; j < n; j += kBlock) row[j] *= scale;
+	}
+	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
+	add_block_counters(vals, bcnt);
+}
+
 template <bool F>
 __global__ __launch_bounds__(kBlock) void k_cast(DScene S, const RayIn *rays, int n, int *hit, int *cells, int *tests)
 {
@@ -759,6 +937,9 @@ struct DeviceEngine::Impl {
 	DevBuf<float4> lnc, lse, leaf_box, nrm, cA, rho, acc, dacc, dB;
 	DevBuf<int> cell_leaf;
 	DevBuf<float> vx, vy, vz;
+	DevBuf<int> fine_leaf, seg_begin;  // segment-visibility index (explicit matrix)
+	DevBuf<float4> seg_nc, seg_se;
+	bool have_seg = false;
 	DevBuf<Loop> loop;
 	DevBuf<long long> bcnt;
 	DevBuf<unsigned long long> cand;
@@ -819,6 +1000,7 @@ DeviceEngine *DeviceEngine::create(std::string &err)
 
 int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &err)
 {
+	d_->have_seg = false;
 	Impl &d = *d_;
 	const int n0 = s.n_orig, n = s.trinum;
 	d.n_orig = n0;
@@ -1079,6 +1261,77 @@ int DeviceEngine::ff_pairs(int n, const int *pairs, float *ff3, std::string &err
 	HIP_OK(hipGetLastError(), "ff launch");
 	HIP_OK(hipMemcpyAsync(ff3, df.p, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost, st), "ff");
 	HIP_OK(hipStreamSynchronize(st), "ff");
+	return 0;
+}
+
+
+int DeviceEngine::form_factors(float *dF, int n, int ld, int vis, FlatOctree &t, const HostScene &s, std::string &err)
+{
+	Impl &d = *d_;
+	if (n <= 0 || n > d.R.nslots || ld < n) {
+		err = "form_factors: bad size";
+		return -1;
+	}
+	if (vis != VIS_SEGMENT && vis != VIS_REFERENCE) {
+		err = "form_factors: unknown visibility mode";
+		return -1;
+	}
+	hipStream_t st = d.stream;
+	if (vis == VIS_SEGMENT && !d.have_seg) {
+		// the segment test reads the originals' planes; pre-partition
+		// rewrites originals in place when it splits, so it needs none
+		if (s.trinum != s.n_orig) {
+			err = "segment visibility needs a scene that pre-partition did not split (" + std::to_string(s.trinum) +
+			      " slots, " + std::to_string(s.n_orig) + " originals)";
+			return -1;
+		}
+		if (build_segment_index(t, s.pool.data(), s.n_orig, s.G0, s.G1, err)) return -1;
+		const size_t ne = t.seg_items.size();
+		std::vector<float4> nc(2 * ne), se(3 * ne);
+		for (size_t e = 0; e < ne; e++) {
+			const Patch &p = s.pool[t.seg_items[e]];
+			float idf;
+			std::memcpy(&idf, &t.seg_items[e], 4);
+			nc[2 * e] = make_float4(p.n[0], p.n[1], p.n[2], p.c[0]);
+			nc[2 * e + 1] = make_float4(p.c[1], p.c[2], idf, 0.f);
+			for (int q = 0; q < 3; q++) se[3 * e + q] = make_float4(p.se[q][0], p.se[q][1], p.se[q][2], p.se[q][3]);
+		}
+		if (d.fine_leaf.put(t.fine_leaf.data(), t.fine_leaf.size(), st, err) ||
+		    d.seg_begin.put(t.seg_begin.data(), t.seg_begin.size(), st, err) ||
+		    d.seg_nc.put(nc.data(), nc.size(), st, err) || d.seg_se.put(se.data(), se.size(), st, err))
+			return -1;
+		HIP_OK(hipStreamSynchronize(st), "segment index upload");
+		d.have_seg = true;
+	}
+	if (d.bcnt.n < (size_t)n * 8) {
+		std::vector<long long> zeros((size_t)n * 8, 0);
+		if (d.bcnt.put(zeros.data(), zeros.size(), st, err)) return -1;
+		HIP_OK(hipStreamSynchronize(st), "counter upload");
+	}
+	HIP_OK(hipEventRecord(d.ev0, st), "event");
+	if (vis == VIS_SEGMENT) {
+		DFine G;
+		G.leaf = d.fine_leaf.p;
+		G.begin = d.seg_begin.p;
+		G.nc = d.seg_nc.p;
+		G.se = d.seg_se.p;
+		G.r = t.fine_r;
+		for (int k = 0; k < 3; k++) {
+			G.g0[k] = s.G0[k];
+			G.h[k] = (s.G1[k] - s.G0[k]) / (float)t.fine_r;
+			G.inv[k] = (float)t.fine_r / (s.G1[k] - s.G0[k]);
+		}
+		if (d.fma) hipLaunchKernelGGL(k_ff_seg<true>, dim3(n), dim3(kBlock), 0, st, G, d.R, n, ld, dF, d.bcnt.p);
+		else hipLaunchKernelGGL(k_ff_seg<false>, dim3(n), dim3(kBlock), 0, st, G, d.R, n, ld, dF, d.bcnt.p);
+	} else {
+		if (d.fma) hipLaunchKernelGGL(k_ff_rows<true>, dim3(n), dim3(kBlock), 0, st, d.S, d.R, n, ld, dF, d.bcnt.p);
+		else hipLaunchKernelGGL(k_ff_rows<false>, dim3(n), dim3(kBlock), 0, st, d.S, d.R, n, ld, dF, d.bcnt.p);
+	}
+	HIP_OK(hipGetLastError(), "form factor launch");
+	HIP_OK(hipEventRecord(d.ev1, st), "event");
+	HIP_OK(hipStreamSynchronize(st), "form factors");
+	float ms = 0;
+	if (hipEventElapsedTime(&ms, d.ev0, d.ev1) == hipSuccess) cast_ms_ = ms;
 	return 0;
 }
 

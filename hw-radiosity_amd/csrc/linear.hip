@@ -1,0 +1,1228 @@
+// Linear radiosity solvers on an explicit form-factor matrix (see linear.h).
+//
+// HBM layout
+//   F32   ld x ld floats, row-major, zero padded (ld = n rounded up to 256);
+//         the matrix of record: FP64 residuals and the FP32 solvers read it.
+//   F16 / BF16  s * F in MFMA fragment order, made on demand: the matrix is
+//         cut into 16 x 32 tiles, tile (rb, kb) at ((rb * ld/32) + kb) * 512
+//         elements, and inside a tile lane l's 8 elements (row rb*16 + l%16,
+//         columns kb*32 + 8*(l/16) .. +7) at l*8.  One wave-wide 16-byte load
+//         per lane fetches a whole A fragment as 1 KB of contiguous HBM.
+//         s is a power of two that lifts the largest |F| to ~2^14 for FP16
+//         (form factors of 1e-6 would otherwise be FP16 subnormals) and 1 for
+//         BF16; the product is scaled back by 1/s exactly.
+//   XT    the low-precision iterate, nrhs (rounded up to 16) x ld, in the
+//         same fragment order (it is the MFMA B operand).
+//
+// MFMA (v_mfma_f32_16x16x32_{f16,bf16}, gfx950): lane l supplies
+// A[l%16][8*(l/16)+j] and B[8*(l/16)+j][l%16], j = 0..7, and receives
+// D[4*(l/16)+j][l%16], j = 0..3.  A wave owns 16*RT rows and all 16*CT
+// right-hand-side columns, so F is read from HBM once per sweep; split-K S
+// spreads the K loop over S blocks whose partial tiles the epilogue sums in a
+// fixed order (results do not depend on timing).  (RT, S) per (CT, precision)
+// come from the autotuner, which times GEMM + epilogue for each candidate.
+#include "linear.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+namespace rgpu {
+namespace {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kB = 256;
+
+__device__ __forceinline__ f4 mfma(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma(b8 a, b8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+template <typename E> struct VecOf;
+template <> struct VecOf<_Float16> { typedef h8 type; };
+template <> struct VecOf<__bf16> { typedef b8 type; };
+
+// element (row, k) of a fragment-ordered matrix with ld columns
+__device__ __forceinline__ size_t frag_index(int row, int k, int ld)
+{
+	return ((size_t)(row >> 4) * (ld >> 5) + (k >> 5)) * 512 + ((((k & 31) >> 3) << 4) + (row & 15)) * 8 + (k & 7);
+}
+
+__global__ void k_absmax(const float *F, size_t count, unsigned *out)
+{
+	float m = 0.f;
+	for (size_t i = blockIdx.x * (size_t)kB + threadIdx.x; i < count; i += (size_t)gridDim.x * kB) m = fmaxf(m, fabsf(F[i]));
+	for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+	if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));  // non-negative floats order as their bits
+}
+
+// one thread per 8-element fragment row: reads 8 consecutive floats of F32
+template <typename E> __global__ void k_to_frag(const float *F, int ld, float s, E *out)
+{
+	typedef typename VecOf<E>::type V;
+	const size_t units = (size_t)ld * ld / 8;
+	for (size_t u = blockIdx.x * (size_t)kB + threadIdx.x; u < units; u += (size_t)gridDim.x * kB) {
+		const int row = (int)(u / (ld / 8)), k = (int)(u % (ld / 8)) * 8;
+		const float4 a = *(const float4 *)(F + (size_t)row * ld + k);
+		const float4 b = *(const float4 *)(F + (size_t)row * ld + k + 4);
+		V v;
+		v[0] = (E)(a.x * s); v[1] = (E)(a.y * s); v[2] = (E)(a.z * s); v[3] = (E)(a.w * s);
+		v[4] = (E)(b.x * s); v[5] = (E)(b.y * s); v[6] = (E)(b.z * s); v[7] = (E)(b.w * s);
+		*(V *)(out + frag_index(row, k, ld)) = v;
+	}
+}
+
+template <typename T> __global__ void k_fill(T *p, size_t count, T v)
+{
+	for (size_t i = blockIdx.x * (size_t)kB + threadIdx.x; i < count; i += (size_t)gridDim.x * kB) p[i] = v;
+}
+
+// part[s][row][col] = sum over the s-th K chunk of A[row][k] X[k][col].
+// Software pipelined: two register sets of U k-steps each, the loads of one
+// in flight while the other feeds the MFMAs (the chunk holds a multiple of 2U
+// k-steps).
+template <typename E, int RT, int CT, int U> struct GemmRegs {
+	typedef typename VecOf<E>::type V;
+	V a[U][RT], b[U][CT];
+};
+
+template <typename E, int RT, int CT, int U>
+__device__ __forceinline__ void gemm_load(GemmRegs<E, RT, CT, U> &g, const typename VecOf<E>::type *Ap,
+                                          const typename VecOf<E>::type *Bp, int rb0, int nkb, int kb)
+{
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+#pragma unroll
+		for (int r = 0; r < RT; r++) g.a[u][r] = __builtin_nontemporal_load(Ap + ((size_t)(rb0 + r) * nkb + kb + u) * 64);
+#pragma unroll
+		for (int c = 0; c < CT; c++) g.b[u][c] = Bp[((size_t)c * nkb + kb + u) * 64];
+	}
+}
+
+template <typename E, int RT, int CT, int U>
+__device__ __forceinline__ void gemm_mma(const GemmRegs<E, RT, CT, U> &g, f4 (&acc)[RT][CT])
+{
+#pragma unroll
+	for (int u = 0; u < U; u++)
+#pragma unroll
+		for (int r = 0; r < RT; r++)
+#pragma unroll
+			for (int c = 0; c < CT; c++) acc[r][c] = mfma(g.a[u][r], g.b[u][c], acc[r][c]);
+}
+
+template <int CT> struct GemmUnroll { static constexpr int U = CT <= 4 ? 4 : 2; };
+
+template <typename E, int RT, int CT>
+__global__ __launch_bounds__(kB) void k_gemm(const E *__restrict__ A, const E *__restrict__ XT, float *__restrict__ part,
+                                             int ld, int rp, int kchunk)
+{
+	typedef typename VecOf<E>::type V;
+	constexpr int U = GemmUnroll<CT>::U;
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const int rb0 = (blockIdx.x * 4 + w) * RT;
+	const int s = blockIdx.y, kb0 = s * (kchunk >> 5), kb1 = kb0 + (kchunk >> 5);
+	const int nkb = ld >> 5;
+	const V *Ap = (const V *)A + lane;
+	const V *Bp = (const V *)XT + lane;
+	f4 acc[RT][CT];
+#pragma unroll
+	for (int r = 0; r < RT; r++)
+#pragma unroll
+		for (int c = 0; c < CT; c++) acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
+	GemmRegs<E, RT, CT, U> g0, g1;
+	gemm_load(g0, Ap, Bp, rb0, nkb, kb0);
+	for (int kb = kb0; kb < kb1; kb += 2 * U) {
+		gemm_load(g1, Ap, Bp, rb0, nkb, kb + U);
+		gemm_mma(g0, acc);
+		if (kb + 2 * U < kb1) gemm_load(g0, Ap, Bp, rb0, nkb, kb + 2 * U);
+		gemm_mma(g1, acc);
+	}
+	const int lr = lane & 15;
+#pragma unroll
+	for (int r = 0; r < RT; r++)
+#pragma unroll
+		for (int c = 0; c < CT; c++)
+#pragma unroll
+			for (int j = 0; j < 4; j++) {
+				const int row = (rb0 + r) * 16 + (lane >> 4) * 4 + j;
+				part[((size_t)s * ld + row) * rp + c * 16 + lr] = acc[r][c][j];
+			}
+}
+
+// d = rhs + rho o (inv_s * sum_s part); XT <- d (fragment order).
+// Padding of XT (rows >= nrhs, k >= n) is never written and stays zero.
+template <typename E>
+__global__ void k_jacobi(const float *part, int S, int ld, int rp, int n, int nrhs, float inv_s, const float *rhs,
+                         const float *rho, float *d, E *XT)
+{
+	const size_t total = (size_t)n * nrhs;
+	for (size_t t = blockIdx.x * (size_t)kB + threadIdx.x; t < total; t += (size_t)gridDim.x * kB) {
+		const int i = (int)(t / nrhs), c = (int)(t % nrhs);
+		float y = 0.f;
+		for (int s = 0; s < S; s++) y += part[((size_t)s * ld + i) * rp + c];
+		const float v = rhs[t] + rho[t] * (y * inv_s);
+		d[t] = v;
+		XT[frag_index(c, i, ld)] = (E)v;
+	}
+}
+
+// FP64 product with the FP32 matrix, RB rows x CC columns per block:
+//   out[i][c] = base[i][c] + rho[i][c] * sum_k F[i][k] xT[c][k] - beta * xT[c][i]
+// (base, rho may be null: 0 and 1).  Per block: sum of squares and max |out|.
+template <int RB, int CC>
+__global__ __launch_bounds__(kB) void k_fx64(const float *__restrict__ F, int ld, int n, int nrhs,
+                                             const double *__restrict__ xT, const double *base, const double *rho,
+                                             double beta, double *out, double *blk)
+{
+	__shared__ double red[4][RB * CC];
+	const int i0 = blockIdx.x * RB, c0 = blockIdx.y * CC;
+	const int nc = min(CC, nrhs - c0);
+	double acc[RB][CC];
+#pragma unroll
+	for (int r = 0; r < RB; r++)
+#pragma unroll
+		for (int c = 0; c < CC; c++) acc[r][c] = 0.0;
+	for (int k = threadIdx.x; k < n; k += kB) {
+		double xv[CC];
+#pragma unroll
+		for (int c = 0; c < CC; c++) xv[c] = c < nc ? xT[(size_t)(c0 + c) * ld + k] : 0.0;
+#pragma unroll
+		for (int r = 0; r < RB; r++) {
+			const double f = (double)F[(size_t)(i0 + r) * ld + k];
+#pragma unroll
+			for (int c = 0; c < CC; c++) acc[r][c] += f * xv[c];
+		}
+	}
+#pragma unroll
+	for (int r = 0; r < RB; r++)
+#pragma unroll
+		for (int c = 0; c < CC; c++) {
+			double v = acc[r][c];
+			for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+			if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][r * CC + c] = v;
+		}
+	__syncthreads();
+	if (threadIdx.x < 64) {
+		double ss = 0.0, mx = 0.0;
+		const int r = threadIdx.x / CC, c = threadIdx.x % CC;
+		if (threadIdx.x < RB * CC && i0 + r < n && c < nc) {
+			const int q = r * CC + c;
+			const double fx = red[0][q] + red[1][q] + red[2][q] + red[3][q];
+			const size_t t = (size_t)(i0 + r) * nrhs + c0 + c;
+			const double v = (base ? base[t] : 0.0) + (rho ? rho[t] : 1.0) * fx - beta * xT[(size_t)(c0 + c) * ld + i0 + r];
+			out[t] = v;
+			ss = v * v;
+			mx = fabs(v);
+		}
+		for (int off = 32; off > 0; off >>= 1) {
+			ss += __shfl_xor(ss, off);
+			mx = fmax(mx, __shfl_xor(mx, off));
+		}
+		if (threadIdx.x == 0) {
+			const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+			blk[2 * b] = ss;
+			blk[2 * b + 1] = mx;
+		}
+	}
+}
+
+// Transposed product split over row chunks, for nrhs <= 4:
+//   part[s][j][c] = sum over rows i of chunk s of F[i][j] * rho[i][c] * x[i][c]
+template <typename T>
+__global__ __launch_bounds__(kB) void k_ftx(const float *__restrict__ F, int ld, int n, int nrhs, const T *__restrict__ x,
+                                            const T *__restrict__ rho, int rows_per_split, T *part)
+{
+	const int j = blockIdx.x * kB + threadIdx.x;
+	const int s = blockIdx.y, i0 = s * rows_per_split, i1 = min(n, i0 + rows_per_split);
+	if (j >= n) return;
+	T acc[4] = {0, 0, 0, 0};
+	for (int i = i0; i < i1; i++) {
+		const T f = (T)F[(size_t)i * ld + j];
+#pragma unroll
+		for (int c = 0; c < 4; c++)
+			if (c < nrhs) acc[c] += f * (rho[(size_t)i * nrhs + c] * x[(size_t)i * nrhs + c]);
+	}
+	for (int c = 0; c < nrhs; c++) part[((size_t)s * n + j) * nrhs + c] = acc[c];
+}
+
+// out = base + sum_s part - beta * x; per block sum of squares and max |out|
+template <typename T>
+__global__ __launch_bounds__(kB) void k_ftx_finish(const T *part, int S, int n, int nrhs, const T *base, const T *x,
+                                                   T beta, T *out, double *blk)
+{
+	const size_t total = (size_t)n * nrhs;
+	const size_t t = blockIdx.x * (size_t)kB + threadIdx.x;
+	double ss = 0.0, mx = 0.0;
+	if (t < total) {
+		T y = 0;
+		for (int s = 0; s < S; s++) y += part[(size_t)s * total + t];
+		const T v = base[t] + y - beta * x[t];
+		out[t] = v;
+		ss = (double)v * (double)v;
+		mx = fabs((double)v);
+	}
+	for (int off = 32; off > 0; off >>= 1) {
+		ss += __shfl_xor(ss, off);
+		mx = fmax(mx, __shfl_xor(mx, off));
+	}
+	__shared__ double r[8];
+	if ((threadIdx.x & 63) == 0) {
+		r[2 * (threadIdx.x >> 6)] = ss;
+		r[2 * (threadIdx.x >> 6) + 1] = mx;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		blk[2 * blockIdx.x] = r[0] + r[2] + r[4] + r[6];
+		blk[2 * blockIdx.x + 1] = fmax(fmax(r[1], r[3]), fmax(r[5], r[7]));
+	}
+}
+
+// r32 = r64 / s; XT (if given) <- r32 in fragment order
+template <typename E>
+__global__ void k_scale(const double *r64, double inv_s, size_t total, int nrhs, int ld, float *r32, E *XT)
+{
+	for (size_t t = blockIdx.x * (size_t)kB + threadIdx.x; t < total; t += (size_t)gridDim.x * kB) {
+		const float v = (float)(r64[t] * inv_s);
+		r32[t] = v;
+		if (XT) XT[frag_index((int)(t % nrhs), (int)(t / nrhs), ld)] = (E)v;
+	}
+}
+
+// xT[c][i] += s * d[i][c]
+__global__ void k_axpy_t(double *xT, int ld, const float *d, double s, int n, int nrhs)
+{
+	const size_t total = (size_t)n * nrhs;
+	for (size_t t = blockIdx.x * (size_t)kB + threadIdx.x; t < total; t += (size_t)gridDim.x * kB)
+		xT[(size_t)(t % nrhs) * ld + t / nrhs] += s * (double)d[t];
+}
+__global__ void k_axpy(double *x, const float *d, double s, size_t total)
+{
+	for (size_t t = blockIdx.x * (size_t)kB + threadIdx.x; t < total; t += (size_t)gridDim.x * kB) x[t] += s * (double)d[t];
+}
+
+// per block: sum (x - ref)^2 and sum ref^2 (x transposed [c][ld] or row-major)
+__global__ void k_err(const double *x, int transposed, int ld, const double *ref, int n, int nrhs, double *blk)
+{
+	const size_t total = (size_t)n * nrhs;
+	double a = 0.0, b = 0.0;
+	for (size_t t = blockIdx.x * (size_t)kB + threadIdx.x; t < total; t += (size_t)gridDim.x * kB) {
+		const double xv = transposed ? x[(size_t)(t % nrhs) * ld + t / nrhs] : x[t];
+		const double e = xv - ref[t];
+		a += e * e;
+		b += ref[t] * ref[t];
+	}
+	for (int off = 32; off > 0; off >>= 1) {
+		a += __shfl_xor(a, off);
+		b += __shfl_xor(b, off);
+	}
+	__shared__ double r[8];
+	if ((threadIdx.x & 63) == 0) {
+		r[2 * (threadIdx.x >> 6)] = a;
+		r[2 * (threadIdx.x >> 6) + 1] = b;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		blk[2 * blockIdx.x] = r[0] + r[2] + r[4] + r[6];
+		blk[2 * blockIdx.x + 1] = r[1] + r[3] + r[5] + r[7];
+	}
+}
+
+// ---- plain progressive shooting (FP32) --------------------------------------
+// One launch per shot.  The launch reads its source from the state (chosen
+// by the previous launch), adds rho_i F_ij dB_j to B_i and dB_i for every
+// patch i, and folds the next source's key (largest unshot power
+// area_i * sum_c dB_i[c], ties to the lowest index) into next_key; the last
+// block to finish (ticket counter) takes the next source: it copies and zeroes
+// its dB and resets the key for the launch after.
+constexpr int kMaxShootRhs = 16;
+struct ShootState {
+	unsigned long long next_key;
+	unsigned ticket;
+	int done, shots, src;
+	float dB[kMaxShootRhs];
+};
+
+__device__ __forceinline__ unsigned long long power_key(float p, int j)
+{
+	// p > 0 here, so its bits order as the value
+	return ((unsigned long long)__float_as_uint(p) << 32) | (0xffffffffu - (uint32_t)j);
+}
+
+// block max of key, folded into *target by thread 0
+__device__ __forceinline__ void block_max_key(unsigned long long key, unsigned long long *target)
+{
+	__shared__ unsigned long long wk[4];
+	for (int off = 32; off > 0; off >>= 1) {
+		const unsigned long long o = __shfl_xor(key, off);
+		key = o > key ? o : key;
+	}
+	if ((threadIdx.x & 63) == 0) wk[threadIdx.x >> 6] = key;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int w = 1; w < 4; w++) key = wk[w] > key ? wk[w] : key;
+		if (key) atomicMax(target, key);
+	}
+}
+
+__device__ void take_next_source(float *dB, int nrhs, ShootState *st)
+{
+	const unsigned long long key = atomicAdd(&st->next_key, 0ull);
+	st->next_key = 0;
+	st->ticket = 0;
+	if (!key) {
+		st->done = 1;
+		return;
+	}
+	const int j = (int)(0xffffffffu - (uint32_t)key);
+	for (int c = 0; c < nrhs; c++) {
+		st->dB[c] = dB[(size_t)j * nrhs + c];
+		dB[(size_t)j * nrhs + c] = 0.f;
+	}
+	st->src = j;
+}
+
+__device__ __forceinline__ void shoot_epilogue(float *dB, int nrhs, ShootState *st)
+{
+	__shared__ bool last;
+	__threadfence();
+	__syncthreads();
+	if (threadIdx.x == 0) last = atomicAdd(&st->ticket, 1u) == gridDim.x - 1;
+	__syncthreads();
+	if (last && threadIdx.x == 0) {
+		__threadfence();
+		take_next_source(dB, nrhs, st);
+	}
+}
+
+// first selection: keys of the initial dB
+__global__ __launch_bounds__(kB) void k_lsel(float *dB, const float *area, int n, int nrhs, ShootState *st)
+{
+	const int i = blockIdx.x * kB + threadIdx.x;
+	unsigned long long key = 0;
+	if (i < n) {
+		float p = 0.f;
+		for (int c = 0; c < nrhs; c++) p += dB[(size_t)i * nrhs + c];
+		p *= area[i];
+		if (p > 0.f) key = power_key(p, i);
+	}
+	block_max_key(key, &st->next_key);
+	shoot_epilogue(dB, nrhs, st);
+}
+
+__global__ __launch_bounds__(kB) void k_lshoot(const float *__restrict__ Ft, int ld, const float *rho, const float *area,
+                                               float *B, float *dB, int n, int nrhs, ShootState *st)
+{
+	if (st->done) return;  // uniform: written only by the previous launch
+	const int i = blockIdx.x * kB + threadIdx.x;
+	const int j = st->src;
+	unsigned long long key = 0;
+	if (i < n) {
+		const float f = Ft[(size_t)j * ld + i];
+		float p = 0.f;
+		for (int c = 0; c < nrhs; c++) {
+			const size_t t = (size_t)i * nrhs + c;
+			const float v = rho[t] * f * st->dB[c];
+			B[t] += v;
+			const float nd = dB[t] + v;
+			dB[t] = nd;
+			p += nd;
+		}
+		p *= area[i];
+		if (p > 0.f) key = power_key(p, i);
+	}
+	if (threadIdx.x == 0 && blockIdx.x == 0) st->shots++;
+	block_max_key(key, &st->next_key);
+	shoot_epilogue(dB, nrhs, st);
+}
+
+__global__ void k_transpose(const float *F, float *Ft, int ld)
+{
+	__shared__ float tile[32][33];
+	const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+	const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+	for (int r = ty; r < 32; r += 8) tile[r][tx] = F[(size_t)(by + r) * ld + bx + tx];
+	__syncthreads();
+	for (int r = ty; r < 32; r += 8) Ft[(size_t)(bx + r) * ld + by + tx] = tile[tx][r];
+}
+
+#define HIP_OK(expr, what)                                          \
+	do {                                                            \
+		hipError_t e_ = (expr);                                     \
+		if (e_ != hipSuccess) {                                     \
+			err = std::string(what) + ": " + hipGetErrorString(e_); \
+			return -1;                                              \
+		}                                                           \
+	} while (0)
+
+template <typename T> struct Buf {
+	T *p = nullptr;
+	size_t n = 0;
+	Buf() = default;
+	Buf(const Buf &) = delete;
+	Buf &operator=(const Buf &) = delete;
+	~Buf()
+	{
+		if (p) (void)hipFree(p);
+	}
+	int alloc(size_t count, std::string &err)
+	{
+		if (count <= n && p) return 0;
+		if (p) (void)hipFree(p);
+		p = nullptr;
+		n = 0;
+		if (count == 0) return 0;
+		HIP_OK(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+		n = count;
+		return 0;
+	}
+};
+
+int grid_for(size_t total) { return (int)std::min<size_t>((total + kB - 1) / kB, 8192); }
+
+const int kCT[] = {1, 2, 4, 8, 12};
+int pick_ct(int nrhs)
+{
+	const int need = (nrhs + 15) / 16;
+	for (int c : kCT)
+		if (c >= need) return c;
+	return -1;
+}
+
+template <typename E>
+int launch_gemm(int rt, int ct, const E *A, const E *XT, float *part, int ld, int rp, int S, hipStream_t st)
+{
+	const dim3 grid(ld / (64 * rt), S), blk(kB);
+	const int kchunk = ld / S;
+#define G(R, C) hipLaunchKernelGGL((k_gemm<E, R, C>), grid, blk, 0, st, A, XT, part, ld, rp, kchunk)
+#define GR(R)                 \
+	switch (ct) {             \
+	case 1: G(R, 1); break;   \
+	case 2: G(R, 2); break;   \
+	case 4: G(R, 4); break;   \
+	case 8: G(R, 8); break;   \
+	case 12: G(R, 12); break; \
+	default: return -1;       \
+	}
+	if (rt == 1) {
+		GR(1)
+	} else if (rt == 2) {
+		GR(2)
+	} else if (rt == 4 && ct <= 8) {
+		switch (ct) {
+		case 1: G(4, 1); break;
+		case 2: G(4, 2); break;
+		case 4: G(4, 4); break;
+		case 8: G(4, 8); break;
+		default: return -1;
+		}
+	} else {
+		return -1;
+	}
+#undef GR
+#undef G
+	return 0;
+}
+
+bool valid_tile(int rt, int ct, int S, int ld)
+{
+	if (rt == 4 && ct > 8) return false;
+	const int u2 = 2 * (ct <= 4 ? 4 : 2);  // k-steps per pipelined loop trip (GemmUnroll)
+	return ld % (64 * rt) == 0 && ld % (32 * S) == 0 && (ld / 32 / S) % u2 == 0;
+}
+
+double power_sum(const std::vector<float> &a, const double *v, int n, int nrhs)
+{
+	double p = 0.0;
+	for (int i = 0; i < n; i++)
+		for (int c = 0; c < nrhs; c++) p += (double)a[i] * v[(size_t)i * nrhs + c];
+	return p;
+}
+
+double ms_since(std::chrono::steady_clock::time_point a)
+{
+	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+}
+
+}  // namespace
+
+struct LinearSystem::Impl {
+	hipStream_t st = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	Buf<float> F32, Ft, area;
+	Buf<_Float16> F16;
+	Buf<__bf16> FB16;
+	Buf<unsigned> umax;
+	float s16 = 1.f, sb16 = 1.f;
+	bool have16 = false, haveb16 = false, havet = false;
+	std::map<std::pair<int, int>, std::pair<int, int>> tiles;  // (ct, prec) -> (rt, S)
+
+	// low-precision fragment copy of F for prec (made once per matrix)
+	int ensure_lowp(int prec, int ld, std::string &err)
+	{
+		const size_t npp = (size_t)ld * ld;
+		if ((prec == PREC_F16 && have16) || (prec == PREC_BF16 && haveb16)) return 0;
+		if (prec == PREC_F16) {
+			if (umax.alloc(1, err)) return -1;
+			HIP_OK(hipMemsetAsync(umax.p, 0, sizeof(unsigned), st), "memset");
+			hipLaunchKernelGGL(k_absmax, dim3(4096), dim3(kB), 0, st, F32.p, npp, umax.p);
+			unsigned bits = 0;
+			HIP_OK(hipMemcpyAsync(&bits, umax.p, sizeof bits, hipMemcpyDeviceToHost, st), "absmax");
+			HIP_OK(hipStreamSynchronize(st), "absmax");
+			float m;
+			std::memcpy(&m, &bits, sizeof m);
+			if (!std::isfinite(m)) {
+				err = "matrix has non-finite entries";
+				return -1;
+			}
+			int e = 0;
+			if (m > 0.f) e = std::max(-60, std::min(60, (int)std::floor(std::log2(16384.0 / m))));
+			const float s = std::ldexp(1.0f, e);
+			if (F16.alloc(npp, err)) return -1;
+			hipLaunchKernelGGL(k_to_frag<_Float16>, dim3(8192), dim3(kB), 0, st, F32.p, ld, s, F16.p);
+			s16 = s;
+			have16 = true;
+		} else if (prec == PREC_BF16) {
+			if (FB16.alloc(npp, err)) return -1;
+			hipLaunchKernelGGL(k_to_frag<__bf16>, dim3(8192), dim3(kB), 0, st, F32.p, ld, 1.0f, FB16.p);
+			sb16 = 1.f;
+			haveb16 = true;
+		} else {
+			err = "precision must be FP16 or BF16";
+			return -1;
+		}
+		HIP_OK(hipGetLastError(), "convert");
+		return 0;
+	}
+};
+
+LinearSystem::~LinearSystem()
+{
+	if (!d_) return;
+	if (d_->st) (void)hipStreamSynchronize(d_->st);
+	if (d_->e0) (void)hipEventDestroy(d_->e0);
+	if (d_->e1) (void)hipEventDestroy(d_->e1);
+	hipStream_t s = d_->st;
+	delete d_;
+	if (s) (void)hipStreamDestroy(s);
+}
+
+LinearSystem *LinearSystem::create(int n, std::string &err)
+{
+	if (n <= 0 || n > (1 << 17)) {
+		err = "linear system size must be 1..131072";
+		return nullptr;
+	}
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+		err = "no HIP device available";
+		return nullptr;
+	}
+	LinearSystem *L = new LinearSystem();
+	L->d_ = new Impl();
+	L->n_ = n;
+	L->np_ = (n + 255) / 256 * 256;
+	const size_t np = L->np_;
+	if (hipStreamCreateWithFlags(&L->d_->st, hipStreamNonBlocking) != hipSuccess ||
+	    hipEventCreate(&L->d_->e0) != hipSuccess || hipEventCreate(&L->d_->e1) != hipSuccess) {
+		err = "stream/event creation failed";
+		delete L;
+		return nullptr;
+	}
+	if (L->d_->F32.alloc(np * np, err) || hipMemset(L->d_->F32.p, 0, np * np * sizeof(float)) != hipSuccess) {
+		if (err.empty()) err = "matrix clear failed";
+		delete L;
+		return nullptr;
+	}
+	std::vector<float> ones(n, 1.0f);
+	if (L->d_->area.alloc(n, err) ||
+	    hipMemcpy(L->d_->area.p, ones.data(), n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+		if (err.empty()) err = "area upload failed";
+		delete L;
+		return nullptr;
+	}
+	return L;
+}
+
+float *LinearSystem::matrix32()
+{
+	d_->have16 = d_->haveb16 = d_->havet = false;  // the caller may rewrite F
+	return d_->F32.p;
+}
+
+void *LinearSystem::stream() const { return (void *)d_->st; }
+
+int LinearSystem::set_matrix(const float *F, std::string &err)
+{
+	HIP_OK(hipMemcpy2D(matrix32(), (size_t)np_ * 4, F, (size_t)n_ * 4, (size_t)n_ * 4, n_, hipMemcpyHostToDevice),
+	       "set matrix");
+	return 0;
+}
+
+int LinearSystem::get_matrix(float *F, std::string &err)
+{
+	HIP_OK(hipMemcpy2D(F, (size_t)n_ * 4, d_->F32.p, (size_t)np_ * 4, (size_t)n_ * 4, n_, hipMemcpyDeviceToHost),
+	       "get matrix");
+	return 0;
+}
+
+int LinearSystem::set_areas(const double *area, std::string &err)
+{
+	std::vector<float> a(n_);
+	for (int i = 0; i < n_; i++) a[i] = (float)area[i];
+	HIP_OK(hipMemcpy(d_->area.p, a.data(), n_ * sizeof(float), hipMemcpyHostToDevice), "set areas");
+	return 0;
+}
+
+void LinearSystem::tile_config(int nrhs, int prec, int *rt, int *splitk) const
+{
+	const int ct = pick_ct(nrhs);
+	auto it = d_->tiles.find({ct, prec});
+	if (it == d_->tiles.end()) {
+		*rt = 0;
+		*splitk = 0;
+	} else {
+		*rt = it->second.first;
+		*splitk = it->second.second;
+	}
+}
+
+int LinearSystem::set_tile(int nrhs, int prec, int rt, int splitk, std::string &err)
+{
+	const int ct = pick_ct(nrhs);
+	if (nrhs <= 0 || ct < 0 || (prec != PREC_F16 && prec != PREC_BF16) || (rt != 1 && rt != 2 && rt != 4) ||
+	    (splitk != 1 && splitk != 2 && splitk != 4 && splitk != 8) || !valid_tile(rt, ct, splitk, np_)) {
+		err = "set_tile: no kernel for this tile at this size";
+		return -1;
+	}
+	d_->tiles[{ct, prec}] = {rt, splitk};
+	return 0;
+}
+
+// Times one sweep (GEMM + Jacobi epilogue) for every valid (rows per wave,
+// split-K) and keeps the fastest for (nrhs column tiles, precision).
+int LinearSystem::autotune(int nrhs, int prec, std::string &err)
+{
+	Impl &d = *d_;
+	const int ct = pick_ct(nrhs);
+	if (nrhs <= 0 || ct < 0 || (prec != PREC_F16 && prec != PREC_BF16)) {
+		err = "autotune: nrhs must be 1..192 and precision FP16 or BF16";
+		return -1;
+	}
+	const int rp = ct * 16, ld = np_, n = n_;
+	if (d.ensure_lowp(prec, ld, err)) return -1;
+	const size_t total = (size_t)n * nrhs;
+	Buf<_Float16> xt16;
+	Buf<__bf16> xtb;
+	Buf<float> part, r32, d32, rho;
+	if (part.alloc((size_t)8 * ld * rp, err) || r32.alloc(total, err) || d32.alloc(total, err) || rho.alloc(total, err))
+		return -1;
+	const int g = grid_for(total);
+	const size_t xts = (size_t)rp * ld;
+	hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kB), 0, d.st, r32.p, total, 1.0f);
+	hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kB), 0, d.st, rho.p, total, 0.5f);
+	if (prec == PREC_F16) {
+		if (xt16.alloc(xts, err)) return -1;
+		hipLaunchKernelGGL(k_fill<_Float16>, dim3(grid_for(xts)), dim3(kB), 0, d.st, xt16.p, xts, (_Float16)0.f);
+	} else {
+		if (xtb.alloc(xts, err)) return -1;
+		hipLaunchKernelGGL(k_fill<__bf16>, dim3(grid_for(xts)), dim3(kB), 0, d.st, xtb.p, xts, (__bf16)0.f);
+	}
+	auto sweep = [&](int rt, int S) -> int {
+		if (prec == PREC_F16) {
+			if (launch_gemm<_Float16>(rt, ct, d.F16.p, xt16.p, part.p, ld, rp, S, d.st)) return -1;
+			hipLaunchKernelGGL(k_jacobi<_Float16>, dim3(g), dim3(kB), 0, d.st, part.p, S, ld, rp, n, nrhs, 1.0f, r32.p,
+			                   rho.p, d32.p, xt16.p);
+		} else {
+			if (launch_gemm<__bf16>(rt, ct, d.FB16.p, xtb.p, part.p, ld, rp, S, d.st)) return -1;
+			hipLaunchKernelGGL(k_jacobi<__bf16>, dim3(g), dim3(kB), 0, d.st, part.p, S, ld, rp, n, nrhs, 1.0f, r32.p,
+			                   rho.p, d32.p, xtb.p);
+		}
+		return 0;
+	};
+	double best = 1e30;
+	int brt = 0, bs = 0;
+	for (int rt : {1, 2, 4}) {
+		for (int S : {1, 2, 4, 8}) {
+			if (!valid_tile(rt, ct, S, ld)) continue;
+			if (sweep(rt, S)) continue;  // warm-up
+			HIP_OK(hipEventRecord(d.e0, d.st), "event");
+			for (int k = 0; k < 3; k++) sweep(rt, S);
+			HIP_OK(hipEventRecord(d.e1, d.st), "event");
+			HIP_OK(hipEventSynchronize(d.e1), "event");
+			float ms = 0;
+			HIP_OK(hipEventElapsedTime(&ms, d.e0, d.e1), "event");
+			if (ms < best) {
+				best = ms;
+				brt = rt;
+				bs = S;
+			}
+		}
+	}
+	HIP_OK(hipGetLastError(), "autotune");
+	HIP_OK(hipStreamSynchronize(d.st), "autotune");
+	if (!brt) {
+		err = "autotune: no valid tile";
+		return -1;
+	}
+	d.tiles[{ct, prec}] = {brt, bs};
+	return 0;
+}
+
+int LinearSystem::gemm_ms(int nrhs, int prec, int reps, double *ms, std::string &err)
+{
+	Impl &d = *d_;
+	const int ct = pick_ct(nrhs);
+	if (nrhs <= 0 || ct < 0 || (prec != PREC_F16 && prec != PREC_BF16) || reps <= 0) {
+		err = "gemm_ms: bad arguments";
+		return -1;
+	}
+	if (!d.tiles.count({ct, prec}) && autotune(nrhs, prec, err)) return -1;
+	int rt, S;
+	tile_config(nrhs, prec, &rt, &S);
+	const int rp = ct * 16, ld = np_;
+	if (d.ensure_lowp(prec, ld, err)) return -1;
+	Buf<_Float16> xt16;
+	Buf<__bf16> xtb;
+	Buf<float> part;
+	const size_t xts = (size_t)rp * ld;
+	if (part.alloc((size_t)S * ld * rp, err)) return -1;
+	if (prec == PREC_F16) {
+		if (xt16.alloc(xts, err)) return -1;
+		hipLaunchKernelGGL(k_fill<_Float16>, dim3(grid_for(xts)), dim3(kB), 0, d.st, xt16.p, xts, (_Float16)1.f);
+	} else {
+		if (xtb.alloc(xts, err)) return -1;
+		hipLaunchKernelGGL(k_fill<__bf16>, dim3(grid_for(xts)), dim3(kB), 0, d.st, xtb.p, xts, (__bf16)1.f);
+	}
+	auto one = [&]() {
+		if (prec == PREC_F16) launch_gemm<_Float16>(rt, ct, d.F16.p, xt16.p, part.p, ld, rp, S, d.st);
+		else launch_gemm<__bf16>(rt, ct, d.FB16.p, xtb.p, part.p, ld, rp, S, d.st);
+	};
+	one();
+	HIP_OK(hipEventRecord(d.e0, d.st), "event");
+	for (int k = 0; k < reps; k++) one();
+	HIP_OK(hipEventRecord(d.e1, d.st), "event");
+	HIP_OK(hipEventSynchronize(d.e1), "event");
+	HIP_OK(hipGetLastError(), "gemm_ms");
+	float t = 0;
+	HIP_OK(hipEventElapsedTime(&t, d.e0, d.e1), "event");
+	*ms = t / reps;
+	return 0;
+}
+
+int LinearSystem::multiply(int prec, int nrhs, const double *X, double *Y, std::string &err)
+{
+	Impl &d = *d_;
+	const int ct = pick_ct(nrhs);
+	if (nrhs <= 0 || ct < 0) {
+		err = "multiply: nrhs must be 1..192";
+		return -1;
+	}
+	const int n = n_, ld = np_, rp = ct * 16;
+	const size_t total = (size_t)n * nrhs;
+	hipStream_t st = d.st;
+	if (prec == PREC_F32) {
+		std::vector<double> xt((size_t)nrhs * ld, 0.0);
+		for (int i = 0; i < n; i++)
+			for (int c = 0; c < nrhs; c++) xt[(size_t)c * ld + i] = X[(size_t)i * nrhs + c];
+		Buf<double> xT, out, blk;
+		const dim3 grid((n + 7) / 8, (nrhs + 3) / 4);
+		if (xT.alloc(xt.size(), err) || out.alloc(total, err) || blk.alloc((size_t)grid.x * grid.y * 2, err)) return -1;
+		HIP_OK(hipMemcpyAsync(xT.p, xt.data(), xt.size() * 8, hipMemcpyHostToDevice, st), "upload");
+		hipLaunchKernelGGL((k_fx64<8, 4>), grid, dim3(kB), 0, st, d.F32.p, ld, n, nrhs, xT.p, (const double *)nullptr,
+		                   (const double *)nullptr, 0.0, out.p, blk.p);
+		HIP_OK(hipGetLastError(), "multiply");
+		HIP_OK(hipMemcpyAsync(Y, out.p, total * 8, hipMemcpyDeviceToHost, st), "download");
+		HIP_OK(hipStreamSynchronize(st), "multiply");
+		return 0;
+	}
+	if (d.ensure_lowp(prec, ld, err)) return -1;
+	int rt, S;
+	tile_config(nrhs, prec, &rt, &S);
+	if (!rt) {
+		rt = 1;
+		S = 1;
+	}
+	Buf<float> part, zero, ones, y;
+	Buf<double> x64;
+	Buf<_Float16> xt16;
+	Buf<__bf16> xtb;
+	if (part.alloc((size_t)S * ld * rp, err) || zero.alloc(total, err) || ones.alloc(total, err) || y.alloc(total, err) ||
+	    x64.alloc(total, err))
+		return -1;
+	const int g = grid_for(total);
+	const size_t xts = (size_t)rp * ld;
+	hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kB), 0, st, zero.p, total, 0.0f);
+	hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kB), 0, st, ones.p, total, 1.0f);
+	HIP_OK(hipMemcpyAsync(x64.p, X, total * 8, hipMemcpyHostToDevice, st), "upload");
+	const float inv = 1.0f / (prec == PREC_F16 ? d.s16 : d.sb16);
+	int rc;
+	if (prec == PREC_F16) {
+		if (xt16.alloc(xts, err)) return -1;
+		hipLaunchKernelGGL(k_fill<_Float16>, dim3(grid_for(xts)), dim3(kB), 0, st, xt16.p, xts, (_Float16)0.f);
+		hipLaunchKernelGGL(k_scale<_Float16>, dim3(g), dim3(kB), 0, st, x64.p, 1.0, total, nrhs, ld, y.p, xt16.p);
+		rc = launch_gemm<_Float16>(rt, ct, d.F16.p, xt16.p, part.p, ld, rp, S, st);
+		hipLaunchKernelGGL(k_jacobi<_Float16>, dim3(g), dim3(kB), 0, st, part.p, S, ld, rp, n, nrhs, inv, zero.p, ones.p,
+		                   y.p, xt16.p);
+	} else {
+		if (xtb.alloc(xts, err)) return -1;
+		hipLaunchKernelGGL(k_fill<__bf16>, dim3(grid_for(xts)), dim3(kB), 0, st, xtb.p, xts, (__bf16)0.f);
+		hipLaunchKernelGGL(k_scale<__bf16>, dim3(g), dim3(kB), 0, st, x64.p, 1.0, total, nrhs, ld, y.p, xtb.p);
+		rc = launch_gemm<__bf16>(rt, ct, d.FB16.p, xtb.p, part.p, ld, rp, S, st);
+		hipLaunchKernelGGL(k_jacobi<__bf16>, dim3(g), dim3(kB), 0, st, part.p, S, ld, rp, n, nrhs, inv, zero.p, ones.p,
+		                   y.p, xtb.p);
+	}
+	if (rc) {
+		err = "multiply: no kernel for this tile";
+		return -1;
+	}
+	HIP_OK(hipGetLastError(), "multiply");
+	std::vector<float> hy(total);
+	HIP_OK(hipMemcpyAsync(hy.data(), y.p, total * 4, hipMemcpyDeviceToHost, st), "download");
+	HIP_OK(hipStreamSynchronize(st), "multiply");
+	for (size_t t = 0; t < total; t++) Y[t] = hy[t];
+	return 0;
+}
+
+int LinearSystem::gather(int prec, int nrhs, const double *rho, const double *E, double tol, int max_outer, int sweeps,
+                         const double *Bref, double target, double *B, double *info, std::string &err)
+{
+	Impl &d = *d_;
+	const int ct = pick_ct(nrhs);
+	if (nrhs <= 0 || ct < 0) {
+		err = "gather: nrhs must be 1..192";
+		return -1;
+	}
+	if (prec != PREC_F16 && prec != PREC_BF16) {
+		err = "gather: precision must be FP16 or BF16";
+		return -1;
+	}
+	if (sweeps <= 0 || max_outer < 0) {
+		err = "gather: sweeps must be > 0 and max_outer >= 0";
+		return -1;
+	}
+	if (d.ensure_lowp(prec, np_, err)) return -1;
+	if (!d.tiles.count({ct, prec}) && autotune(nrhs, prec, err)) return -1;
+	int rt, S;
+	tile_config(nrhs, prec, &rt, &S);
+	const int n = n_, ld = np_, rp = ct * 16;
+	const size_t total = (size_t)n * nrhs;
+	const float inv = 1.0f / (prec == PREC_F16 ? d.s16 : d.sb16);
+	hipStream_t st = d.st;
+	Buf<float> rho32, r32, d32, part;
+	Buf<double> rho64, E64, xT, r64, blk, Bref64;
+	Buf<_Float16> XT16;
+	Buf<__bf16> XTb;
+	if (rho32.alloc(total, err) || r32.alloc(total, err) || d32.alloc(total, err) ||
+	    part.alloc((size_t)S * ld * rp, err) || rho64.alloc(total, err) || E64.alloc(total, err) ||
+	    xT.alloc((size_t)nrhs * ld, err) || r64.alloc(total, err))
+		return -1;
+	constexpr int RB = 8, CC = 4;
+	const dim3 fgrid((n + RB - 1) / RB, (nrhs + CC - 1) / CC);
+	const size_t nblk = (size_t)fgrid.x * fgrid.y;
+	if (blk.alloc(std::max<size_t>(nblk, 8192) * 2, err)) return -1;
+	const size_t xts = (size_t)rp * ld;
+	if (prec == PREC_F16) {
+		if (XT16.alloc(xts, err)) return -1;
+		hipLaunchKernelGGL(k_fill<_Float16>, dim3(grid_for(xts)), dim3(kB), 0, st, XT16.p, xts, (_Float16)0.f);
+	} else {
+		if (XTb.alloc(xts, err)) return -1;
+		hipLaunchKernelGGL(k_fill<__bf16>, dim3(grid_for(xts)), dim3(kB), 0, st, XTb.p, xts, (__bf16)0.f);
+	}
+	std::vector<float> h32(total);
+	for (size_t t = 0; t < total; t++) h32[t] = (float)rho[t];
+	HIP_OK(hipMemcpyAsync(rho32.p, h32.data(), total * 4, hipMemcpyHostToDevice, st), "upload");
+	HIP_OK(hipMemcpyAsync(rho64.p, rho, total * 8, hipMemcpyHostToDevice, st), "upload");
+	HIP_OK(hipMemcpyAsync(E64.p, E, total * 8, hipMemcpyHostToDevice, st), "upload");
+	hipLaunchKernelGGL(k_fill<double>, dim3(grid_for((size_t)nrhs * ld)), dim3(kB), 0, st, xT.p, (size_t)nrhs * ld, 0.0);
+	if (Bref) {
+		if (Bref64.alloc(total, err)) return -1;
+		HIP_OK(hipMemcpyAsync(Bref64.p, Bref, total * 8, hipMemcpyHostToDevice, st), "upload");
+	}
+	HIP_OK(hipStreamSynchronize(st), "upload");
+	std::vector<double> hb(std::max<size_t>(nblk, 8192) * 2);
+	const int g = grid_for(total);
+	auto err_vs_ref = [&](double &e) -> int {
+		hipLaunchKernelGGL(k_err, dim3(g), dim3(kB), 0, st, xT.p, 1, ld, Bref64.p, n, nrhs, blk.p);
+		HIP_OK(hipMemcpyAsync(hb.data(), blk.p, (size_t)g * 2 * 8, hipMemcpyDeviceToHost, st), "err");
+		HIP_OK(hipStreamSynchronize(st), "err");
+		double a = 0.0, b = 0.0;
+		for (int k = 0; k < g; k++) {
+			a += hb[2 * k];
+			b += hb[2 * k + 1];
+		}
+		e = std::sqrt(a / std::max(b, 1e-300));
+		return 0;
+	};
+	const auto t0 = std::chrono::steady_clock::now();
+	double enorm = -1.0, rel = 0.0, ms_target = -1.0, errv = -1.0, check_ms = 0.0;
+	int outer = 0, total_sweeps = 0;
+	for (;; outer++) {
+		hipLaunchKernelGGL((k_fx64<RB, CC>), fgrid, dim3(kB), 0, st, d.F32.p, ld, n, nrhs, xT.p, E64.p, rho64.p, 1.0,
+		                   r64.p, blk.p);
+		HIP_OK(hipGetLastError(), "residual launch");
+		HIP_OK(hipMemcpyAsync(hb.data(), blk.p, nblk * 2 * 8, hipMemcpyDeviceToHost, st), "reduce");
+		HIP_OK(hipStreamSynchronize(st), "residual");
+		double ss = 0.0, mx = 0.0;
+		for (size_t b = 0; b < nblk; b++) {
+			ss += hb[2 * b];
+			mx = std::max(mx, hb[2 * b + 1]);
+		}
+		if (!std::isfinite(ss)) {
+			err = "gather diverged (non-finite residual): the iteration needs spectral radius of diag(rho) F below 1";
+			return -1;
+		}
+		if (enorm < 0) enorm = std::sqrt(ss);  // x = 0: the first residual is E
+		rel = enorm > 0 ? std::sqrt(ss) / enorm : 0.0;
+		if (Bref && outer > 0) {
+			const auto c0 = std::chrono::steady_clock::now();
+			if (err_vs_ref(errv)) return -1;
+			check_ms += ms_since(c0);
+			if (ms_target < 0 && errv <= target) ms_target = ms_since(t0) - check_ms;
+		}
+		if (rel <= tol || outer >= max_outer || mx == 0.0) break;
+		const double s = mx;
+		if (prec == PREC_F16)
+			hipLaunchKernelGGL(k_scale<_Float16>, dim3(g), dim3(kB), 0, st, r64.p, 1.0 / s, total, nrhs, ld, r32.p, XT16.p);
+		else
+			hipLaunchKernelGGL(k_scale<__bf16>, dim3(g), dim3(kB), 0, st, r64.p, 1.0 / s, total, nrhs, ld, r32.p, XTb.p);
+		for (int m = 0; m < sweeps; m++) {
+			int rc;
+			if (prec == PREC_F16) {
+				rc = launch_gemm<_Float16>(rt, ct, d.F16.p, XT16.p, part.p, ld, rp, S, st);
+				hipLaunchKernelGGL(k_jacobi<_Float16>, dim3(g), dim3(kB), 0, st, part.p, S, ld, rp, n, nrhs, inv, r32.p,
+				                   rho32.p, d32.p, XT16.p);
+			} else {
+				rc = launch_gemm<__bf16>(rt, ct, d.FB16.p, XTb.p, part.p, ld, rp, S, st);
+				hipLaunchKernelGGL(k_jacobi<__bf16>, dim3(g), dim3(kB), 0, st, part.p, S, ld, rp, n, nrhs, inv, r32.p,
+				                   rho32.p, d32.p, XTb.p);
+			}
+			if (rc) {
+				err = "gather: no kernel for this tile";
+				return -1;
+			}
+			total_sweeps++;
+		}
+		hipLaunchKernelGGL(k_axpy_t, dim3(g), dim3(kB), 0, st, xT.p, ld, d32.p, s, n, nrhs);
+		HIP_OK(hipGetLastError(), "sweep launch");
+	}
+	const double ms = ms_since(t0) - check_ms;
+	std::vector<double> hx((size_t)nrhs * ld);
+	HIP_OK(hipMemcpy(hx.data(), xT.p, hx.size() * 8, hipMemcpyDeviceToHost), "download");
+	for (int i = 0; i < n; i++)
+		for (int c = 0; c < nrhs; c++) B[(size_t)i * nrhs + c] = hx[(size_t)c * ld + i];
+	if (info) {
+		std::vector<float> a(n);
+		HIP_OK(hipMemcpy(a.data(), d.area.p, n * 4, hipMemcpyDeviceToHost), "area");
+		info[INFO_OUTER] = outer;
+		info[INFO_SWEEPS] = total_sweeps;
+		info[INFO_REL_RES] = rel;
+		info[INFO_MS] = ms;
+		info[INFO_MS_TARGET] = ms_target;
+		info[INFO_ERR] = errv;
+		info[INFO_POWER_E] = power_sum(a, E, n, nrhs);
+		info[INFO_POWER_B] = power_sum(a, B, n, nrhs);
+	}
+	return 0;
+}
+
+int LinearSystem::shoot(int nrhs, const double *rho, const double *E, const double *Bref, double target, int max_shots,
+                        double *B, double *info, std::string &err)
+{
+	Impl &d = *d_;
+	if (nrhs <= 0 || nrhs > kMaxShootRhs) {
+		err = "shoot: nrhs must be 1..16";
+		return -1;
+	}
+	if (max_shots < 0) {
+		err = "shoot: max_shots must be >= 0";
+		return -1;
+	}
+	const int n = n_, ld = np_;
+	hipStream_t st = d.st;
+	if (!d.havet) {
+		if (d.Ft.alloc((size_t)ld * ld, err)) return -1;
+		hipLaunchKernelGGL(k_transpose, dim3(ld / 32, ld / 32), dim3(kB), 0, st, d.F32.p, d.Ft.p, ld);
+		HIP_OK(hipGetLastError(), "transpose");
+		d.havet = true;
+	}
+	const size_t total = (size_t)n * nrhs;
+	Buf<float> rho32, B32, dB32;
+	Buf<ShootState> state;
+	if (rho32.alloc(total, err) || B32.alloc(total, err) || dB32.alloc(total, err) || state.alloc(1, err)) return -1;
+	std::vector<float> h(total);
+	for (size_t t = 0; t < total; t++) h[t] = (float)rho[t];
+	HIP_OK(hipMemcpyAsync(rho32.p, h.data(), total * 4, hipMemcpyHostToDevice, st), "upload");
+	std::vector<float> e32(total);
+	for (size_t t = 0; t < total; t++) e32[t] = (float)E[t];
+	HIP_OK(hipMemcpyAsync(B32.p, e32.data(), total * 4, hipMemcpyHostToDevice, st), "upload");
+	HIP_OK(hipMemcpyAsync(dB32.p, e32.data(), total * 4, hipMemcpyHostToDevice, st), "upload");
+	ShootState zero;
+	std::memset(&zero, 0, sizeof zero);
+	HIP_OK(hipMemcpyAsync(state.p, &zero, sizeof zero, hipMemcpyHostToDevice, st), "upload");
+	const int nb = (n + kB - 1) / kB;
+	HIP_OK(hipStreamSynchronize(st), "upload");
+	std::vector<float> hb(total);
+	auto err_vs_ref = [&](double &e) -> int {
+		HIP_OK(hipMemcpyAsync(hb.data(), B32.p, total * 4, hipMemcpyDeviceToHost, st), "err");
+		HIP_OK(hipStreamSynchronize(st), "err");
+		double a = 0.0, b = 0.0;
+		for (size_t t = 0; t < total; t++) {
+			const double x = (double)hb[t] - Bref[t];
+			a += x * x;
+			b += Bref[t] * Bref[t];
+		}
+		e = std::sqrt(a / std::max(b, 1e-300));
+		return 0;
+	};
+	const auto t0 = std::chrono::steady_clock::now();
+	hipLaunchKernelGGL(k_lsel, dim3(nb), dim3(kB), 0, st, dB32.p, d.area.p, n, nrhs, state.p);
+	double check_ms = 0.0, ms_target = -1.0, errv = -1.0;
+	int launched = 0, chunk = 64;
+	ShootState hs;
+	std::memset(&hs, 0, sizeof hs);
+	for (;;) {
+		const int b = std::min(chunk, max_shots - launched);
+		if (b <= 0) break;
+		for (int k = 0; k < b; k++)
+			hipLaunchKernelGGL(k_lshoot, dim3(nb), dim3(kB), 0, st, d.Ft.p, ld, rho32.p, d.area.p, B32.p, dB32.p, n, nrhs,
+			                   state.p);
+		launched += b;
+		HIP_OK(hipGetLastError(), "shoot launch");
+		HIP_OK(hipMemcpyAsync(&hs, state.p, sizeof hs, hipMemcpyDeviceToHost, st), "state");
+		HIP_OK(hipStreamSynchronize(st), "shoot");
+		if (Bref) {
+			const auto c0 = std::chrono::steady_clock::now();
+			if (err_vs_ref(errv)) return -1;
+			check_ms += ms_since(c0);
+			if (errv <= target) {
+				ms_target = ms_since(t0) - check_ms;
+				break;
+			}
+		}
+		if (hs.done) break;
+		chunk = std::min(chunk * 2, 4096);
+	}
+	const double ms = ms_since(t0) - check_ms;
+	HIP_OK(hipMemcpy(hb.data(), B32.p, total * 4, hipMemcpyDeviceToHost), "download");
+	for (size_t t = 0; t < total; t++) B[t] = hb[t];
+	if (Bref && errv < 0 && err_vs_ref(errv)) return -1;
+	if (info) {
+		std::vector<float> a(n);
+		HIP_OK(hipMemcpy(a.data(), d.area.p, n * 4, hipMemcpyDeviceToHost), "area");
+		info[INFO_OUTER] = hs.shots;
+		info[INFO_SWEEPS] = 0;
+		info[INFO_REL_RES] = -1;
+		info[INFO_MS] = ms;
+		info[INFO_MS_TARGET] = ms_target;
+		info[INFO_ERR] = errv;
+		info[INFO_POWER_E] = power_sum(a, E, n, nrhs);
+		info[INFO_POWER_B] = power_sum(a, B, n, nrhs);
+	}
+	return 0;
+}
+
+int LinearSystem::adjoint(int nrhs, const double *rho, const double *B, const double *g, double tol, int max_outer,
+                          int sweeps, double *lambda, double *drho, double *info, std::string &err)
+{
+	Impl &d = *d_;
+	if (nrhs <= 0 || nrhs > 4) {
+		err = "adjoint: nrhs must be 1..4";
+		return -1;
+	}
+	if (sweeps <= 0 || max_outer < 0) {
+		err = "adjoint: sweeps must be > 0 and max_outer >= 0";
+		return -1;
+	}
+	const int n = n_, ld = np_;
+	const size_t total = (size_t)n * nrhs;
+	hipStream_t st = d.st;
+	const int S = std::max(1, std::min(64, n / 256));
+	const int rows_per_split = (n + S - 1) / S;
+	const int fin_blocks = (int)((total + kB - 1) / kB);
+	Buf<double> rho64, g64, x64, r64, part64, blk, B64T, fb, fblk;
+	Buf<float> rho32, r32, l32, ln32, part32;
+	if (rho64.alloc(total, err) || g64.alloc(total, err) || x64.alloc(total, err) || r64.alloc(total, err) ||
+	    part64.alloc((size_t)S * total, err) || rho32.alloc(total, err) || r32.alloc(total, err) ||
+	    l32.alloc(total, err) || ln32.alloc(total, err) || part32.alloc((size_t)S * total, err) ||
+	    blk.alloc((size_t)std::max(fin_blocks, 8192) * 2, err) || B64T.alloc((size_t)nrhs * ld, err) ||
+	    fb.alloc(total, err) || fblk.alloc((size_t)((n + 7) / 8) * 2, err))
+		return -1;
+	std::vector<float> h(total);
+	for (size_t t = 0; t < total; t++) h[t] = (float)rho[t];
+	HIP_OK(hipMemcpyAsync(rho32.p, h.data(), total * 4, hipMemcpyHostToDevice, st), "upload");
+	HIP_OK(hipMemcpyAsync(rho64.p, rho, total * 8, hipMemcpyHostToDevice, st), "upload");
+	HIP_OK(hipMemcpyAsync(g64.p, g, total * 8, hipMemcpyHostToDevice, st), "upload");
+	hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(total)), dim3(kB), 0, st, x64.p, total, 0.0);
+	const dim3 tgrid((n + kB - 1) / kB, S);
+	std::vector<double> hb((size_t)fin_blocks * 2);
+	HIP_OK(hipStreamSynchronize(st), "upload");
+	const auto t0 = std::chrono::steady_clock::now();
+	double enorm = -1.0, rel = 0.0;
+	int outer = 0, total_sweeps = 0;
+	for (;; outer++) {
+		// r = g + F^T (rho o x) - x in FP64
+		hipLaunchKernelGGL(k_ftx<double>, tgrid, dim3(kB), 0, st, d.F32.p, ld, n, nrhs, x64.p, rho64.p, rows_per_split,
+		                   part64.p);
+		hipLaunchKernelGGL(k_ftx_finish<double>, dim3(fin_blocks), dim3(kB), 0, st, part64.p, S, n, nrhs, g64.p, x64.p,
+		                   1.0, r64.p, blk.p);
+		HIP_OK(hipGetLastError(), "adjoint residual");
+		HIP_OK(hipMemcpyAsync(hb.data(), blk.p, hb.size() * 8, hipMemcpyDeviceToHost, st), "reduce");
+		HIP_OK(hipStreamSynchronize(st), "reduce");
+		double ss = 0.0, mx = 0.0;
+		for (int b = 0; b < fin_blocks; b++) {
+			ss += hb[2 * b];
+			mx = std::max(mx, hb[2 * b + 1]);
+		}
+		if (!std::isfinite(ss)) {
+			err = "adjoint diverged (non-finite residual)";
+			return -1;
+		}
+		if (enorm < 0) enorm = std::sqrt(ss);
+		rel = enorm > 0 ? std::sqrt(ss) / enorm : 0.0;
+		if (rel <= tol || outer >= max_outer || mx == 0.0) break;
+		const int gg = grid_for(total);
+		hipLaunchKernelGGL(k_scale<_Float16>, dim3(gg), dim3(kB), 0, st, r64.p, 1.0 / mx, total, nrhs, ld, r32.p,
+		                   (_Float16 *)nullptr);
+		HIP_OK(hipMemcpyAsync(l32.p, r32.p, total * 4, hipMemcpyDeviceToDevice, st), "copy");
+		for (int m = 0; m < sweeps; m++) {
+			hipLaunchKernelGGL(k_ftx<float>, tgrid, dim3(kB), 0, st, d.F32.p, ld, n, nrhs, l32.p, rho32.p,
+			                   rows_per_split, part32.p);
+			hipLaunchKernelGGL(k_ftx_finish<float>, dim3(fin_blocks), dim3(kB), 0, st, part32.p, S, n, nrhs, r32.p, l32.p,
+			                   0.0f, ln32.p, blk.p);
+			std::swap(l32.p, ln32.p);
+			total_sweeps++;
+		}
+		hipLaunchKernelGGL(k_axpy, dim3(gg), dim3(kB), 0, st, x64.p, l32.p, mx, total);
+		HIP_OK(hipGetLastError(), "adjoint sweep");
+	}
+	// dL/drho = lambda o (F B), F B in FP64
+	std::vector<double> bt((size_t)nrhs * ld, 0.0);
+	for (int i = 0; i < n; i++)
+		for (int c = 0; c < nrhs; c++) bt[(size_t)c * ld + i] = B[(size_t)i * nrhs + c];
+	HIP_OK(hipMemcpyAsync(B64T.p, bt.data(), bt.size() * 8, hipMemcpyHostToDevice, st), "upload");
+	hipLaunchKernelGGL((k_fx64<8, 4>), dim3((n + 7) / 8, 1), dim3(kB), 0, st, d.F32.p, ld, n, nrhs, B64T.p,
+	                   (const double *)nullptr, (const double *)nullptr, 0.0, fb.p, fblk.p);
+	HIP_OK(hipGetLastError(), "FB");
+	std::vector<double> hfb(total);
+	HIP_OK(hipMemcpyAsync(lambda, x64.p, total * 8, hipMemcpyDeviceToHost, st), "download");
+	HIP_OK(hipMemcpyAsync(hfb.data(), fb.p, total * 8, hipMemcpyDeviceToHost, st), "download");
+	HIP_OK(hipStreamSynchronize(st), "adjoint");
+	const double ms = ms_since(t0);
+	for (size_t t = 0; t < total; t++) drho[t] = lambda[t] * hfb[t];
+	if (info) {
+		for (int k = 0; k < INFO_N; k++) info[k] = -1;
+		info[INFO_OUTER] = outer;
+		info[INFO_SWEEPS] = total_sweeps;
+		info[INFO_REL_RES] = rel;
+		info[INFO_MS] = ms;
+	}
+	return 0;
+}
+
+}  // namespace rgpu

@@ -17,6 +17,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 
@@ -238,6 +239,82 @@ FlatOctree build_octree(const Patch *pool, int n, const float *G0, const float *
 				T.cell_leaf[((size_t)gx * gt + gy) * gt + gz] = st.nodes[node].leaf;
 			}
 	return T;
+}
+
+int build_segment_index(FlatOctree &T, const Patch *pool, int n, const float *G0, const float *G1, std::string &err)
+{
+	if (T.fine_r > 0) return 0;
+	int D = 0;
+	for (int l = 0; l < T.n_leaves; l++) {
+		int cd;
+		std::memcpy(&cd, &T.leaf_box[(size_t)l * 8 + 7], 4);
+		D = std::max(D, cd >> 24);
+	}
+	const int R = 1 << D;
+	double h[3];
+	for (int k = 0; k < 3; k++) h[k] = ((double)G1[k] - (double)G0[k]) / R;
+	// a leaf at depth d is a block of 2^(D - d) cells per axis starting at
+	// its lower corner (the float midpoints sit within rounding of the
+	// dyadic grid lines, so the nearest grid line is the corner)
+	std::vector<int> fine((size_t)R * R * R, -1);
+	for (int l = 0; l < T.n_leaves; l++) {
+		const float *b = &T.leaf_box[(size_t)l * 8];
+		int cd;
+		std::memcpy(&cd, &b[7], 4);
+		const int size = 1 << (D - (cd >> 24));
+		int base[3];
+		for (int k = 0; k < 3; k++) base[k] = (int)std::lround(((double)b[k] - G0[k]) / h[k]);
+		for (int k = 0; k < 3; k++)
+			if (base[k] < 0 || base[k] + size > R || base[k] % size) {
+				err = "octree leaf off the dyadic grid";
+				return -1;
+			}
+		for (int a = base[0]; a < base[0] + size; a++)
+			for (int bb = base[1]; bb < base[1] + size; bb++)
+				for (int c = base[2]; c < base[2] + size; c++) {
+					int &cell = fine[((size_t)a * R + bb) * R + c];
+					if (cell >= 0) {
+						err = "octree leaves overlap";
+						return -1;
+					}
+					cell = l;
+				}
+	}
+	for (int x : fine)
+		if (x < 0) {
+			err = "octree leaves do not tile the root box";
+			return -1;
+		}
+	// complete leaf lists: the reference's CrossOver/Clip test misses
+	// triangles that cross a box through their interior, so the segment
+	// walk lists a triangle in every leaf holding a fine cell that its
+	// bounding box, grown by 1e-3 of a cell, touches
+	std::vector<std::vector<int>> per(T.n_leaves);
+	for (int i = 0; i < n; i++) {
+		int lo[3], hi[3];
+		for (int k = 0; k < 3; k++) {
+			double mn = pool[i].p[0][k], mx = mn;
+			for (int v = 1; v < 3; v++) {
+				mn = std::min(mn, (double)pool[i].p[v][k]);
+				mx = std::max(mx, (double)pool[i].p[v][k]);
+			}
+			lo[k] = std::max(0, (int)std::floor((mn - G0[k]) / h[k] - 1e-3));
+			hi[k] = std::min(R - 1, (int)std::floor((mx - G0[k]) / h[k] + 1e-3));
+		}
+		for (int a = lo[0]; a <= hi[0]; a++)
+			for (int b = lo[1]; b <= hi[1]; b++)
+				for (int c = lo[2]; c <= hi[2]; c++) {
+					std::vector<int> &L = per[fine[((size_t)a * R + b) * R + c]];
+					if (L.empty() || L.back() != i) L.push_back(i);
+				}
+	}
+	T.seg_begin.assign(T.n_leaves + 1, 0);
+	for (int l = 0; l < T.n_leaves; l++) T.seg_begin[l + 1] = T.seg_begin[l] + (int)per[l].size();
+	T.seg_items.resize(T.seg_begin[T.n_leaves]);
+	for (int l = 0; l < T.n_leaves; l++) std::copy(per[l].begin(), per[l].end(), T.seg_items.begin() + T.seg_begin[l]);
+	T.fine_leaf = std::move(fine);
+	T.fine_r = R;
+	return 0;
 }
 
 }  // namespace rgpu

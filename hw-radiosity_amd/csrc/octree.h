@@ -2,6 +2,7 @@
 // it (raycast.cpp:168-233) and flattened for the device.
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "patch.h"
@@ -20,9 +21,23 @@ struct FlatOctree {
 	std::vector<float> leaf_box;  // 8 floats per leaf: g0.xyz, begin, g1.xyz, count|depth<<24
 	std::vector<int> items;       // leaf triangle lists, ascending original index
 	int n_leaves = 0;
+	// geometric point location (the explicit form-factor matrix's segment
+	// visibility, made by build_segment_index): fine_r = 2^(deepest leaf
+	// level) cells per axis over [G0, G1], fine_leaf[(a * fine_r + b) *
+	// fine_r + c] = the leaf holding that cell
+	int fine_r = 0;
+	std::vector<int> fine_leaf;
+	// complete leaf lists for that walk: seg_items[seg_begin[l] ..
+	// seg_begin[l + 1]) = every original whose bounding box (grown by 1e-3
+	// of a fine cell) touches a cell of leaf l, ascending
+	std::vector<int> seg_begin, seg_items;
 };
 
 // Octree over the originals [0, n) of `pool` inside [G0, G1].
 FlatOctree build_octree(const Patch *pool, int n, const float *G0, const float *G1, bool fma, int threads);
+
+// The segment-visibility index (fine_r, fine_leaf, seg_begin, seg_items) of
+// T over the originals [0, n); once per tree.
+int build_segment_index(FlatOctree &T, const Patch *pool, int n, const float *G0, const float *G1, std::string &err);
 
 }  // namespace rgpu

@@ -36,7 +36,15 @@ RAY_DTYPE = np.dtype([("p", "<f4", 3), ("v", "<f4", 3), ("ban", "<i4"), ("target
 
 EXPORTS = ("rad_create", "rad_destroy", "rad_last_error", "rad_load_tri", "rad_set_triangles",
            "rad_prepare_host", "rad_upload", "rad_shoot", "rad_run", "rad_write_fin", "rad_records",
-           "rad_tree_info", "rad_cast", "rad_ff_pairs", "rad_counters", "rad_timings")
+           "rad_tree_info", "rad_cast", "rad_ff_pairs", "rad_counters", "rad_timings",
+           "rad_lin_create", "rad_lin_destroy", "rad_lin_last_error", "rad_lin_from_scene",
+           "rad_lin_set_matrix", "rad_lin_get_matrix", "rad_lin_set_areas", "rad_lin_autotune", "rad_lin_set_tile",
+           "rad_lin_gemm_ms", "rad_lin_multiply", "rad_lin_gather", "rad_lin_shoot", "rad_lin_adjoint")
+
+PREC_F32, PREC_F16, PREC_BF16 = 0, 1, 2
+VIS_SEGMENT, VIS_REFERENCE = 0, 1
+PRECISIONS = {"f32": PREC_F32, "f16": PREC_F16, "bf16": PREC_BF16}
+INFO_NAMES = ("outer", "sweeps", "rel_res", "ms", "ms_target", "err", "power_E", "power_B")
 
 
 class RadError(RuntimeError):
@@ -55,7 +63,7 @@ def library() -> ctypes.CDLL:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libradgpu.so is not built: run `make -C hw-radiosity_amd` (%s)" % LIB_PATH)
         L = ctypes.CDLL(LIB_PATH)
-        vp, ip, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p
+        vp, ip, cp, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_double
         sig = {
             "rad_create": [ctypes.POINTER(vp), ip, ctypes.POINTER(cp)],
             "rad_destroy": [vp],
@@ -72,13 +80,27 @@ def library() -> ctypes.CDLL:
             "rad_ff_pairs": [vp, ip, vp, vp],
             "rad_counters": [vp, vp],
             "rad_timings": [vp, vp],
+            "rad_lin_create": [ctypes.POINTER(vp), ip],
+            "rad_lin_destroy": [vp],
+            "rad_lin_from_scene": [vp, vp, ip, vp],
+            "rad_lin_set_matrix": [vp, vp],
+            "rad_lin_get_matrix": [vp, vp],
+            "rad_lin_set_areas": [vp, vp],
+            "rad_lin_autotune": [vp, ip, ip, ctypes.POINTER(ip), ctypes.POINTER(ip)],
+            "rad_lin_gemm_ms": [vp, ip, ip, ip, vp],
+            "rad_lin_set_tile": [vp, ip, ip, ip, ip],
+            "rad_lin_multiply": [vp, ip, ip, vp, vp],
+            "rad_lin_gather": [vp, ip, ip, vp, vp, dp, ip, ip, vp, dp, vp, vp],
+            "rad_lin_shoot": [vp, ip, vp, vp, vp, dp, ip, vp, vp],
+            "rad_lin_adjoint": [vp, ip, vp, vp, vp, dp, ip, ip, vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
             f.argtypes = args
-            f.restype = None if name == "rad_destroy" else ip
-        L.rad_last_error.argtypes = [vp]
-        L.rad_last_error.restype = cp
+            f.restype = None if name in ("rad_destroy", "rad_lin_destroy") else ip
+        for name in ("rad_last_error", "rad_lin_last_error"):
+            getattr(L, name).argtypes = [vp]
+            getattr(L, name).restype = cp
         _lib = L
     return _lib
 
@@ -189,6 +211,134 @@ class Engine:
         self._ok(library().rad_timings(self.h, _p(out)))
         return {"setup_ms": out[0], "upload_ms": out[1], "shoot_ms": out[2], "iterations_launched": int(out[3]),
                 "kernel_ms": out[4], "kernels_timed": int(out[5]), "cast_ms": out[6]}
+
+
+def _prec(p):
+    return PRECISIONS[p] if isinstance(p, str) else int(p)
+
+
+def _rhs(a, n):
+    """n x nrhs float64, C-contiguous (a 1-D array is one column)."""
+    a = np.asarray(a, np.float64)
+    if a.ndim == 1:
+        a = a[:, None]
+    if a.shape[0] != n:
+        raise ValueError("expected %d rows, got %s" % (n, a.shape))
+    return np.ascontiguousarray(a)
+
+
+class LinearSystem:
+    """The radiosity system B = E + diag(rho) F B on an explicit form-factor
+    matrix held on the device (rad_lin_* in include/radiosity_gpu.h).
+
+    Right-hand sides are (n, nrhs) float64 arrays; rho has the same shape
+    (per patch and column, e.g. one column per RGB channel)."""
+
+    def __init__(self, n: int):
+        L = library()
+        h = ctypes.c_void_p()
+        st = L.rad_lin_create(ctypes.byref(h), int(n))
+        if st != RAD_OK:
+            raise RadError(st, "rad_lin_create(%d) failed" % n)
+        self.h, self.n = h, int(n)
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().rad_lin_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _ok(self, st):
+        if st != RAD_OK:
+            raise RadError(st, library().rad_lin_last_error(self.h).decode())
+
+    def from_scene(self, engine: "Engine", visibility: str = "segment") -> float:
+        """F and areas from an uploaded scene; returns the kernel's ms.
+        visibility: "segment" (clear segment between centroids) or
+        "reference" (RayHitted from the centroid)."""
+        ms = ctypes.c_double(0)
+        vis = {"segment": VIS_SEGMENT, "reference": VIS_REFERENCE}[visibility]
+        self._ok(library().rad_lin_from_scene(self.h, engine.h, vis, ctypes.byref(ms)))
+        return ms.value
+
+    def set_matrix(self, F):
+        F = np.ascontiguousarray(F, np.float32)
+        assert F.shape == (self.n, self.n)
+        self._ok(library().rad_lin_set_matrix(self.h, _p(F)))
+        return self
+
+    def matrix(self):
+        F = np.zeros((self.n, self.n), np.float32)
+        self._ok(library().rad_lin_get_matrix(self.h, _p(F)))
+        return F
+
+    def set_areas(self, area):
+        a = np.ascontiguousarray(area, np.float64)
+        assert a.shape == (self.n,)
+        self._ok(library().rad_lin_set_areas(self.h, _p(a)))
+        return self
+
+    def autotune(self, nrhs: int, prec="f16"):
+        rt, sk = ctypes.c_int(0), ctypes.c_int(0)
+        self._ok(library().rad_lin_autotune(self.h, int(nrhs), _prec(prec), ctypes.byref(rt), ctypes.byref(sk)))
+        return rt.value, sk.value
+
+    def set_tile(self, nrhs: int, prec, rt: int, splitk: int):
+        self._ok(library().rad_lin_set_tile(self.h, int(nrhs), _prec(prec), int(rt), int(splitk)))
+        return self
+
+    def gemm_ms(self, nrhs: int, prec="f16", reps: int = 20) -> float:
+        ms = np.zeros(1, np.float64)
+        self._ok(library().rad_lin_gemm_ms(self.h, int(nrhs), _prec(prec), int(reps), _p(ms)))
+        return float(ms[0])
+
+    def multiply(self, X, prec="f32"):
+        X = _rhs(X, self.n)
+        Y = np.zeros_like(X)
+        self._ok(library().rad_lin_multiply(self.h, _prec(prec), X.shape[1], _p(X), _p(Y)))
+        return Y
+
+    def gather(self, rho, E, prec="f16", tol=1e-6, max_outer=50, sweeps=8, Bref=None, target=1e-3):
+        E = _rhs(E, self.n)
+        rho = _rhs(rho, self.n)
+        assert rho.shape == E.shape
+        ref = None if Bref is None else _rhs(Bref, self.n)
+        B = np.zeros_like(E)
+        info = np.zeros(8, np.float64)
+        self._ok(library().rad_lin_gather(self.h, _prec(prec), E.shape[1], _p(rho), _p(E), float(tol),
+                                          int(max_outer), int(sweeps), _p(ref), float(target), _p(B), _p(info)))
+        return B, dict(zip(INFO_NAMES, info.tolist()))
+
+    def shoot(self, rho, E, Bref=None, target=1e-3, max_shots=1 << 30):
+        E = _rhs(E, self.n)
+        rho = _rhs(rho, self.n)
+        assert rho.shape == E.shape
+        ref = None if Bref is None else _rhs(Bref, self.n)
+        B = np.zeros_like(E)
+        info = np.zeros(8, np.float64)
+        self._ok(library().rad_lin_shoot(self.h, E.shape[1], _p(rho), _p(E), _p(ref), float(target),
+                                         int(max_shots), _p(B), _p(info)))
+        return B, dict(zip(INFO_NAMES, info.tolist()))
+
+    def adjoint(self, rho, B, dLdB, tol=1e-10, max_outer=50, sweeps=8):
+        """Returns (dL/dE, dL/drho, info)."""
+        B = _rhs(B, self.n)
+        rho = _rhs(rho, self.n)
+        g = _rhs(dLdB, self.n)
+        assert rho.shape == B.shape == g.shape
+        dE, drho = np.zeros_like(B), np.zeros_like(B)
+        info = np.zeros(8, np.float64)
+        self._ok(library().rad_lin_adjoint(self.h, B.shape[1], _p(rho), _p(B), _p(g), float(tol), int(max_outer),
+                                           int(sweeps), _p(dE), _p(drho), _p(info)))
+        return dE, drho, dict(zip(INFO_NAMES, info.tolist()))
 
 
 def run_cli(args, cwd=None, timeout=None):

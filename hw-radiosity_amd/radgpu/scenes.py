@@ -21,6 +21,9 @@ Scenes
   hall_scene(seed)  100 x 100 x 30 hall, 400 yawed boxes (sides U[1,8]),
                     8 emissive ceiling quads with emission U[0.5,2] RGB,
                     reflectance U[0.1,0.9]; about 200k triangles.
+  patch_room(seed, n)  the room_scene geometry (same walls, boxes,
+                    reflectances and light rectangle) triangulated to exactly
+                    n patches, for the explicit form-factor matrix.
 """
 from __future__ import annotations
 
@@ -163,6 +166,137 @@ def room_scene(seed: int = 1) -> Scene:
         cx, cy = rng.uniform(1 + a / 2, 9 - a / 2), rng.uniform(1 + b / 2, 7 - b / 2)
         _box(bld, rng, cx, cy, a, b, c, 1.0, 0.0, None, (0.2, 0.8))
     return bld.scene()
+
+
+def _room_cells(h: float, boxes) -> int:
+    """Grid cells of the room walls and box faces at cell size h."""
+    nx, ny, nz = _cells(10.0, h), _cells(8.0, h), _cells(6.0, h)
+    t = 2 * (nx * ny + ny * nz + nz * nx)
+    for a, b, c, _, _ in boxes:
+        t += 2 * _cells(a, h) * _cells(c, h) + 2 * _cells(b, h) * _cells(c, h) + _cells(a, h) * _cells(b, h)
+    return t
+
+
+def patch_room(seed: int = 1, n: int = 32768) -> Scene:
+    """room_scene(seed) re-triangulated to exactly n patches.
+
+    Walls and box faces are gridded with the largest common cell size whose
+    triangle count 2 * cells stays <= n; then the largest triangle (ties: the
+    lowest index) is bisected at the midpoint of its longest edge until there
+    are n.  Ceiling cells whose centre lies in room_scene's light rectangle
+    (x in [3.75, 6.25], y in [40/13, 64/13]) emit Brgb 255 255 255."""
+    if n < 64:
+        raise ValueError("patch_room needs n >= 64")
+    rng = SplitMix64(seed)
+    rho = (0.2, 0.8)
+    wall_f = [_frgb(rng, *rho) for _ in range(6)]
+    boxes, box_f = [], []
+    for _ in range(2):
+        a, b, c = rng.uniform(1, 3), rng.uniform(1, 3), rng.uniform(1, 3)
+        cx, cy = rng.uniform(1 + a / 2, 9 - a / 2), rng.uniform(1 + b / 2, 7 - b / 2)
+        boxes.append((a, b, c, cx, cy))
+        box_f.append([_frgb(rng, *rho) for _ in range(5)])
+    lo, hi = 0.001, 10.0
+    for _ in range(200):  # largest h with 2 * cells(h) <= n
+        mid = 0.5 * (lo + hi)
+        if 2 * _room_cells(mid, boxes) <= n:
+            hi = mid
+        else:
+            lo = mid
+    h = hi
+    bld = _Builder()
+
+    class _Feed:  # replays the drawn reflectances in room_scene's order
+        def __init__(self, seq):
+            self.seq = list(seq)
+
+        def uniform(self, lo_, hi_):
+            return self.seq.pop(0)
+
+    X, Y, Z = 10.0, 8.0, 6.0
+    nx, ny = _cells(X, h), _cells(Y, h)
+    lx0, lx1, ly0, ly1 = 3.75, 6.25, 40.0 / 13.0, 64.0 / 13.0
+
+    def ceil_b(a, c):
+        x, y = (a + 0.5) * X / nx, (c + 0.5) * Y / ny
+        return (255, 255, 255) if lx0 <= x <= lx1 and ly0 <= y <= ly1 else (0, 0, 0)
+
+    # the wall reflectances were drawn already: feed them back as Frgb
+    def inv(f):
+        return [(v - 0.5 + 0.25) / 510.0 for v in f]  # round(510 * u + 0.5) gives f back
+
+    feed = _Feed([u for f in wall_f for u in inv(f)])
+    _room_walls(bld, feed, X, Y, Z, h, (0.0, 1.0))
+    # relabel the ceiling lights (the builder kept walls in order floor, ceiling, ...)
+    nz = _cells(Z, h)
+    floor_n = 2 * nx * ny
+    for k in range(floor_n, floor_n + 2 * nx * ny):
+        cell = (k - floor_n) // 2
+        bld.b[k] = ceil_b(cell // ny, cell % ny)
+    for (a, b, c, cx, cy), fs in zip(boxes, box_f):
+        _box(bld, _Feed([u for f in fs for u in inv(f)]), cx, cy, a, b, c, 1.0, 0.0, h, (0.0, 1.0))
+    sc = bld.scene()
+    assert sc.n <= n and sc.n == 2 * _room_cells(h, boxes), (sc.n, n)
+    # bisect the largest triangles up to exactly n
+    import heapq
+    p = [sc.p[k].astype(np.float64) for k in range(sc.n)]
+    f, b = list(map(tuple, sc.frgb)), list(map(tuple, sc.brgb))
+
+    def area(t):
+        return 0.5 * float(np.linalg.norm(np.cross(t[1] - t[0], t[2] - t[0])))
+
+    heap = [(-area(t), k) for k, t in enumerate(p)]
+    heapq.heapify(heap)
+    while len(p) < n:
+        _, k = heapq.heappop(heap)
+        t = p[k]
+        e = [float(np.dot(t[(v + 1) % 3] - t[v], t[(v + 1) % 3] - t[v])) for v in range(3)]
+        v = int(np.argmax(e))
+        a, bb, c = t[v], t[(v + 1) % 3], t[(v + 2) % 3]
+        m = ((a + bb) * 0.5).astype(np.float32).astype(np.float64)
+        t1, t2 = np.stack([a, m, c]), np.stack([m, bb, c])
+        p[k] = t1
+        p.append(t2)
+        f.append(f[k])
+        b.append(b[k])
+        heapq.heappush(heap, (-area(t1), k))
+        heapq.heappush(heap, (-area(t2), len(p) - 1))
+    return Scene(np.stack(p).astype(np.float32), np.asarray(f, np.int32), np.asarray(b, np.int32))
+
+
+def lighting_vectors(rho, count: int = 64, lights: int = 8, seed: int = 7, lo: float = 0.5, hi: float = 2.0):
+    """count emission vectors x RGB = 3 * count right-hand sides: vector k
+    lights `lights` patches drawn without replacement (seeded), each with
+    emission U[lo, hi] per channel.  Returns (E, rho) of shape
+    (n, 3 * count); column 3k + c is channel c of vector k and carries
+    rho[:, c]."""
+    rho = np.asarray(rho, np.float64)
+    n = rho.shape[0]
+    rng = SplitMix64(seed)
+    E = np.zeros((n, 3 * count))
+    for k in range(count):
+        chosen = []
+        while len(chosen) < lights:
+            j = int(rng.next_u64() % n)
+            if j not in chosen:
+                chosen.append(j)
+        for j in chosen:
+            for c in range(3):
+                E[j, 3 * k + c] = rng.uniform(lo, hi)
+    return E, np.tile(rho, (1, count))
+
+
+def hidden_reflectance(n: int, seed: int, lo: float = 0.2, hi: float = 0.8):
+    """A seeded per-patch RGB reflectance U[lo, hi], shape (n, 3)."""
+    rng = SplitMix64(seed)
+    return np.array([[rng.uniform(lo, hi) for _ in range(3)] for _ in range(n)])
+
+
+def batch_sizes(count: int = 256, lo: int = 500, hi: int = 8000, seed: int = 4):
+    """count scene sizes drawn log-uniform in [lo, hi] (seeded)."""
+    rng = SplitMix64(seed)
+    import math
+    return [int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(count)]
 
 
 def hall_scene(seed: int = 2, n_boxes: int = 400, box_h: float = 0.65,

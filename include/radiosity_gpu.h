@@ -114,6 +114,94 @@ rad_status rad_counters(rad_ctx *ctx, long long *out);
  * device ms of the last rad_cast kernel. */
 rad_status rad_timings(rad_ctx *ctx, double *out7);
 
+/* ---- Explicit form-factor matrix: gathering, shooting and adjoint --------
+ *
+ * The reference has no matrix form of its solver; these entry points solve
+ * the radiosity system its progressive shooting converges to,
+ *     B = E + diag(rho) F B      (one column per right-hand side),
+ * with F[i][j] the form factor from the centroid of patch i to patch j as
+ * CalFF computes it (area-weighted point-to-patch term, shade.cpp:107-127,
+ * visibility by RayHitted, raycast.cpp:335-360), rows scaled to sum <= 1.
+ * Right-hand-side arrays are n x nrhs, row-major (patch-major), in double.
+ * A handle owns one HIP stream and the matrix (n rounded up to 256 squared,
+ * FP32, plus FP16/BF16/transposed copies made on demand). */
+typedef struct rad_lin rad_lin;
+
+enum { RAD_PREC_F32 = 0, RAD_PREC_F16 = 1, RAD_PREC_BF16 = 2 };
+
+/* info[8] written by the solvers */
+enum {
+	RAD_INFO_OUTER = 0,     /* refinement steps (gather, adjoint) or shots (shoot) */
+	RAD_INFO_SWEEPS = 1,    /* low-precision sweeps */
+	RAD_INFO_REL_RES = 2,   /* final FP64 relative residual */
+	RAD_INFO_MS = 3,        /* solve time, ms */
+	RAD_INFO_MS_TARGET = 4, /* time when the error against Bref first met target (-1: never / no Bref) */
+	RAD_INFO_ERR = 5,       /* final relative L2 error against Bref (-1: no Bref) */
+	RAD_INFO_POWER_E = 6,   /* sum_i area_i E_i over all columns (FP64) */
+	RAD_INFO_POWER_B = 7,   /* sum_i area_i B_i (FP64) */
+	RAD_INFO_N = 8
+};
+
+rad_status rad_lin_create(rad_lin **out, int n);
+void rad_lin_destroy(rad_lin *lin);
+const char *rad_lin_last_error(const rad_lin *lin);
+
+/* F and patch areas from an uploaded scene (n must equal its slot count;
+ * logical slots get zero rows and columns).  F[i][j] is CalFF's value from
+ * the centroid of i to the whole patch j (shade.cpp:107-123) times a
+ * visibility:
+ *   RAD_VIS_SEGMENT    nothing crosses the open segment between the two
+ *                      centroids (parameter in (1e-4, 1 - 1e-4)), found by a
+ *                      grid walk over complete octree leaf lists; needs a
+ *                      scene that pre-partition left unsplit
+ *   RAD_VIS_REFERENCE  RayHitted from the centroid returns j
+ *                      (raycast.cpp:335-360), blind spots included
+ * *ms (may be NULL) = device time of the form-factor kernel.  Replaces the
+ * per-pair CalFF of Shade (shade.cpp:435-554) with one n x n evaluation. */
+enum { RAD_VIS_SEGMENT = 0, RAD_VIS_REFERENCE = 1 };
+rad_status rad_lin_from_scene(rad_lin *lin, rad_ctx *scene, int visibility, double *ms);
+rad_status rad_lin_set_matrix(rad_lin *lin, const float *F);   /* n x n row-major */
+rad_status rad_lin_get_matrix(rad_lin *lin, float *F);
+rad_status rad_lin_set_areas(rad_lin *lin, const double *area); /* power sums and shooting's source order */
+
+/* Picks the MFMA tile (rows per wave rt*16, split-K factor) for nrhs columns
+ * by timing every candidate on this matrix; gather does it on first use. */
+rad_status rad_lin_autotune(rad_lin *lin, int nrhs, int prec, int *rt, int *splitk);
+/* Forces the tile for nrhs columns instead of the autotuner's choice:
+ * rt in {1, 2, 4} (rt = 4 only up to 128 columns), splitk in {1, 2, 4, 8},
+ * subject to the matrix size. */
+rad_status rad_lin_set_tile(rad_lin *lin, int nrhs, int prec, int rt, int splitk);
+/* Average device time of one low-precision product F X, reps launches. */
+rad_status rad_lin_gemm_ms(rad_lin *lin, int nrhs, int prec, int reps, double *ms);
+/* Y = F X: RAD_PREC_F32 in FP64 arithmetic on the FP32 matrix; FP16/BF16
+ * through the MFMA path (X rounded to that precision, FP32 accumulation). */
+rad_status rad_lin_multiply(rad_lin *lin, int prec, int nrhs, const double *X, double *Y);
+
+/* Mixed-precision gathering: Jacobi sweeps d <- r + rho o (F d) with F in
+ * FP16 or BF16 on MFMA (FP32 accumulation), inside iterative refinement whose
+ * residual E + rho o (F x) - x and energy sums are FP64 over the FP32 F.
+ * Stops when ||r|| / ||E|| <= tol or after max_outer refinements.  Bref (may
+ * be NULL) is a reference solution; info reports the error against it and
+ * the time at which it first fell to target.  nrhs 1..192 in one GEMM. */
+rad_status rad_lin_gather(rad_lin *lin, int prec, int nrhs, const double *rho, const double *E, double tol,
+                          int max_outer, int sweeps, const double *Bref, double target, double *B, double *info);
+
+/* Plain FP32 progressive shooting on the same system, the reference's method
+ * (source = largest unshot power area_j * sum_c dB_j, ties to the lowest
+ * index as MaxDeltaRad, rad.cpp:359-386; update rho_i F_ij dB_j as
+ * CalRadiosity, shade.cpp:244-259), one device launch per shot.  Stops when
+ * no unshot power is left, after max_shots, or when the error against Bref
+ * (if given) reaches target.  nrhs 1..16. */
+rad_status rad_lin_shoot(rad_lin *lin, int nrhs, const double *rho, const double *E, const double *Bref,
+                         double target, int max_shots, double *B, double *info);
+
+/* Adjoint of the solve for a loss L(B): solves (I - diag(rho) F)^T lambda =
+ * dL/dB (FP32 sweeps, FP64 residual refinement, no stored iterates) and
+ * returns dL/dE = lambda and dL/drho = lambda o (F B), visibility fixed.
+ * nrhs 1..4. */
+rad_status rad_lin_adjoint(rad_lin *lin, int nrhs, const double *rho, const double *B, const double *dLdB, double tol,
+                           int max_outer, int sweeps, double *dLdE, double *dLdrho, double *info);
+
 #ifdef __cplusplus
 }
 #endif

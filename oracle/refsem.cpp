@@ -168,6 +168,8 @@ struct Counters {
 };
 
 // A half of the source (PartitionSource, shade.cpp:76-99) or the whole source.
+constexpr float kSegLo = 0.0001f, kSegHi = 0.9999f;  // open segment of the matrix visibility
+
 struct SrcPatch {
 	float c[3], n[3], area;
 };
@@ -192,6 +194,7 @@ struct rs_ctx {
 	std::vector<int> lists;
 	float min_grid = 0;
 	int grid_num = 0;
+	int n_orig = 0;  // originals the octree was built over
 	int loop = 0;
 	bool warned_full = false;
 	char fmt_buf[64] = {0};  // the reference's static buffer (rad.cpp:179)
@@ -363,6 +366,8 @@ struct rs_ctx {
 	{
 		nodes.clear();
 		lists.clear();
+		fine.clear();
+		n_orig = trinum;
 		std::vector<int> all(trinum);
 		for (int i = 0; i < trinum; i++) all[i] = i;
 		min_grid = G1[0] - G0[0];
@@ -414,6 +419,11 @@ struct rs_ctx {
 	{
 		float t = 0.f, q[3] = {p[0], p[1], p[2]};
 		int prev = -1, cur, steps = 0;
+		// The leaf searched next is a function of the leaf searched last
+		// (hits and the exit point depend on the ray and that leaf only), so
+		// a leaf searched twice means the reference cycles forever: Brent's
+		// cycle detection reports such a ray at once.
+		int tort = -1, power = 1, lam = 0;
 		c[RS_CNT_RAYS]++;
 		while ((cur = cell_of(q, &c[RS_CNT_LEVELS])) >= 0) {
 			// The reference has no bound on this walk.  Report a ray instead
@@ -439,6 +449,8 @@ struct rs_ctx {
 				if (tri_hit<F>(p, v, T[id], &tmin)) best = id;
 			}
 			if (best >= 0) return best;
+			if (cur == tort) { c[RS_CNT_CAPPED]++; return -2; }
+			if (++lam == power) { tort = cur; power <<= 1; lam = 0; }
 			prev = cur;
 			// NextPoint (raycast.cpp:235-251)
 			float tm = 1000000.0f;
@@ -453,6 +465,143 @@ struct rs_ctx {
 			t = tm;
 		}
 		return -1;
+	}
+
+	// ---------------- segment visibility of the explicit matrix ----------------
+	// Not a reference function: the gathering matrix asks whether the open
+	// segment between two centroids is blocked.  Point location is
+	// geometric (the 2^D grid of the deepest level; a cell descends by the
+	// bits of its coordinates) and the walk is Amanatides-Woo over that grid;
+	// a leaf's list is tested once (its cells are consecutive on a segment).
+	std::vector<int> fine;
+	int fine_r = 0;
+	void build_fine()
+	{
+		if (!fine.empty() || nodes.empty()) return;
+		int D = 0;
+		std::vector<std::pair<int, int>> stack{{0, 0}};
+		while (!stack.empty()) {
+			auto [nd, d] = stack.back();
+			stack.pop_back();
+			if (nodes[nd].list >= 0) { D = std::max(D, d); continue; }
+			for (int k = 0; k < 8; k++) stack.push_back({nodes[nd].sub[k], d + 1});
+		}
+		fine_r = 1 << D;
+		const int R = fine_r;
+		fine.assign((size_t)R * R * R, -1);
+		for (int a = 0; a < R; a++)
+			for (int b = 0; b < R; b++)
+				for (int c = 0; c < R; c++) {
+					int nd = 0;
+					for (int bit = D - 1; nodes[nd].list < 0; bit--)
+						nd = nodes[nd].sub[((a >> bit) & 1) | (((b >> bit) & 1) << 1) | (((c >> bit) & 1) << 2)];
+					fine[((size_t)a * R + b) * R + c] = nd;
+				}
+		// complete lists (the reference's CrossOver/Clip listing misses
+		// triangles that cross a box through their interior): an original
+		// goes to every leaf holding a fine cell its bounding box, grown by
+		// 1e-3 of a cell, touches
+		seg.assign(nodes.size(), {});
+		for (int i = 0; i < n_orig; i++) {
+			int lo[3], hi[3];
+			for (int k = 0; k < 3; k++) {
+				const double h = ((double)G1[k] - (double)G0[k]) / R;
+				double mn = T[i].p[0][k], mx = mn;
+				for (int v = 1; v < 3; v++) {
+					mn = std::min(mn, (double)T[i].p[v][k]);
+					mx = std::max(mx, (double)T[i].p[v][k]);
+				}
+				lo[k] = std::max(0, (int)std::floor((mn - G0[k]) / h - 1e-3));
+				hi[k] = std::min(R - 1, (int)std::floor((mx - G0[k]) / h + 1e-3));
+			}
+			for (int a = lo[0]; a <= hi[0]; a++)
+				for (int b = lo[1]; b <= hi[1]; b++)
+					for (int c = lo[2]; c <= hi[2]; c++) {
+						std::vector<int> &L = seg[fine[((size_t)a * R + b) * R + c]];
+						if (L.empty() || L.back() != i) L.push_back(i);
+					}
+		}
+	}
+	std::vector<std::vector<int>> seg;  // complete list per leaf node
+	// one original against the open segment (the walk's and the brute test)
+	template <bool F> bool seg_hits(const float *p, const float *v, const Rec &t) const
+	{
+		const float a1 = dot3<F>(v, t.n);
+		const float vv[3] = {t.c[0] - p[0], t.c[1] - p[1], t.c[2] - p[2]};
+		const float a2 = dot3<F>(vv, t.n);
+		const float tt = a2 / a1;
+		if (!(tt > kSegLo && tt < kSegHi)) return false;
+		float q[3];
+		for (int k = 0; k < 3; k++) q[k] = axpy<F>(tt, v[k], p[k]);
+		for (int s2 = 0; s2 < 3; s2++)
+			if (dot3<F>(q, t.se[s2]) + t.se[s2][3] < 0.0f) return false;
+		return true;
+	}
+	template <bool F> bool seg_brute(const float *p, const float *v, int ban0, int ban1) const
+	{
+		for (int id = 0; id < n_orig; id++)
+			if (id != ban0 && id != ban1 && seg_hits<F>(p, v, T[id])) return true;
+		return false;
+	}
+	template <bool F>
+	bool seg_occluded(const float *p, const float *v, int ban0, int ban1, long long *c) const
+	{
+		const int R = fine_r;
+		int cell[3], step[3];
+		float tmax[3], tdelta[3];
+		for (int k = 0; k < 3; k++) {
+			const float h = (G1[k] - G0[k]) / (float)R, inv = (float)R / (G1[k] - G0[k]);
+			const float rel = p[k] - G0[k];
+			int ck = (int)floorf(rel * inv);
+			ck = ck < 0 ? 0 : (ck >= R ? R - 1 : ck);
+			cell[k] = ck;
+			if (v[k] > 0.0f) {
+				step[k] = 1;
+				tmax[k] = ((float)(ck + 1) * h - rel) / v[k];
+				tdelta[k] = h / v[k];
+			} else if (v[k] < 0.0f) {
+				step[k] = -1;
+				tmax[k] = ((float)ck * h - rel) / v[k];
+				tdelta[k] = -h / v[k];
+			} else {
+				step[k] = 0;
+				tmax[k] = tdelta[k] = INFINITY;
+			}
+		}
+		c[RS_CNT_RAYS]++;
+		int last = -1;
+		for (;;) {
+			const int nd = fine[((size_t)cell[0] * R + cell[1]) * R + cell[2]];
+			if (nd != last) {
+				last = nd;
+				c[RS_CNT_CELLS]++;
+				for (int id : seg[nd]) {
+					if (id == ban0 || id == ban1) continue;
+					c[RS_CNT_TESTS]++;
+					if (seg_hits<F>(p, v, T[id])) return true;
+				}
+			}
+			const int ax = tmax[0] <= tmax[1] ? (tmax[0] <= tmax[2] ? 0 : 2) : (tmax[1] <= tmax[2] ? 1 : 2);
+			if (tmax[ax] >= 1.0f) return false;
+			cell[ax] += step[ax];
+			if (cell[ax] < 0 || cell[ax] >= R) return false;
+			tmax[ax] += tdelta[ax];
+		}
+	}
+	// CalFF's value without the ray (shade.cpp:107-123)
+	template <bool F> float ff_value(const SrcPatch &s, const float *dn, const float *p) const
+	{
+		float dir[3] = {s.c[0] - p[0], s.c[1] - p[1], s.c[2] - p[2]};
+		float ds = dot3<F>(dir, s.n);
+		float dd = -dot3<F>(dir, dn);
+		if ((ds < 0) != (dd < 0)) return 0.0f;
+		float dl = norm<F>(dir), sl = norm<F>(s.n), el = norm<F>(dn);
+		float c1 = ds / dl / sl;
+		float c2 = dd / dl / el;
+		float ff = c1 * c2;
+		if (ff <= 0.0f) return 0.0f;
+		ff = (float)((double)ff * ((double)s.area / ((double)norm2<F>(dir) * kPi)));
+		return ff <= 0.0f ? 0.0f : ff;
 	}
 
 	// ---------------- form factor (shade.cpp:76-168) ----------------
@@ -1054,6 +1203,84 @@ int rs_ff_pairs(rs_ctx *c, int n, const int *pairs, float *out3)
 	}
 	return 0;
 }
+// Gathering form factor of the explicit matrix: F[i][j] = CalFF from the
+// centroid of receiver i (normal n_i, ban = its logical index) to the whole
+// patch j (no AdaptCalFF halving), visible iff RayHitted returns j's logical
+// index (shade.cpp:107-127, raycast.cpp:335-360); 0 on the diagonal and for
+// logical slots.
+static float centroid_ff(rs_ctx *c, int i, int j, int vis, long long *k)
+{
+	const Rec &d = c->T[i], &s = c->T[j];
+	if (i == j || d.parent < 0 || s.parent < 0) return 0.0f;
+	SrcPatch w;
+	memcpy(w.c, s.c, 12);
+	memcpy(w.n, s.n, 12);
+	w.area = s.area;
+	if (vis == RS_VIS_REFERENCE)
+		return c->o.fma ? c->cal_ff<true>(w, s.parent, d.n, d.parent, d.c, k)
+		                : c->cal_ff<false>(w, s.parent, d.n, d.parent, d.c, k);
+	const float ff = c->o.fma ? c->ff_value<true>(w, d.n, d.c) : c->ff_value<false>(w, d.n, d.c);
+	if (ff == 0.0f || vis == RS_VIS_NONE) return ff;
+	const float v[3] = {s.c[0] - d.c[0], s.c[1] - d.c[1], s.c[2] - d.c[2]};
+	bool occ;
+	if (vis == RS_VIS_BRUTE)
+		occ = c->o.fma ? c->seg_brute<true>(d.c, v, d.parent, s.parent) : c->seg_brute<false>(d.c, v, d.parent, s.parent);
+	else
+		occ = c->o.fma ? c->seg_occluded<true>(d.c, v, d.parent, s.parent, k)
+		               : c->seg_occluded<false>(d.c, v, d.parent, s.parent, k);
+	return occ ? 0.0f : ff;
+}
+
+int rs_ff_centroid(rs_ctx *c, int vis, int n, const int *pairs, float *out)
+{
+	c->build_fine();
+#pragma omp parallel for schedule(dynamic, 256)
+	for (int q = 0; q < n; q++) {
+		long long k[RS_CNT_N] = {0};
+		out[q] = centroid_ff(c, pairs[2 * q], pairs[2 * q + 1], vis, k);
+	}
+	return 0;
+}
+
+// Rows of the matrix for receivers rows[0..nr): out[r * n + j].  A row whose
+// sum exceeds 1 is scaled by (float)(1 / sum); the sum is formed in double in
+// the engine kernel's fixed order: 256 lane sums over j = lane (mod 256) in
+// ascending j, an xor butterfly (offsets 32, 16, .., 1) inside each group of
+// 64 lanes, then the four group sums left to right.
+int rs_ff_rows(rs_ctx *c, int vis, int nr, const int *rows, float *out)
+{
+	const int n = c->trinum;
+	c->build_fine();
+#pragma omp parallel for schedule(dynamic, 1)
+	for (int r = 0; r < nr; r++) {
+		const int i = rows[r];
+		float *row = out + (size_t)r * n;
+		long long k[RS_CNT_N] = {0};
+		double lane[256] = {0};
+		for (int j = 0; j < n; j++) {
+			row[j] = centroid_ff(c, i, j, vis, k);
+			lane[j & 255] += (double)row[j];
+		}
+		double wsum[4];
+		for (int w = 0; w < 4; w++) {
+			double v[64];
+			for (int l = 0; l < 64; l++) v[l] = lane[64 * w + l];
+			for (int off = 32; off > 0; off >>= 1) {
+				double nv[64];
+				for (int l = 0; l < 64; l++) nv[l] = v[l] + v[l ^ off];
+				memcpy(v, nv, sizeof v);
+			}
+			wsum[w] = v[0];
+		}
+		const double total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+		if (total > 1.0) {
+			const float scale = (float)(1.0 / total);
+			for (int j = 0; j < n; j++) row[j] *= scale;
+		}
+	}
+	return 0;
+}
+
 int rs_shade_pairs(rs_ctx *c, int n, const int *pairs, float *out9)
 {
 	for (int i = 0; i < n; i++) {

@@ -78,6 +78,19 @@ int rs_raycast(rs_ctx *, int n, const void *rays, int *hit, int *cells, int *tes
 int rs_shade_pairs(rs_ctx *, int n, const int *pairs, float *out9);
 /* Per-vertex form factors ff[v] for (src, dst) pairs, as Shade computes them. */
 int rs_ff_pairs(rs_ctx *, int n, const int *pairs, float *out3);
+/* Explicit gathering matrix: F[i][j] = CalFF's value from the centroid of i
+ * to the whole patch j, times its visibility:
+ *   RS_VIS_SEGMENT    no other original crosses the open segment between the
+ *                     centroids, t in (1e-4, 1 - 1e-4) (grid walk with
+ *                     geometric point location; not a reference function)
+ *   RS_VIS_REFERENCE  RayHitted from the centroid returns j's logical index
+ *   RS_VIS_NONE       always visible (checks)
+ *   RS_VIS_BRUTE      RS_VIS_SEGMENT's test against every original (checks)
+ * for (i, j) pairs (unscaled)... */
+enum { RS_VIS_SEGMENT = 0, RS_VIS_REFERENCE = 1, RS_VIS_NONE = 2, RS_VIS_BRUTE = 3 };
+int rs_ff_centroid(rs_ctx *, int vis, int n, const int *pairs, float *out);
+/* ...and whole rows (n = trinum entries each), scaled to sum <= 1. */
+int rs_ff_rows(rs_ctx *, int vis, int nrows, const int *rows, float *out);
 void rs_counters(rs_ctx *, long long *out);
 void rs_reset_counters(rs_ctx *);
 

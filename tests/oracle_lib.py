@@ -21,6 +21,7 @@ assert REC_DTYPE.itemsize == 236
 RAY_DTYPE = np.dtype([("p", "<f4", 3), ("v", "<f4", 3), ("ban", "<i4"), ("target", "<i4")])
 NODE_DTYPE = np.dtype([("g0", "<f4", 3), ("g1", "<f4", 3), ("sub", "<i4", 8), ("list", "<i4")])
 
+VIS_SEGMENT, VIS_REFERENCE, VIS_NONE, VIS_BRUTE = 0, 1, 2, 3
 CNT_RAYS, CNT_CELLS, CNT_TESTS, CNT_LEVELS, CNT_PRECISION, CNT_SHADES, CNT_ITERS = range(7)
 
 _lib = None
@@ -43,7 +44,8 @@ def lib():
                            ("rs_tree_info", [vp, vp, ctypes.POINTER(ctypes.c_float)]),
                            ("rs_descend", [vp, ip, ip, ip]), ("rs_tree_arrays", [vp, vp, vp]),
                            ("rs_raycast", [vp, ip, vp, vp, vp, vp]), ("rs_shade_pairs", [vp, ip, vp, vp]),
-                           ("rs_ff_pairs", [vp, ip, vp, vp])]:
+                           ("rs_ff_pairs", [vp, ip, vp, vp]), ("rs_ff_centroid", [vp, ip, ip, vp, vp]),
+                           ("rs_ff_rows", [vp, ip, ip, vp, vp])]:
             f = getattr(L, name)
             f.restype = ip
             f.argtypes = args
@@ -150,6 +152,20 @@ class Oracle:
         pairs = np.ascontiguousarray(pairs, np.int32)
         out = np.zeros((len(pairs), 3), np.float32)
         lib().rs_ff_pairs(self.h, len(pairs), _ptr(pairs), _ptr(out))
+        return out
+
+    def ff_centroid(self, pairs, vis=VIS_SEGMENT):
+        """Unscaled gathering form factors F[i][j] for (i, j) pairs."""
+        pairs = np.ascontiguousarray(pairs, np.int32)
+        out = np.zeros(len(pairs), np.float32)
+        lib().rs_ff_centroid(self.h, int(vis), len(pairs), _ptr(pairs), _ptr(out))
+        return out
+
+    def ff_rows(self, rows, vis=VIS_SEGMENT):
+        """Rows of the explicit matrix (scaled to sum <= 1), shape (len(rows), trinum)."""
+        rows = np.ascontiguousarray(rows, np.int32)
+        out = np.zeros((len(rows), self.trinum), np.float32)
+        lib().rs_ff_rows(self.h, int(vis), len(rows), _ptr(rows), _ptr(out))
         return out
 
     def counters(self):
