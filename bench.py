@@ -22,6 +22,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hw-radiosity_amd"))
 
@@ -29,6 +31,7 @@ METRIC = "rays/s + time-to-converge at 1e-3 rel-L2 vs FP64; mixed/batched vs pla
 OPTS = ["-converge", "1200", "-delta_ff", "5e-6", "-adapt_area", "5", "-triangle", "1000000",
         "-write_cycle", "1000000"]
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = 2500.0  # dense FP16/BF16
 # measured HBM bytes per k_trace dispatch on this workload (rocprofv3 --pmc
 # FETCH_SIZE + WRITE_SIZE, copy-calibrated); see profiles/
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_trace_hall200k_pmc.json")
@@ -65,6 +68,104 @@ def cpu_baseline(scene, seconds, cores):
     return {"rays": int(c[ol.CNT_RAYS]), "seconds": dt, "shots": shots}
 
 
+def dense_reference(F, rho, E, tmp):
+    """FP64 dense LU solve on the GPU in a child process (radgpu.dense_ref)."""
+    import subprocess
+    np.save(os.path.join(tmp, "F.npy"), F)
+    np.save(os.path.join(tmp, "rho.npy"), rho)
+    np.save(os.path.join(tmp, "E.npy"), E)
+    env = dict(os.environ, PYTHONPATH=os.path.join(ROOT, "hw-radiosity_amd"))
+    r = subprocess.run([sys.executable, "-m", "radgpu.dense_ref", tmp], env=env, capture_output=True, text=True,
+                       timeout=600)
+    log(r.stdout.strip() or r.stderr[-2000:])
+    if r.returncode != 0:
+        raise RuntimeError("dense reference failed: %s" % r.stderr[-2000:])
+    return np.load(os.path.join(tmp, "B.npy"))
+
+
+def linear_leg(n=32768, n_adj=16384):
+    """Explicit-matrix solvers on the patch room: the mixed-precision gather
+    (FP16 / BF16 MFMA, FP64 refinement) against plain FP32 progressive
+    shooting, time to 1e-3 relative L2 against an FP64 dense solve; 192
+    right-hand sides in one GEMM; the adjoint."""
+    import radgpu
+    from radgpu import scenes
+    out = {}
+    t0 = time.perf_counter()
+    sc = scenes.patch_room(1, n)
+    eng = radgpu.Engine("-triangle", str(n), "-quiet", "1")
+    eng.set_scene(sc).prepare_host().upload()
+    lin = radgpu.LinearSystem(n)
+    ff_ms = lin.from_scene(eng)
+    F = lin.matrix()
+    E, rho = scenes.emission_reflectance(sc.frgb, sc.brgb)
+    area = eng.records()[0]["area"].astype(np.float64)
+    log("linear: n=%d, F built in %.1f ms (%.3f nonzero), setup %.1f s" % (n, ff_ms, (F > 0).mean(),
+                                                                          time.perf_counter() - t0))
+    E192, rho192 = scenes.lighting_vectors(rho, count=64, lights=8, seed=7)
+    with tempfile.TemporaryDirectory() as tmp:
+        ref = dense_reference(F, np.hstack([rho, rho192]), np.hstack([E, E192]), tmp)
+    Bref, Bref192 = ref[:, :3], ref[:, 3:]
+    out["scene"] = {"workload": "patch room (room_scene seed 1 geometry) at %d patches" % n,
+                    "nonzero_fraction": float((F > 0).mean()), "form_factor_ms": ff_ms,
+                    "form_factor_rays_per_s": float(eng.counters()["rays"]) / (ff_ms * 1e-3),
+                    "reference": "FP64 dense LU (torch.linalg.solve on the GPU, separate process)"}
+    res = {}
+    for prec in ("f16", "bf16"):
+        lin.autotune(3, prec)
+        lin.gather(rho, E, prec=prec, tol=1e-6, max_outer=40, sweeps=8)  # warm
+        B, info = lin.gather(rho, E, prec=prec, tol=1e-6, max_outer=40, sweeps=8, Bref=Bref, target=1e-3)
+        info["tile"] = lin.autotune(3, prec)
+        res["gather_" + prec] = info
+        log("gather %s: %s" % (prec, info))
+    B, info = lin.shoot(rho, E, Bref=Bref, target=1e-3)
+    res["shoot_f32"] = info
+    log("shoot: %s" % info)
+    res["speedup_gather_f16_vs_shoot_at_1e-3"] = info["ms_target"] / res["gather_f16"]["ms_target"]
+    res["speedup_gather_bf16_vs_shoot_at_1e-3"] = info["ms_target"] / res["gather_bf16"]["ms_target"]
+    out["precision"] = res
+    # roofline of the low-precision product, per sweep: algorithmic bytes
+    # 2 N^2 (F in FP16/BF16) + 8 N R (X in, Y out) and flops 2 N^2 R
+    roof = {}
+    for nrhs in (3, 192):
+        for prec in ("f16", "bf16"):
+            tile = lin.autotune(nrhs, prec)
+            ms = lin.gemm_ms(nrhs, prec, reps=30)
+            byts = 2.0 * n * n + 8.0 * n * nrhs
+            flops = 2.0 * n * n * nrhs
+            roof["%s_%d" % (prec, nrhs)] = {"ms": ms, "achieved_GBps": byts / ms / 1e6,
+                                           "hbm_frac": byts / ms / 1e6 / HBM_PEAK_GBS,
+                                           "achieved_TFLOPs": flops / ms / 1e9,
+                                           "mfma_frac": flops / ms / 1e9 / MFMA_PEAK_TFLOPS,
+                                           "tile_rows_per_wave_x16_splitk": tile}
+    out["gemm_roofline"] = roof
+    # batching: 64 lighting vectors x RGB in one GEMM vs one vector at a time
+    B, info = lin.gather(rho192, E192, prec="f16", tol=1e-6, max_outer=40, sweeps=8, Bref=Bref192, target=1e-3)
+    _, one = lin.gather(rho192[:, :3], E192[:, :3], prec="f16", tol=1e-6, max_outer=40, sweeps=8,
+                        Bref=Bref192[:, :3], target=1e-3)
+    out["batching_192_rhs"] = {"gather_f16": info, "one_vector_ms_target": one["ms_target"],
+                               "speedup_vs_64_single_vector_solves": 64 * one["ms_target"] / info["ms_target"]}
+    log("batched 192: %s" % info)
+    lin.close()
+    eng.close()
+    # adjoint at n_adj
+    sc = scenes.patch_room(5, n_adj)
+    eng = radgpu.Engine("-triangle", str(n_adj), "-quiet", "1")
+    eng.set_scene(sc).prepare_host().upload()
+    lin = radgpu.LinearSystem(n_adj)
+    lin.from_scene(eng)
+    E, rho = scenes.emission_reflectance(sc.frgb, sc.brgb)
+    rho_t = scenes.hidden_reflectance(n_adj, seed=11)
+    Bt, _ = lin.gather(rho_t, E, prec="f16", tol=1e-10, max_outer=60, sweeps=8)
+    B, _ = lin.gather(rho, E, prec="f16", tol=1e-10, max_outer=60, sweeps=8)
+    dE, drho, info = lin.adjoint(rho, B, 2 * (B - Bt), tol=1e-8, max_outer=40, sweeps=8)
+    out["adjoint"] = {"n": n_adj, "loss": float(np.sum((B - Bt) ** 2)), "info": info}
+    log("adjoint: %s" % info)
+    lin.close()
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +174,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--boxes", type=int, default=400)
+    ap.add_argument("--no-linear", action="store_true", help="skip the explicit-matrix leg")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -158,6 +260,9 @@ def main():
                              "1 means the kernel is latency/divergence-bound, not HBM-bound"},
         "counters": cnt_sum,
     }
+    eng.close()
+    if not args.no_linear:
+        out["linear"] = linear_leg()
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
         cb = cpu_baseline(scene, args.cpu_seconds, cores)

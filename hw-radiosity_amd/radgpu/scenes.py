@@ -264,6 +264,17 @@ def patch_room(seed: int = 1, n: int = 32768) -> Scene:
     return Scene(np.stack(p).astype(np.float32), np.asarray(f, np.int32), np.asarray(b, np.int32))
 
 
+def emission_reflectance(frgb, brgb, light: float = 1.0):
+    """Right-hand side and reflectance of the matrix system for a .tri scene:
+    rho = Frgb * 0.5 / 255 (shade.cpp:246); a triangle emits iff a Brgb
+    channel is 255 (rad.h:47-49), with E = light * Brgb / 255.  (n, 3) each."""
+    frgb = np.asarray(frgb, np.float64)
+    brgb = np.asarray(brgb, np.float64)
+    rho = frgb * 0.5 / 255.0
+    emits = (brgb == 255).any(axis=1)
+    return np.where(emits[:, None], light * brgb / 255.0, 0.0), rho
+
+
 def lighting_vectors(rho, count: int = 64, lights: int = 8, seed: int = 7, lo: float = 0.5, hi: float = 2.0):
     """count emission vectors x RGB = 3 * count right-hand sides: vector k
     lights `lights` patches drawn without replacement (seeded), each with

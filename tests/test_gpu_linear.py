@@ -7,7 +7,7 @@ refined gather meets its FP64 residual and agrees with the FP64 dense-LU
 oracle; plain shooting equals the oracle's float32 restatement bit for bit;
 the adjoint agrees with the oracle's FP64 adjoint and with FP64 central
 finite differences; full-size systems are checked through their FP64
-residual, recomputed independently in torch float64."""
+residual, recomputed independently on the host."""
 import os
 import sys
 
@@ -237,16 +237,15 @@ def test_adjoint_through_torch_autograd(sys2k):
 
 # ---- full sizes -------------------------------------------------------------
 
-def _torch_residual(F, rho, E, B):
-    import torch
-    dev = torch.device("cuda:0")
-    Ft = torch.from_numpy(F).to(dev).double()
-    r = torch.from_numpy(E).to(dev) + torch.from_numpy(rho).to(dev) * (Ft @ torch.from_numpy(B).to(dev)) - \
-        torch.from_numpy(B).to(dev)
-    out = float(torch.linalg.norm(r) / torch.linalg.norm(torch.from_numpy(E).to(dev)))
-    del Ft
-    torch.cuda.empty_cache()
-    return out
+def _host_residual(F, rho, E, B, chunk=2048):
+    """||E + rho o (F B) - B|| / ||E|| in float64 on the host, by row chunks
+    (independent of the device)."""
+    ss = 0.0
+    for r0 in range(0, F.shape[0], chunk):
+        FB = F[r0:r0 + chunk].astype(np.float64) @ B
+        r = E[r0:r0 + chunk] + rho[r0:r0 + chunk] * FB - B[r0:r0 + chunk]
+        ss += float(np.sum(r * r))
+    return np.sqrt(ss) / np.linalg.norm(E)
 
 
 @pytest.mark.parametrize("n", [32768, 16384])
@@ -262,7 +261,7 @@ def test_full_size_gather_residual_and_sampled_rows(n):
         B, info = lin.gather(rho, E, prec=prec, tol=1e-6, max_outer=40, sweeps=8)
         print(prec, info, flush=True)
         assert info["rel_res"] <= 1e-6
-        assert _torch_residual(F, rho, E, B) <= 1.5e-6
+        assert _host_residual(F, rho, E, B) <= 1.5e-6
         assert abs(info["power_B"] - float(np.sum(area[:, None] * B))) <= 1e-9 * info["power_B"]
     lin.close()
     eng.close()
