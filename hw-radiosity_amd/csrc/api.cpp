@@ -10,6 +10,7 @@
 #include "../../include/radiosity_gpu.h"
 #include "device.h"
 #include "fin.h"
+#include "batch.h"
 #include "linear.h"
 #include "octree.h"
 #include "scene.h"
@@ -28,6 +29,11 @@ struct rad_ctx {
 
 struct rad_lin {
 	std::unique_ptr<rgpu::LinearSystem> sys;
+	std::string err;
+};
+
+struct rad_batch {
+	std::unique_ptr<rgpu::SceneBatch> b;
 	std::string err;
 };
 
@@ -399,6 +405,74 @@ rad_status rad_lin_adjoint(rad_lin *l, int nrhs, const double *rho, const double
 		return l ? lfail(l, RAD_EINVAL, "adjoint: bad arguments") : RAD_EINVAL;
 	return l->sys->adjoint(nrhs, rho, B, dLdB, tol, max_outer, sweeps, dLdE, dLdrho, info, l->err) ? RAD_EDEVICE
 	                                                                                                : RAD_OK;
+}
+
+/* ---- many scenes ----------------------------------------------------------- */
+
+rad_status rad_batch_create(rad_batch **out, int count, const int *n)
+{
+	if (!out || count <= 0 || !n) return RAD_EINVAL;
+	*out = nullptr;
+	std::string err;
+	SceneBatch *b = SceneBatch::create(count, n, err);
+	if (!b) {
+		std::fprintf(stderr, "rad_batch_create: %s\n", err.c_str());
+		return err.find("device") != std::string::npos || err.find("hip") != std::string::npos ? RAD_EDEVICE
+		                                                                                       : RAD_EINVAL;
+	}
+	std::unique_ptr<rad_batch> h(new rad_batch());
+	h->b.reset(b);
+	*out = h.release();
+	return RAD_OK;
+}
+
+void rad_batch_destroy(rad_batch *b) { delete b; }
+
+const char *rad_batch_last_error(const rad_batch *b) { return b ? b->err.c_str() : "null handle"; }
+
+rad_status rad_batch_from_scene(rad_batch *b, int scene, rad_ctx *c, int visibility, double *ms)
+{
+	if (!b || !c) return RAD_EINVAL;
+	if (scene < 0 || scene >= b->b->count()) {
+		b->err = "scene index out of range";
+		return RAD_EINVAL;
+	}
+	if (visibility != RAD_VIS_SEGMENT && visibility != RAD_VIS_REFERENCE) {
+		b->err = "unknown visibility mode";
+		return RAD_EINVAL;
+	}
+	if (!c->uploaded) {
+		b->err = "rad_upload the scene first";
+		return RAD_ESTATE;
+	}
+	const HostScene &s = c->scene;
+	if (s.trinum != b->b->n(scene)) {
+		b->err = "scene " + std::to_string(scene) + " has " + std::to_string(b->b->n(scene)) + " patches, the context " +
+		         std::to_string(s.trinum);
+		return RAD_EINVAL;
+	}
+	if (c->dev->form_factors(b->b->matrix32(scene), s.trinum, b->b->ld(scene), visibility, c->tree, s, b->err))
+		return RAD_EDEVICE;
+	if (ms) *ms = c->dev->cast_ms();
+	return RAD_OK;
+}
+
+rad_status rad_batch_set_matrix(rad_batch *b, int scene, const float *F)
+{
+	if (!b || !F) return RAD_EINVAL;
+	return b->b->set_matrix(scene, F, b->err) ? RAD_EINVAL : RAD_OK;
+}
+
+rad_status rad_batch_solve(rad_batch *b, int prec, int use_graph, const double *rho, const double *E, double tol,
+                           int outer, int sweeps, double *B, double *rel_res, double *info4)
+{
+	if (!b || !rho || !E || !B || !rel_res) return RAD_EINVAL;
+	if ((prec != RAD_PREC_F16 && prec != RAD_PREC_BF16) || outer < 0 || outer > 64 || sweeps <= 0 || sweeps > 64) {
+		b->err = "batch solve: precision FP16/BF16, outer 0..64, sweeps 1..64";
+		return RAD_EINVAL;
+	}
+	return b->b->solve(prec, use_graph != 0, rho, E, tol, outer, sweeps, B, rel_res, info4, b->err) ? RAD_EDEVICE
+	                                                                                              : RAD_OK;
 }
 
 }  // extern "C"

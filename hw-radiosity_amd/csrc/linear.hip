@@ -35,124 +35,16 @@
 namespace rgpu {
 namespace {
 
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef __bf16 b8 __attribute__((ext_vector_type(8)));
-typedef float f4 __attribute__((ext_vector_type(4)));
+#include "lin_common.h"
 
-constexpr int kB = 256;
-
-__device__ __forceinline__ f4 mfma(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
-__device__ __forceinline__ f4 mfma(b8 a, b8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
-
-template <typename E> struct VecOf;
-template <> struct VecOf<_Float16> { typedef h8 type; };
-template <> struct VecOf<__bf16> { typedef b8 type; };
-
-// element (row, k) of a fragment-ordered matrix with ld columns
-__device__ __forceinline__ size_t frag_index(int row, int k, int ld)
-{
-	return ((size_t)(row >> 4) * (ld >> 5) + (k >> 5)) * 512 + ((((k & 31) >> 3) << 4) + (row & 15)) * 8 + (k & 7);
-}
-
-__global__ void k_absmax(const float *F, size_t count, unsigned *out)
-{
-	float m = 0.f;
-	for (size_t i = blockIdx.x * (size_t)kB + threadIdx.x; i < count; i += (size_t)gridDim.x * kB) m = fmaxf(m, fabsf(F[i]));
-	for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-	if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));  // non-negative floats order as their bits
-}
-
-// one thread per 8-element fragment row: reads 8 consecutive floats of F32
-template <typename E> __global__ void k_to_frag(const float *F, int ld, float s, E *out)
-{
-	typedef typename VecOf<E>::type V;
-	const size_t units = (size_t)ld * ld / 8;
-	for (size_t u = blockIdx.x * (size_t)kB + threadIdx.x; u < units; u += (size_t)gridDim.x * kB) {
-		const int row = (int)(u / (ld / 8)), k = (int)(u % (ld / 8)) * 8;
-		const float4 a = *(const float4 *)(F + (size_t)row * ld + k);
-		const float4 b = *(const float4 *)(F + (size_t)row * ld + k + 4);
-		V v;
-		v[0] = (E)(a.x * s); v[1] = (E)(a.y * s); v[2] = (E)(a.z * s); v[3] = (E)(a.w * s);
-		v[4] = (E)(b.x * s); v[5] = (E)(b.y * s); v[6] = (E)(b.z * s); v[7] = (E)(b.w * s);
-		*(V *)(out + frag_index(row, k, ld)) = v;
-	}
-}
-
-template <typename T> __global__ void k_fill(T *p, size_t count, T v)
-{
-	for (size_t i = blockIdx.x * (size_t)kB + threadIdx.x; i < count; i += (size_t)gridDim.x * kB) p[i] = v;
-}
-
-// part[s][row][col] = sum over the s-th K chunk of A[row][k] X[k][col].
-// Software pipelined: two register sets of U k-steps each, the loads of one
-// in flight while the other feeds the MFMAs (the chunk holds a multiple of 2U
-// k-steps).
-template <typename E, int RT, int CT, int U> struct GemmRegs {
-	typedef typename VecOf<E>::type V;
-	V a[U][RT], b[U][CT];
-};
-
-template <typename E, int RT, int CT, int U>
-__device__ __forceinline__ void gemm_load(GemmRegs<E, RT, CT, U> &g, const typename VecOf<E>::type *Ap,
-                                          const typename VecOf<E>::type *Bp, int rb0, int nkb, int kb)
-{
-#pragma unroll
-	for (int u = 0; u < U; u++) {
-#pragma unroll
-		for (int r = 0; r < RT; r++) g.a[u][r] = __builtin_nontemporal_load(Ap + ((size_t)(rb0 + r) * nkb + kb + u) * 64);
-#pragma unroll
-		for (int c = 0; c < CT; c++) g.b[u][c] = Bp[((size_t)c * nkb + kb + u) * 64];
-	}
-}
-
-template <typename E, int RT, int CT, int U>
-__device__ __forceinline__ void gemm_mma(const GemmRegs<E, RT, CT, U> &g, f4 (&acc)[RT][CT])
-{
-#pragma unroll
-	for (int u = 0; u < U; u++)
-#pragma unroll
-		for (int r = 0; r < RT; r++)
-#pragma unroll
-			for (int c = 0; c < CT; c++) acc[r][c] = mfma(g.a[u][r], g.b[u][c], acc[r][c]);
-}
-
-template <int CT> struct GemmUnroll { static constexpr int U = CT <= 4 ? 4 : 2; };
-
+// part[s][row][col] = sum over the s-th K chunk of A[row][k] X[k][col]
 template <typename E, int RT, int CT>
 __global__ __launch_bounds__(kB) void k_gemm(const E *__restrict__ A, const E *__restrict__ XT, float *__restrict__ part,
                                              int ld, int rp, int kchunk)
 {
-	typedef typename VecOf<E>::type V;
-	constexpr int U = GemmUnroll<CT>::U;
-	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-	const int rb0 = (blockIdx.x * 4 + w) * RT;
-	const int s = blockIdx.y, kb0 = s * (kchunk >> 5), kb1 = kb0 + (kchunk >> 5);
-	const int nkb = ld >> 5;
-	const V *Ap = (const V *)A + lane;
-	const V *Bp = (const V *)XT + lane;
-	f4 acc[RT][CT];
-#pragma unroll
-	for (int r = 0; r < RT; r++)
-#pragma unroll
-		for (int c = 0; c < CT; c++) acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
-	GemmRegs<E, RT, CT, U> g0, g1;
-	gemm_load(g0, Ap, Bp, rb0, nkb, kb0);
-	for (int kb = kb0; kb < kb1; kb += 2 * U) {
-		gemm_load(g1, Ap, Bp, rb0, nkb, kb + U);
-		gemm_mma(g0, acc);
-		if (kb + 2 * U < kb1) gemm_load(g0, Ap, Bp, rb0, nkb, kb + 2 * U);
-		gemm_mma(g1, acc);
-	}
-	const int lr = lane & 15;
-#pragma unroll
-	for (int r = 0; r < RT; r++)
-#pragma unroll
-		for (int c = 0; c < CT; c++)
-#pragma unroll
-			for (int j = 0; j < 4; j++) {
-				const int row = (rb0 + r) * 16 + (lane >> 4) * 4 + j;
-				part[((size_t)s * ld + row) * rp + c * 16 + lr] = acc[r][c][j];
-			}
+	const int w = threadIdx.x >> 6;
+	const int s = blockIdx.y, kb0 = s * (kchunk >> 5);
+	gemm_tile<E, RT, CT>(A, XT, part + (size_t)s * ld * rp, ld, rp, (blockIdx.x * 4 + w) * RT, kb0, kb0 + (kchunk >> 5));
 }
 
 // d = rhs + rho o (inv_s * sum_s part); XT <- d (fragment order).
@@ -172,63 +64,50 @@ __global__ void k_jacobi(const float *part, int S, int ld, int rp, int n, int nr
 	}
 }
 
-// FP64 product with the FP32 matrix, RB rows x CC columns per block:
-//   out[i][c] = base[i][c] + rho[i][c] * sum_k F[i][k] xT[c][k] - beta * xT[c][i]
-// (base, rho may be null: 0 and 1).  Per block: sum of squares and max |out|.
-template <int RB, int CC>
-__global__ __launch_bounds__(kB) void k_fx64(const float *__restrict__ F, int ld, int n, int nrhs,
-                                             const double *__restrict__ xT, const double *base, const double *rho,
-                                             double beta, double *out, double *blk)
+// Split-K partials part[s][row][col] (FP64) of F X, summed in a fixed order
+// by k_resid.
+template <int RT, int CT>
+__global__ __launch_bounds__(kB) void k_fx64m(const float *__restrict__ F, const double *__restrict__ xT,
+                                              double *__restrict__ part, int ld, int rp, int kchunk)
 {
-	__shared__ double red[4][RB * CC];
-	const int i0 = blockIdx.x * RB, c0 = blockIdx.y * CC;
-	const int nc = min(CC, nrhs - c0);
-	double acc[RB][CC];
-#pragma unroll
-	for (int r = 0; r < RB; r++)
-#pragma unroll
-		for (int c = 0; c < CC; c++) acc[r][c] = 0.0;
-	for (int k = threadIdx.x; k < n; k += kB) {
-		double xv[CC];
-#pragma unroll
-		for (int c = 0; c < CC; c++) xv[c] = c < nc ? xT[(size_t)(c0 + c) * ld + k] : 0.0;
-#pragma unroll
-		for (int r = 0; r < RB; r++) {
-			const double f = (double)F[(size_t)(i0 + r) * ld + k];
-#pragma unroll
-			for (int c = 0; c < CC; c++) acc[r][c] += f * xv[c];
-		}
+	const int w = threadIdx.x >> 6;
+	const int s = blockIdx.z;
+	fx64_tile<RT, CT>(F, xT, part + (size_t)s * ld * rp, ld, rp, (blockIdx.x * 4 + w) * RT, blockIdx.y * CT,
+	                  s * kchunk, (s + 1) * kchunk);
+}
+
+// out[i][c] = base[i][c] + rho[i][c] * y - beta * xT[c][i], y = the split-K
+// sum of part (FP64, fixed order); base, rho may be null (0 and 1).  Per
+// block: sum of squares and max |out|.
+__global__ __launch_bounds__(kB) void k_resid(const double *part, int S, int ld, int rp, int n, int nrhs,
+                                              const double *xT, const double *base, const double *rho, double beta,
+                                              double *out, double *blk)
+{
+	const size_t total = (size_t)n * nrhs;
+	const size_t t = blockIdx.x * (size_t)kB + threadIdx.x;
+	double ss = 0.0, mx = 0.0;
+	if (t < total) {
+		const int i = (int)(t / nrhs), c = (int)(t % nrhs);
+		double y = 0.0;
+		for (int q = 0; q < S; q++) y += part[((size_t)q * ld + i) * rp + c];
+		const double v = (base ? base[t] : 0.0) + (rho ? rho[t] : 1.0) * y - beta * xT[(size_t)c * ld + i];
+		out[t] = v;
+		ss = v * v;
+		mx = fabs(v);
 	}
-#pragma unroll
-	for (int r = 0; r < RB; r++)
-#pragma unroll
-		for (int c = 0; c < CC; c++) {
-			double v = acc[r][c];
-			for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-			if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][r * CC + c] = v;
-		}
+	for (int off = 32; off > 0; off >>= 1) {
+		ss += __shfl_xor(ss, off);
+		mx = fmax(mx, __shfl_xor(mx, off));
+	}
+	__shared__ double r[8];
+	if ((threadIdx.x & 63) == 0) {
+		r[2 * (threadIdx.x >> 6)] = ss;
+		r[2 * (threadIdx.x >> 6) + 1] = mx;
+	}
 	__syncthreads();
-	if (threadIdx.x < 64) {
-		double ss = 0.0, mx = 0.0;
-		const int r = threadIdx.x / CC, c = threadIdx.x % CC;
-		if (threadIdx.x < RB * CC && i0 + r < n && c < nc) {
-			const int q = r * CC + c;
-			const double fx = red[0][q] + red[1][q] + red[2][q] + red[3][q];
-			const size_t t = (size_t)(i0 + r) * nrhs + c0 + c;
-			const double v = (base ? base[t] : 0.0) + (rho ? rho[t] : 1.0) * fx - beta * xT[(size_t)(c0 + c) * ld + i0 + r];
-			out[t] = v;
-			ss = v * v;
-			mx = fabs(v);
-		}
-		for (int off = 32; off > 0; off >>= 1) {
-			ss += __shfl_xor(ss, off);
-			mx = fmax(mx, __shfl_xor(mx, off));
-		}
-		if (threadIdx.x == 0) {
-			const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-			blk[2 * b] = ss;
-			blk[2 * b + 1] = mx;
-		}
+	if (threadIdx.x == 0) {
+		blk[2 * blockIdx.x] = r[0] + r[2] + r[4] + r[6];
+		blk[2 * blockIdx.x + 1] = fmax(fmax(r[1], r[3]), fmax(r[5], r[7]));
 	}
 }
 
@@ -334,18 +213,19 @@ __global__ void k_err(const double *x, int transposed, int ld, const double *ref
 }
 
 // ---- plain progressive shooting (FP32) --------------------------------------
-// One launch per shot.  The launch reads its source from the state (chosen
-// by the previous launch), adds rho_i F_ij dB_j to B_i and dB_i for every
-// patch i, and folds the next source's key (largest unshot power
-// area_i * sum_c dB_i[c], ties to the lowest index) into next_key; the last
-// block to finish (ticket counter) takes the next source: it copies and zeroes
-// its dB and resets the key for the launch after.
+// One launch per shot, no atomics and no device-wide fences.  Launch k reads
+// the per-block records of launch k-1 (each: the block's largest unshot-power
+// key, area_i * sum_c dB_i[c] with ties to the lowest index, and that patch's
+// dB row), reduces them to the source j and its dB_j, adds
+// rho_i F_ij dB_j to B_i and dB_i for every patch i (the source's own dB
+// restarting from 0), and writes its own records for launch k+1.  Records are
+// double-buffered by shot parity.  A zero best key means nothing is left to
+// shoot: the launch only passes the zero on.
 constexpr int kMaxShootRhs = 16;
-struct ShootState {
-	unsigned long long next_key;
-	unsigned ticket;
-	int done, shots, src;
+struct ShotRec {
+	unsigned long long key;
 	float dB[kMaxShootRhs];
+	float pad[2];
 };
 
 __device__ __forceinline__ unsigned long long power_key(float p, int j)
@@ -354,91 +234,95 @@ __device__ __forceinline__ unsigned long long power_key(float p, int j)
 	return ((unsigned long long)__float_as_uint(p) << 32) | (0xffffffffu - (uint32_t)j);
 }
 
-// block max of key, folded into *target by thread 0
-__device__ __forceinline__ void block_max_key(unsigned long long key, unsigned long long *target)
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long key)
 {
-	__shared__ unsigned long long wk[4];
 	for (int off = 32; off > 0; off >>= 1) {
 		const unsigned long long o = __shfl_xor(key, off);
 		key = o > key ? o : key;
 	}
-	if ((threadIdx.x & 63) == 0) wk[threadIdx.x >> 6] = key;
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		for (int w = 1; w < 4; w++) key = wk[w] > key ? wk[w] : key;
-		if (key) atomicMax(target, key);
-	}
+	return key;
 }
 
-__device__ void take_next_source(float *dB, int nrhs, ShootState *st)
+// block maximum of key; the thread holding it writes its dB row to rec
+__device__ __forceinline__ void block_record(unsigned long long key, const float *row, int nrhs, ShotRec *rec)
 {
-	const unsigned long long key = atomicAdd(&st->next_key, 0ull);
-	st->next_key = 0;
-	st->ticket = 0;
-	if (!key) {
-		st->done = 1;
-		return;
-	}
-	const int j = (int)(0xffffffffu - (uint32_t)key);
-	for (int c = 0; c < nrhs; c++) {
-		st->dB[c] = dB[(size_t)j * nrhs + c];
-		dB[(size_t)j * nrhs + c] = 0.f;
-	}
-	st->src = j;
+	__shared__ unsigned long long wk[4];
+	const unsigned long long wm = wave_max(key);
+	if ((threadIdx.x & 63) == 0) wk[threadIdx.x >> 6] = wm;
+	__syncthreads();
+	unsigned long long m = wk[0];
+	for (int w = 1; w < 4; w++) m = wk[w] > m ? wk[w] : m;
+	if (threadIdx.x == 0) rec->key = m;
+	if (m != 0 && key == m)  // keys are unique (the index is in them)
+		for (int c = 0; c < nrhs; c++) rec->dB[c] = row[c];
 }
 
-__device__ __forceinline__ void shoot_epilogue(float *dB, int nrhs, ShootState *st)
-{
-	__shared__ bool last;
-	__threadfence();
-	__syncthreads();
-	if (threadIdx.x == 0) last = atomicAdd(&st->ticket, 1u) == gridDim.x - 1;
-	__syncthreads();
-	if (last && threadIdx.x == 0) {
-		__threadfence();
-		take_next_source(dB, nrhs, st);
-	}
-}
-
-// first selection: keys of the initial dB
-__global__ __launch_bounds__(kB) void k_lsel(float *dB, const float *area, int n, int nrhs, ShootState *st)
+// records of the initial dB (launch 0's input)
+__global__ __launch_bounds__(kB) void k_lsel(const float *dB, const float *area, int n, int nrhs, ShotRec *rout)
 {
 	const int i = blockIdx.x * kB + threadIdx.x;
 	unsigned long long key = 0;
+	float row[kMaxShootRhs];
 	if (i < n) {
 		float p = 0.f;
-		for (int c = 0; c < nrhs; c++) p += dB[(size_t)i * nrhs + c];
+		for (int c = 0; c < nrhs; c++) {
+			row[c] = dB[(size_t)i * nrhs + c];
+			p += row[c];
+		}
 		p *= area[i];
 		if (p > 0.f) key = power_key(p, i);
 	}
-	block_max_key(key, &st->next_key);
-	shoot_epilogue(dB, nrhs, st);
+	block_record(key, row, nrhs, rout + blockIdx.x);
 }
 
 __global__ __launch_bounds__(kB) void k_lshoot(const float *__restrict__ Ft, int ld, const float *rho, const float *area,
-                                               float *B, float *dB, int n, int nrhs, ShootState *st)
+                                               float *B, float *dB, int n, int nrhs, const ShotRec *rin,
+                                               ShotRec *rout, int *shots, int shot)
 {
-	if (st->done) return;  // uniform: written only by the previous launch
+	__shared__ unsigned long long wk[4];
+	__shared__ int sb;
+	__shared__ float src[kMaxShootRhs];
+	// the previous launch's records -> source
+	unsigned long long k = 0;
+	for (int b = threadIdx.x; b < (int)gridDim.x; b += kB) {
+		const unsigned long long o = rin[b].key;
+		k = o > k ? o : k;
+	}
+	const unsigned long long wm = wave_max(k);
+	if ((threadIdx.x & 63) == 0) wk[threadIdx.x >> 6] = wm;
+	__syncthreads();
+	unsigned long long best = wk[0];
+	for (int w = 1; w < 4; w++) best = wk[w] > best ? wk[w] : best;
+	if (best == 0) {  // nothing left to shoot (uniform over the grid)
+		if (threadIdx.x == 0) rout[blockIdx.x].key = 0;
+		return;
+	}
+	for (int b = threadIdx.x; b < (int)gridDim.x; b += kB)
+		if (rin[b].key == best) sb = b;
+	__syncthreads();
+	if (threadIdx.x < nrhs) src[threadIdx.x] = rin[sb].dB[threadIdx.x];
+	__syncthreads();
+	const int j = (int)(0xffffffffu - (uint32_t)best);
 	const int i = blockIdx.x * kB + threadIdx.x;
-	const int j = st->src;
 	unsigned long long key = 0;
+	float row[kMaxShootRhs];
 	if (i < n) {
 		const float f = Ft[(size_t)j * ld + i];
 		float p = 0.f;
 		for (int c = 0; c < nrhs; c++) {
 			const size_t t = (size_t)i * nrhs + c;
-			const float v = rho[t] * f * st->dB[c];
+			const float v = rho[t] * f * src[c];
 			B[t] += v;
-			const float nd = dB[t] + v;
+			const float nd = (i == j ? 0.f : dB[t]) + v;
 			dB[t] = nd;
+			row[c] = nd;
 			p += nd;
 		}
 		p *= area[i];
 		if (p > 0.f) key = power_key(p, i);
 	}
-	if (threadIdx.x == 0 && blockIdx.x == 0) st->shots++;
-	block_max_key(key, &st->next_key);
-	shoot_epilogue(dB, nrhs, st);
+	block_record(key, row, nrhs, rout + blockIdx.x);
+	if (blockIdx.x == 0 && threadIdx.x == 0) *shots = shot + 1;
 }
 
 __global__ void k_transpose(const float *F, float *Ft, int ld)
@@ -450,40 +334,6 @@ __global__ void k_transpose(const float *F, float *Ft, int ld)
 	__syncthreads();
 	for (int r = ty; r < 32; r += 8) Ft[(size_t)(bx + r) * ld + by + tx] = tile[tx][r];
 }
-
-#define HIP_OK(expr, what)                                          \
-	do {                                                            \
-		hipError_t e_ = (expr);                                     \
-		if (e_ != hipSuccess) {                                     \
-			err = std::string(what) + ": " + hipGetErrorString(e_); \
-			return -1;                                              \
-		}                                                           \
-	} while (0)
-
-template <typename T> struct Buf {
-	T *p = nullptr;
-	size_t n = 0;
-	Buf() = default;
-	Buf(const Buf &) = delete;
-	Buf &operator=(const Buf &) = delete;
-	~Buf()
-	{
-		if (p) (void)hipFree(p);
-	}
-	int alloc(size_t count, std::string &err)
-	{
-		if (count <= n && p) return 0;
-		if (p) (void)hipFree(p);
-		p = nullptr;
-		n = 0;
-		if (count == 0) return 0;
-		HIP_OK(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
-		n = count;
-		return 0;
-	}
-};
-
-int grid_for(size_t total) { return (int)std::min<size_t>((total + kB - 1) / kB, 8192); }
 
 const int kCT[] = {1, 2, 4, 8, 12};
 int pick_ct(int nrhs)
@@ -544,12 +394,56 @@ double power_sum(const std::vector<float> &a, const double *v, int n, int nrhs)
 	return p;
 }
 
-double ms_since(std::chrono::steady_clock::time_point a)
-{
-	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
-}
-
 }  // namespace
+
+// FP64 residual machinery: the caller's xT holds rp x ld doubles with rows
+// >= nrhs and columns >= n zero; split-K partials; per-block stats.
+struct Fx64 {
+	int n = 0, ld = 0, nrhs = 0, rp = 0, rt = 4, ct = 1, S = 1;
+	Buf<double> part, blk;
+	size_t nblk = 0;
+	std::vector<double> hb;
+
+	int setup(int n_, int ld_, int nrhs_, std::string &err)
+	{
+		n = n_;
+		ld = ld_;
+		nrhs = nrhs_;
+		const int nct = (nrhs + 15) / 16;
+		ct = nct >= 4 ? 4 : (nct >= 2 ? 2 : 1);
+		rt = ct == 4 ? 2 : 4;
+		rp = (nct + ct - 1) / ct * ct * 16;  // whole column groups: every tile the grid covers exists
+		const int tiles = (ld / (64 * rt)) * (rp / (16 * ct));
+		S = 1;
+		while (S < 8 && tiles * S < 512 && (ld / (2 * S)) % 32 == 0) S *= 2;
+		nblk = ((size_t)n * nrhs + kB - 1) / kB;
+		if (part.alloc((size_t)S * ld * rp, err) || blk.alloc(std::max<size_t>(nblk, 8192) * 2, err)) return -1;
+		hb.assign(nblk * 2, 0.0);
+		return 0;
+	}
+	// out = base + rho o (F x) - beta x; returns (sum of squares, max |out|)
+	int run(const float *F, const double *xT, const double *base, const double *rho, double beta, double *out,
+	        hipStream_t st, double &ss, double &mx, bool want_stats, std::string &err)
+	{
+		const dim3 grid(ld / (64 * rt), rp / (16 * ct), S);
+		const int kchunk = ld / S;
+		if (ct == 4) hipLaunchKernelGGL((k_fx64m<2, 4>), grid, dim3(kB), 0, st, F, xT, part.p, ld, rp, kchunk);
+		else if (ct == 2) hipLaunchKernelGGL((k_fx64m<4, 2>), grid, dim3(kB), 0, st, F, xT, part.p, ld, rp, kchunk);
+		else hipLaunchKernelGGL((k_fx64m<4, 1>), grid, dim3(kB), 0, st, F, xT, part.p, ld, rp, kchunk);
+		hipLaunchKernelGGL(k_resid, dim3((unsigned)nblk), dim3(kB), 0, st, part.p, S, ld, rp, n, nrhs, xT, base, rho,
+		                   beta, out, blk.p);
+		HIP_OK(hipGetLastError(), "FP64 product");
+		ss = mx = 0.0;
+		if (!want_stats) return 0;
+		HIP_OK(hipMemcpyAsync(hb.data(), blk.p, nblk * 2 * 8, hipMemcpyDeviceToHost, st), "reduce");
+		HIP_OK(hipStreamSynchronize(st), "residual");
+		for (size_t b = 0; b < nblk; b++) {
+			ss += hb[2 * b];
+			mx = std::max(mx, hb[2 * b + 1]);
+		}
+		return 0;
+	}
+};
 
 struct LinearSystem::Impl {
 	hipStream_t st = nullptr;
@@ -827,16 +721,16 @@ int LinearSystem::multiply(int prec, int nrhs, const double *X, double *Y, std::
 	const size_t total = (size_t)n * nrhs;
 	hipStream_t st = d.st;
 	if (prec == PREC_F32) {
-		std::vector<double> xt((size_t)nrhs * ld, 0.0);
+		Fx64 fx;
+		if (fx.setup(n, ld, nrhs, err)) return -1;
+		std::vector<double> xt((size_t)fx.rp * ld, 0.0);
 		for (int i = 0; i < n; i++)
 			for (int c = 0; c < nrhs; c++) xt[(size_t)c * ld + i] = X[(size_t)i * nrhs + c];
-		Buf<double> xT, out, blk;
-		const dim3 grid((n + 7) / 8, (nrhs + 3) / 4);
-		if (xT.alloc(xt.size(), err) || out.alloc(total, err) || blk.alloc((size_t)grid.x * grid.y * 2, err)) return -1;
+		Buf<double> xT, out;
+		if (xT.alloc(xt.size(), err) || out.alloc(total, err)) return -1;
 		HIP_OK(hipMemcpyAsync(xT.p, xt.data(), xt.size() * 8, hipMemcpyHostToDevice, st), "upload");
-		hipLaunchKernelGGL((k_fx64<8, 4>), grid, dim3(kB), 0, st, d.F32.p, ld, n, nrhs, xT.p, (const double *)nullptr,
-		                   (const double *)nullptr, 0.0, out.p, blk.p);
-		HIP_OK(hipGetLastError(), "multiply");
+		double ss, mx;
+		if (fx.run(d.F32.p, xT.p, nullptr, nullptr, 0.0, out.p, st, ss, mx, false, err)) return -1;
 		HIP_OK(hipMemcpyAsync(Y, out.p, total * 8, hipMemcpyDeviceToHost, st), "download");
 		HIP_OK(hipStreamSynchronize(st), "multiply");
 		return 0;
@@ -918,14 +812,13 @@ int LinearSystem::gather(int prec, int nrhs, const double *rho, const double *E,
 	Buf<double> rho64, E64, xT, r64, blk, Bref64;
 	Buf<_Float16> XT16;
 	Buf<__bf16> XTb;
+	Fx64 fx;
+	if (fx.setup(n, ld, nrhs, err)) return -1;
+	const size_t xtn = (size_t)fx.rp * ld;
 	if (rho32.alloc(total, err) || r32.alloc(total, err) || d32.alloc(total, err) ||
 	    part.alloc((size_t)S * ld * rp, err) || rho64.alloc(total, err) || E64.alloc(total, err) ||
-	    xT.alloc((size_t)nrhs * ld, err) || r64.alloc(total, err))
+	    xT.alloc(xtn, err) || r64.alloc(total, err) || blk.alloc(8192 * 2, err))
 		return -1;
-	constexpr int RB = 8, CC = 4;
-	const dim3 fgrid((n + RB - 1) / RB, (nrhs + CC - 1) / CC);
-	const size_t nblk = (size_t)fgrid.x * fgrid.y;
-	if (blk.alloc(std::max<size_t>(nblk, 8192) * 2, err)) return -1;
 	const size_t xts = (size_t)rp * ld;
 	if (prec == PREC_F16) {
 		if (XT16.alloc(xts, err)) return -1;
@@ -939,13 +832,13 @@ int LinearSystem::gather(int prec, int nrhs, const double *rho, const double *E,
 	HIP_OK(hipMemcpyAsync(rho32.p, h32.data(), total * 4, hipMemcpyHostToDevice, st), "upload");
 	HIP_OK(hipMemcpyAsync(rho64.p, rho, total * 8, hipMemcpyHostToDevice, st), "upload");
 	HIP_OK(hipMemcpyAsync(E64.p, E, total * 8, hipMemcpyHostToDevice, st), "upload");
-	hipLaunchKernelGGL(k_fill<double>, dim3(grid_for((size_t)nrhs * ld)), dim3(kB), 0, st, xT.p, (size_t)nrhs * ld, 0.0);
+	hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(xtn)), dim3(kB), 0, st, xT.p, xtn, 0.0);
 	if (Bref) {
 		if (Bref64.alloc(total, err)) return -1;
 		HIP_OK(hipMemcpyAsync(Bref64.p, Bref, total * 8, hipMemcpyHostToDevice, st), "upload");
 	}
 	HIP_OK(hipStreamSynchronize(st), "upload");
-	std::vector<double> hb(std::max<size_t>(nblk, 8192) * 2);
+	std::vector<double> hb(8192 * 2);
 	const int g = grid_for(total);
 	auto err_vs_ref = [&](double &e) -> int {
 		hipLaunchKernelGGL(k_err, dim3(g), dim3(kB), 0, st, xT.p, 1, ld, Bref64.p, n, nrhs, blk.p);
@@ -962,16 +855,20 @@ int LinearSystem::gather(int prec, int nrhs, const double *rho, const double *E,
 	const auto t0 = std::chrono::steady_clock::now();
 	double enorm = -1.0, rel = 0.0, ms_target = -1.0, errv = -1.0, check_ms = 0.0;
 	int outer = 0, total_sweeps = 0;
+	// x = 0 at the start, so the first residual is E itself
+	double ss0 = 0.0, mx0 = 0.0;
+	for (size_t t = 0; t < total; t++) {
+		ss0 += E[t] * E[t];
+		mx0 = std::max(mx0, std::fabs(E[t]));
+	}
 	for (;; outer++) {
-		hipLaunchKernelGGL((k_fx64<RB, CC>), fgrid, dim3(kB), 0, st, d.F32.p, ld, n, nrhs, xT.p, E64.p, rho64.p, 1.0,
-		                   r64.p, blk.p);
-		HIP_OK(hipGetLastError(), "residual launch");
-		HIP_OK(hipMemcpyAsync(hb.data(), blk.p, nblk * 2 * 8, hipMemcpyDeviceToHost, st), "reduce");
-		HIP_OK(hipStreamSynchronize(st), "residual");
 		double ss = 0.0, mx = 0.0;
-		for (size_t b = 0; b < nblk; b++) {
-			ss += hb[2 * b];
-			mx = std::max(mx, hb[2 * b + 1]);
+		if (outer == 0) {
+			HIP_OK(hipMemcpyAsync(r64.p, E64.p, total * 8, hipMemcpyDeviceToDevice, st), "residual");
+			ss = ss0;
+			mx = mx0;
+		} else if (fx.run(d.F32.p, xT.p, E64.p, rho64.p, 1.0, r64.p, st, ss, mx, true, err)) {
+			return -1;
 		}
 		if (!std::isfinite(ss)) {
 			err = "gather diverged (non-finite residual): the iteration needs spectral radius of diag(rho) F below 1";
@@ -1013,7 +910,7 @@ int LinearSystem::gather(int prec, int nrhs, const double *rho, const double *E,
 	}
 	const double ms = ms_since(t0) - check_ms;
 	std::vector<double> hx((size_t)nrhs * ld);
-	HIP_OK(hipMemcpy(hx.data(), xT.p, hx.size() * 8, hipMemcpyDeviceToHost), "download");
+	HIP_OK(hipMemcpy(hx.data(), xT.p, hx.size() * 8, hipMemcpyDeviceToHost), "download");  // rows < nrhs
 	for (int i = 0; i < n; i++)
 		for (int c = 0; c < nrhs; c++) B[(size_t)i * nrhs + c] = hx[(size_t)c * ld + i];
 	if (info) {
@@ -1045,6 +942,7 @@ int LinearSystem::shoot(int nrhs, const double *rho, const double *E, const doub
 	}
 	const int n = n_, ld = np_;
 	hipStream_t st = d.st;
+	static_assert(sizeof(ShotRec) % 8 == 0, "record alignment");
 	if (!d.havet) {
 		if (d.Ft.alloc((size_t)ld * ld, err)) return -1;
 		hipLaunchKernelGGL(k_transpose, dim3(ld / 32, ld / 32), dim3(kB), 0, st, d.F32.p, d.Ft.p, ld);
@@ -1052,9 +950,13 @@ int LinearSystem::shoot(int nrhs, const double *rho, const double *E, const doub
 		d.havet = true;
 	}
 	const size_t total = (size_t)n * nrhs;
+	const int nb = (n + kB - 1) / kB;
 	Buf<float> rho32, B32, dB32;
-	Buf<ShootState> state;
-	if (rho32.alloc(total, err) || B32.alloc(total, err) || dB32.alloc(total, err) || state.alloc(1, err)) return -1;
+	Buf<ShotRec> rec;
+	Buf<int> shots_d;
+	if (rho32.alloc(total, err) || B32.alloc(total, err) || dB32.alloc(total, err) || rec.alloc(2 * (size_t)nb, err) ||
+	    shots_d.alloc(1, err))
+		return -1;
 	std::vector<float> h(total);
 	for (size_t t = 0; t < total; t++) h[t] = (float)rho[t];
 	HIP_OK(hipMemcpyAsync(rho32.p, h.data(), total * 4, hipMemcpyHostToDevice, st), "upload");
@@ -1062,10 +964,7 @@ int LinearSystem::shoot(int nrhs, const double *rho, const double *E, const doub
 	for (size_t t = 0; t < total; t++) e32[t] = (float)E[t];
 	HIP_OK(hipMemcpyAsync(B32.p, e32.data(), total * 4, hipMemcpyHostToDevice, st), "upload");
 	HIP_OK(hipMemcpyAsync(dB32.p, e32.data(), total * 4, hipMemcpyHostToDevice, st), "upload");
-	ShootState zero;
-	std::memset(&zero, 0, sizeof zero);
-	HIP_OK(hipMemcpyAsync(state.p, &zero, sizeof zero, hipMemcpyHostToDevice, st), "upload");
-	const int nb = (n + kB - 1) / kB;
+	hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(kB), 0, st, shots_d.p, (size_t)1, 0);
 	HIP_OK(hipStreamSynchronize(st), "upload");
 	std::vector<float> hb(total);
 	auto err_vs_ref = [&](double &e) -> int {
@@ -1081,20 +980,18 @@ int LinearSystem::shoot(int nrhs, const double *rho, const double *E, const doub
 		return 0;
 	};
 	const auto t0 = std::chrono::steady_clock::now();
-	hipLaunchKernelGGL(k_lsel, dim3(nb), dim3(kB), 0, st, dB32.p, d.area.p, n, nrhs, state.p);
+	hipLaunchKernelGGL(k_lsel, dim3(nb), dim3(kB), 0, st, dB32.p, d.area.p, n, nrhs, rec.p);
 	double check_ms = 0.0, ms_target = -1.0, errv = -1.0;
-	int launched = 0, chunk = 64;
-	ShootState hs;
-	std::memset(&hs, 0, sizeof hs);
+	int launched = 0, chunk = 64, done_shots = 0;
 	for (;;) {
 		const int b = std::min(chunk, max_shots - launched);
 		if (b <= 0) break;
-		for (int k = 0; k < b; k++)
-			hipLaunchKernelGGL(k_lshoot, dim3(nb), dim3(kB), 0, st, d.Ft.p, ld, rho32.p, d.area.p, B32.p, dB32.p, n, nrhs,
-			                   state.p);
-		launched += b;
+		for (int k = 0; k < b; k++, launched++)
+			hipLaunchKernelGGL(k_lshoot, dim3(nb), dim3(kB), 0, st, d.Ft.p, ld, rho32.p, d.area.p, B32.p, dB32.p, n,
+			                   nrhs, rec.p + (size_t)(launched & 1) * nb, rec.p + (size_t)((launched + 1) & 1) * nb,
+			                   shots_d.p, launched);
 		HIP_OK(hipGetLastError(), "shoot launch");
-		HIP_OK(hipMemcpyAsync(&hs, state.p, sizeof hs, hipMemcpyDeviceToHost, st), "state");
+		HIP_OK(hipMemcpyAsync(&done_shots, shots_d.p, sizeof(int), hipMemcpyDeviceToHost, st), "state");
 		HIP_OK(hipStreamSynchronize(st), "shoot");
 		if (Bref) {
 			const auto c0 = std::chrono::steady_clock::now();
@@ -1105,7 +1002,7 @@ int LinearSystem::shoot(int nrhs, const double *rho, const double *E, const doub
 				break;
 			}
 		}
-		if (hs.done) break;
+		if (done_shots < launched) break;  // a launch found nothing left to shoot
 		chunk = std::min(chunk * 2, 4096);
 	}
 	const double ms = ms_since(t0) - check_ms;
@@ -1115,7 +1012,7 @@ int LinearSystem::shoot(int nrhs, const double *rho, const double *E, const doub
 	if (info) {
 		std::vector<float> a(n);
 		HIP_OK(hipMemcpy(a.data(), d.area.p, n * 4, hipMemcpyDeviceToHost), "area");
-		info[INFO_OUTER] = hs.shots;
+		info[INFO_OUTER] = done_shots;
 		info[INFO_SWEEPS] = 0;
 		info[INFO_REL_RES] = -1;
 		info[INFO_MS] = ms;
@@ -1145,13 +1042,15 @@ int LinearSystem::adjoint(int nrhs, const double *rho, const double *B, const do
 	const int S = std::max(1, std::min(64, n / 256));
 	const int rows_per_split = (n + S - 1) / S;
 	const int fin_blocks = (int)((total + kB - 1) / kB);
-	Buf<double> rho64, g64, x64, r64, part64, blk, B64T, fb, fblk;
+	Buf<double> rho64, g64, x64, r64, part64, blk, B64T, fb;
+	Fx64 fx;
+	if (fx.setup(n, ld, nrhs, err)) return -1;
 	Buf<float> rho32, r32, l32, ln32, part32;
 	if (rho64.alloc(total, err) || g64.alloc(total, err) || x64.alloc(total, err) || r64.alloc(total, err) ||
 	    part64.alloc((size_t)S * total, err) || rho32.alloc(total, err) || r32.alloc(total, err) ||
 	    l32.alloc(total, err) || ln32.alloc(total, err) || part32.alloc((size_t)S * total, err) ||
-	    blk.alloc((size_t)std::max(fin_blocks, 8192) * 2, err) || B64T.alloc((size_t)nrhs * ld, err) ||
-	    fb.alloc(total, err) || fblk.alloc((size_t)((n + 7) / 8) * 2, err))
+	    blk.alloc((size_t)std::max(fin_blocks, 8192) * 2, err) || B64T.alloc((size_t)fx.rp * ld, err) ||
+	    fb.alloc(total, err))
 		return -1;
 	std::vector<float> h(total);
 	for (size_t t = 0; t < total; t++) h[t] = (float)rho[t];
@@ -1165,19 +1064,30 @@ int LinearSystem::adjoint(int nrhs, const double *rho, const double *B, const do
 	const auto t0 = std::chrono::steady_clock::now();
 	double enorm = -1.0, rel = 0.0;
 	int outer = 0, total_sweeps = 0;
+	double ss0 = 0.0, mx0 = 0.0;  // lambda = 0 at the start: the first residual is g
+	for (size_t t = 0; t < total; t++) {
+		ss0 += g[t] * g[t];
+		mx0 = std::max(mx0, std::fabs(g[t]));
+	}
 	for (;; outer++) {
-		// r = g + F^T (rho o x) - x in FP64
-		hipLaunchKernelGGL(k_ftx<double>, tgrid, dim3(kB), 0, st, d.F32.p, ld, n, nrhs, x64.p, rho64.p, rows_per_split,
-		                   part64.p);
-		hipLaunchKernelGGL(k_ftx_finish<double>, dim3(fin_blocks), dim3(kB), 0, st, part64.p, S, n, nrhs, g64.p, x64.p,
-		                   1.0, r64.p, blk.p);
-		HIP_OK(hipGetLastError(), "adjoint residual");
-		HIP_OK(hipMemcpyAsync(hb.data(), blk.p, hb.size() * 8, hipMemcpyDeviceToHost, st), "reduce");
-		HIP_OK(hipStreamSynchronize(st), "reduce");
 		double ss = 0.0, mx = 0.0;
-		for (int b = 0; b < fin_blocks; b++) {
-			ss += hb[2 * b];
-			mx = std::max(mx, hb[2 * b + 1]);
+		if (outer == 0) {
+			HIP_OK(hipMemcpyAsync(r64.p, g64.p, total * 8, hipMemcpyDeviceToDevice, st), "residual");
+			ss = ss0;
+			mx = mx0;
+		} else {
+			// r = g + F^T (rho o x) - x in FP64
+			hipLaunchKernelGGL(k_ftx<double>, tgrid, dim3(kB), 0, st, d.F32.p, ld, n, nrhs, x64.p, rho64.p,
+			                   rows_per_split, part64.p);
+			hipLaunchKernelGGL(k_ftx_finish<double>, dim3(fin_blocks), dim3(kB), 0, st, part64.p, S, n, nrhs, g64.p,
+			                   x64.p, 1.0, r64.p, blk.p);
+			HIP_OK(hipGetLastError(), "adjoint residual");
+			HIP_OK(hipMemcpyAsync(hb.data(), blk.p, hb.size() * 8, hipMemcpyDeviceToHost, st), "reduce");
+			HIP_OK(hipStreamSynchronize(st), "reduce");
+			for (int b = 0; b < fin_blocks; b++) {
+				ss += hb[2 * b];
+				mx = std::max(mx, hb[2 * b + 1]);
+			}
 		}
 		if (!std::isfinite(ss)) {
 			err = "adjoint diverged (non-finite residual)";
@@ -1202,13 +1112,14 @@ int LinearSystem::adjoint(int nrhs, const double *rho, const double *B, const do
 		HIP_OK(hipGetLastError(), "adjoint sweep");
 	}
 	// dL/drho = lambda o (F B), F B in FP64
-	std::vector<double> bt((size_t)nrhs * ld, 0.0);
+	std::vector<double> bt((size_t)fx.rp * ld, 0.0);
 	for (int i = 0; i < n; i++)
 		for (int c = 0; c < nrhs; c++) bt[(size_t)c * ld + i] = B[(size_t)i * nrhs + c];
 	HIP_OK(hipMemcpyAsync(B64T.p, bt.data(), bt.size() * 8, hipMemcpyHostToDevice, st), "upload");
-	hipLaunchKernelGGL((k_fx64<8, 4>), dim3((n + 7) / 8, 1), dim3(kB), 0, st, d.F32.p, ld, n, nrhs, B64T.p,
-	                   (const double *)nullptr, (const double *)nullptr, 0.0, fb.p, fblk.p);
-	HIP_OK(hipGetLastError(), "FB");
+	{
+		double ss, mx;
+		if (fx.run(d.F32.p, B64T.p, nullptr, nullptr, 0.0, fb.p, st, ss, mx, false, err)) return -1;
+	}
 	std::vector<double> hfb(total);
 	HIP_OK(hipMemcpyAsync(lambda, x64.p, total * 8, hipMemcpyDeviceToHost, st), "download");
 	HIP_OK(hipMemcpyAsync(hfb.data(), fb.p, total * 8, hipMemcpyDeviceToHost, st), "download");

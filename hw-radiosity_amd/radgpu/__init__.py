@@ -39,7 +39,9 @@ EXPORTS = ("rad_create", "rad_destroy", "rad_last_error", "rad_load_tri", "rad_s
            "rad_tree_info", "rad_cast", "rad_ff_pairs", "rad_counters", "rad_timings",
            "rad_lin_create", "rad_lin_destroy", "rad_lin_last_error", "rad_lin_from_scene",
            "rad_lin_set_matrix", "rad_lin_get_matrix", "rad_lin_set_areas", "rad_lin_autotune", "rad_lin_set_tile",
-           "rad_lin_gemm_ms", "rad_lin_multiply", "rad_lin_gather", "rad_lin_shoot", "rad_lin_adjoint")
+           "rad_lin_gemm_ms", "rad_lin_multiply", "rad_lin_gather", "rad_lin_shoot", "rad_lin_adjoint",
+           "rad_batch_create", "rad_batch_destroy", "rad_batch_last_error", "rad_batch_from_scene",
+           "rad_batch_set_matrix", "rad_batch_solve")
 
 PREC_F32, PREC_F16, PREC_BF16 = 0, 1, 2
 VIS_SEGMENT, VIS_REFERENCE = 0, 1
@@ -93,12 +95,17 @@ def library() -> ctypes.CDLL:
             "rad_lin_gather": [vp, ip, ip, vp, vp, dp, ip, ip, vp, dp, vp, vp],
             "rad_lin_shoot": [vp, ip, vp, vp, vp, dp, ip, vp, vp],
             "rad_lin_adjoint": [vp, ip, vp, vp, vp, dp, ip, ip, vp, vp, vp],
+            "rad_batch_create": [ctypes.POINTER(vp), ip, vp],
+            "rad_batch_destroy": [vp],
+            "rad_batch_from_scene": [vp, ip, vp, ip, vp],
+            "rad_batch_set_matrix": [vp, ip, vp],
+            "rad_batch_solve": [vp, ip, ip, vp, vp, dp, ip, ip, vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
             f.argtypes = args
-            f.restype = None if name in ("rad_destroy", "rad_lin_destroy") else ip
-        for name in ("rad_last_error", "rad_lin_last_error"):
+            f.restype = None if name in ("rad_destroy", "rad_lin_destroy", "rad_batch_destroy") else ip
+        for name in ("rad_last_error", "rad_lin_last_error", "rad_batch_last_error"):
             getattr(L, name).argtypes = [vp]
             getattr(L, name).restype = cp
         _lib = L
@@ -339,6 +346,63 @@ class LinearSystem:
         self._ok(library().rad_lin_adjoint(self.h, B.shape[1], _p(rho), _p(B), _p(g), float(tol), int(max_outer),
                                            int(sweeps), _p(dE), _p(drho), _p(info)))
         return dE, drho, dict(zip(INFO_NAMES, info.tolist()))
+
+
+class SceneBatch:
+    """count independent RGB systems of sizes n[s], solved together
+    (rad_batch_* in include/radiosity_gpu.h)."""
+
+    def __init__(self, sizes):
+        L = library()
+        self.sizes = [int(x) for x in sizes]
+        arr = np.ascontiguousarray(self.sizes, np.int32)
+        h = ctypes.c_void_p()
+        st = L.rad_batch_create(ctypes.byref(h), len(arr), _p(arr))
+        if st != RAD_OK:
+            raise RadError(st, "rad_batch_create failed")
+        self.h = h
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().rad_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _ok(self, st):
+        if st != RAD_OK:
+            raise RadError(st, library().rad_batch_last_error(self.h).decode())
+
+    def from_scene(self, s: int, engine: "Engine", visibility: str = "segment") -> float:
+        ms = ctypes.c_double(0)
+        vis = {"segment": VIS_SEGMENT, "reference": VIS_REFERENCE}[visibility]
+        self._ok(library().rad_batch_from_scene(self.h, int(s), engine.h, vis, ctypes.byref(ms)))
+        return ms.value
+
+    def set_matrix(self, s: int, F):
+        F = np.ascontiguousarray(F, np.float32)
+        assert F.shape == (self.sizes[s], self.sizes[s])
+        self._ok(library().rad_batch_set_matrix(self.h, int(s), _p(F)))
+
+    def solve(self, rho, E, prec="f16", graph=True, tol=1e-6, outer=4, sweeps=8):
+        """rho, E: (sum n, 3).  Returns (B, rel_res per scene, info)."""
+        rho = np.ascontiguousarray(rho, np.float64)
+        E = np.ascontiguousarray(E, np.float64)
+        assert rho.shape == E.shape == (int(self.offsets[-1]), 3)
+        B = np.zeros_like(E)
+        rel = np.zeros(len(self.sizes), np.float64)
+        info = np.zeros(4, np.float64)
+        self._ok(library().rad_batch_solve(self.h, _prec(prec), 1 if graph else 0, _p(rho), _p(E), float(tol),
+                                           int(outer), int(sweeps), _p(B), _p(rel), _p(info)))
+        return B, rel, {"ms": info[0], "build_ms": info[1], "launches": int(info[2]), "above_tol": int(info[3])}
 
 
 def run_cli(args, cwd=None, timeout=None):

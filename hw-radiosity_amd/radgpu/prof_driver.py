@@ -1,7 +1,9 @@
-"""Profiling driver: one hall-200k solve, then a plain 256 MiB device copy
+"""Profiling driver: one hall-200k solve (or, with --linear, the explicit
+matrix workload: the 32768-patch room's form factors, FP16 gathers with 3
+and 192 right-hand sides, 2000 plain shots), then a plain 256 MiB device copy
 that calibrates the FETCH_SIZE / WRITE_SIZE counters in the same run.
 
-    rocprofv3 --pmc FETCH_SIZE -d <dir> -o p -- python3 hw-radiosity_amd/radgpu/prof_driver.py
+    rocprofv3 --pmc FETCH_SIZE -d <dir> -o p -- python3 hw-radiosity_amd/radgpu/prof_driver.py [--linear]
 """
 import ctypes
 import os
@@ -14,12 +16,35 @@ from radgpu import scenes  # noqa: E402
 COPY_BYTES = 256 << 20
 
 
+def linear():
+    n = 32768
+    sc = scenes.patch_room(1, n)
+    e = radgpu.Engine("-triangle", str(n), "-quiet", "1")
+    e.set_scene(sc).prepare_host().upload()
+    lin = radgpu.LinearSystem(n)
+    print("form factors %.1f ms" % lin.from_scene(e), flush=True)
+    E, rho = scenes.emission_reflectance(sc.frgb, sc.brgb)
+    for prec in ("f16", "bf16"):
+        B, info = lin.gather(rho, E, prec=prec, tol=1e-6, max_outer=40, sweeps=8)
+        print(prec, info, flush=True)
+    E192, rho192 = scenes.lighting_vectors(rho, count=64, lights=8, seed=7)
+    B, info = lin.gather(rho192, E192, prec="f16", tol=1e-6, max_outer=40, sweeps=8)
+    print("192", info, flush=True)
+    B, info = lin.shoot(rho, E, max_shots=2000)
+    print("shoot", info, flush=True)
+    lin.close()
+    e.close()
+
+
 def main():
-    s = scenes.hall_scene(2)
-    e = radgpu.Engine("-converge", "1200", "-triangle", "1000000", "-quiet", "1")
-    e.set_scene(s).prepare_host().upload()
-    n, conv = e.shoot()
-    print("shots", n, conv, e.counters(), flush=True)
+    if "--linear" in sys.argv:
+        linear()
+    else:
+        s = scenes.hall_scene(2)
+        e = radgpu.Engine("-converge", "1200", "-triangle", "1000000", "-quiet", "1")
+        e.set_scene(s).prepare_host().upload()
+        n, conv = e.shoot()
+        print("shots", n, conv, e.counters(), flush=True)
     hip = ctypes.CDLL("libamdhip64.so")
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(COPY_BYTES)) == 0

@@ -71,30 +71,33 @@ class _Builder:
     def grid(self, origin, du, dv, nu, nv, frgb, brgb_of=None, flip=False):
         """Triangulate the parallelogram origin + s*du + t*dv, s,t in [0,1],
         into nu x nv cells of two right triangles.  The normal
-        (p1-p0) x (p2-p0) is du x dv, or its opposite with ``flip``."""
+        (p1-p0) x (p2-p0) is du x dv, or its opposite with ``flip``.
+        Triangles come cell by cell (a outer, c inner), two per cell."""
         o = np.asarray(origin, np.float64)
         du = np.asarray(du, np.float64)
         dv = np.asarray(dv, np.float64)
         i = np.arange(nu + 1, dtype=np.float64)[:, None, None]
         j = np.arange(nv + 1, dtype=np.float64)[None, :, None]
         verts = (o + du * (i / nu) + dv * (j / nv)).astype(np.float32)
-        for a in range(nu):
-            for c in range(nv):
-                v00, v10 = verts[a, c], verts[a + 1, c]
-                v01, v11 = verts[a, c + 1], verts[a + 1, c + 1]
-                tris = ((v00, v10, v11), (v00, v11, v01))
-                br = brgb_of(a, c) if brgb_of else (0, 0, 0)
-                for t in tris:
-                    if flip:
-                        t = (t[0], t[2], t[1])
-                    self.p.append(np.stack(t))
-                    self.f.append(frgb)
-                    self.b.append(br)
+        v00, v10 = verts[:-1, :-1], verts[1:, :-1]
+        v01, v11 = verts[:-1, 1:], verts[1:, 1:]
+        t1 = np.stack([v00, v10, v11], axis=2)  # (nu, nv, 3, 3)
+        t2 = np.stack([v00, v11, v01], axis=2)
+        tris = np.stack([t1, t2], axis=2).reshape(-1, 3, 3)
+        if flip:
+            tris = tris[:, [0, 2, 1]]
+        self.p.append(tris)
+        self.f.append(np.tile(np.asarray(frgb, np.int32), (len(tris), 1)))
+        if brgb_of is None:
+            self.b.append(np.zeros((len(tris), 3), np.int32))
+        else:
+            br = np.array([brgb_of(a, c) for a in range(nu) for c in range(nv)], np.int32).reshape(-1, 3)
+            self.b.append(np.repeat(br, 2, axis=0))
 
     def scene(self) -> Scene:
-        return Scene(np.stack(self.p).astype(np.float32),
-                     np.asarray(self.f, np.int32).reshape(-1, 3),
-                     np.asarray(self.b, np.int32).reshape(-1, 3))
+        return Scene(np.concatenate(self.p).astype(np.float32),
+                     np.concatenate(self.f).astype(np.int32).reshape(-1, 3),
+                     np.concatenate(self.b).astype(np.int32).reshape(-1, 3))
 
 
 def _frgb(rng: SplitMix64, lo: float, hi: float):
@@ -227,41 +230,46 @@ def patch_room(seed: int = 1, n: int = 32768) -> Scene:
 
     feed = _Feed([u for f in wall_f for u in inv(f)])
     _room_walls(bld, feed, X, Y, Z, h, (0.0, 1.0))
-    # relabel the ceiling lights (the builder kept walls in order floor, ceiling, ...)
-    nz = _cells(Z, h)
-    floor_n = 2 * nx * ny
-    for k in range(floor_n, floor_n + 2 * nx * ny):
-        cell = (k - floor_n) // 2
-        bld.b[k] = ceil_b(cell // ny, cell % ny)
+    # the ceiling (the builder's second face) emits where its cells are lit
+    bld.b[1] = np.repeat(np.array([ceil_b(a, c) for a in range(nx) for c in range(ny)], np.int32), 2, axis=0)
     for (a, b, c, cx, cy), fs in zip(boxes, box_f):
         _box(bld, _Feed([u for f in fs for u in inv(f)]), cx, cy, a, b, c, 1.0, 0.0, h, (0.0, 1.0))
     sc = bld.scene()
     assert sc.n <= n and sc.n == 2 * _room_cells(h, boxes), (sc.n, n)
     # bisect the largest triangles up to exactly n
     import heapq
-    p = [sc.p[k].astype(np.float64) for k in range(sc.n)]
-    f, b = list(map(tuple, sc.frgb)), list(map(tuple, sc.brgb))
+    import math
+    p = [tuple(tuple(float(x) for x in v) for v in t) for t in sc.p]
+    f, b = [tuple(x) for x in sc.frgb.tolist()], [tuple(x) for x in sc.brgb.tolist()]
+
+    def sub(u, v):
+        return (u[0] - v[0], u[1] - v[1], u[2] - v[2])
 
     def area(t):
-        return 0.5 * float(np.linalg.norm(np.cross(t[1] - t[0], t[2] - t[0])))
+        a, c = sub(t[1], t[0]), sub(t[2], t[0])
+        x = (a[1] * c[2] - a[2] * c[1], a[2] * c[0] - a[0] * c[2], a[0] * c[1] - a[1] * c[0])
+        return 0.5 * math.sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2])
 
     heap = [(-area(t), k) for k, t in enumerate(p)]
     heapq.heapify(heap)
     while len(p) < n:
         _, k = heapq.heappop(heap)
         t = p[k]
-        e = [float(np.dot(t[(v + 1) % 3] - t[v], t[(v + 1) % 3] - t[v])) for v in range(3)]
-        v = int(np.argmax(e))
+        e = []
+        for v in range(3):
+            d = sub(t[(v + 1) % 3], t[v])
+            e.append(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
+        v = e.index(max(e))
         a, bb, c = t[v], t[(v + 1) % 3], t[(v + 2) % 3]
-        m = ((a + bb) * 0.5).astype(np.float32).astype(np.float64)
-        t1, t2 = np.stack([a, m, c]), np.stack([m, bb, c])
+        m = tuple(float(np.float32((a[q] + bb[q]) * 0.5)) for q in range(3))
+        t1, t2 = (a, m, c), (m, bb, c)
         p[k] = t1
         p.append(t2)
         f.append(f[k])
         b.append(b[k])
         heapq.heappush(heap, (-area(t1), k))
         heapq.heappush(heap, (-area(t2), len(p) - 1))
-    return Scene(np.stack(p).astype(np.float32), np.asarray(f, np.int32), np.asarray(b, np.int32))
+    return Scene(np.asarray(p, np.float64).astype(np.float32), np.asarray(f, np.int32), np.asarray(b, np.int32))
 
 
 def emission_reflectance(frgb, brgb, light: float = 1.0):

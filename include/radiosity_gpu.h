@@ -202,6 +202,32 @@ rad_status rad_lin_shoot(rad_lin *lin, int nrhs, const double *rho, const double
 rad_status rad_lin_adjoint(rad_lin *lin, int nrhs, const double *rho, const double *B, const double *dLdB, double tol,
                            int max_outer, int sweeps, double *dLdE, double *dLdrho, double *info);
 
+/* ---- Many scenes of uneven size in one launch ------------------------------
+ *
+ * count independent systems B_s = E_s + diag(rho_s) F_s B_s with three
+ * columns (RGB) each, solved by the mixed-precision gather above with every
+ * step of every scene in one kernel (tile lists spanning the scenes, grids
+ * sized to the machine, converged scenes dropping out), the whole schedule
+ * captured once into a HIP graph and replayed with one launch.  No reference
+ * counterpart (the reference solves one scene per process). */
+typedef struct rad_batch rad_batch;
+
+rad_status rad_batch_create(rad_batch **out, int count, const int *n);
+void rad_batch_destroy(rad_batch *batch);
+const char *rad_batch_last_error(const rad_batch *batch);
+/* Scene `scene`'s F from an uploaded context with n[scene] slots (see
+ * rad_lin_from_scene), or from a host n x n row-major matrix. */
+rad_status rad_batch_from_scene(rad_batch *batch, int scene, rad_ctx *ctx, int visibility, double *ms);
+rad_status rad_batch_set_matrix(rad_batch *batch, int scene, const float *F);
+/* rho, E, B: (sum of n) x 3 row-major, scenes concatenated in order.
+ * `outer` refinements of `sweeps` sweeps each, a scene stopping once its
+ * FP64 relative residual is <= tol.  use_graph = 1: one graph launch;
+ * 0: the same kernels scene after scene (bit-identical results).
+ * rel_res[count] = final relative residuals; info4 = {device ms, graph
+ * build ms (0 when reused), kernel launches, scenes still above tol}. */
+rad_status rad_batch_solve(rad_batch *batch, int prec, int use_graph, const double *rho, const double *E, double tol,
+                           int outer, int sweeps, double *B, double *rel_res, double *info4);
+
 #ifdef __cplusplus
 }
 #endif

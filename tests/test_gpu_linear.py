@@ -265,3 +265,53 @@ def test_full_size_gather_residual_and_sampled_rows(n):
         assert abs(info["power_B"] - float(np.sum(area[:, None] * B))) <= 1e-9 * info["power_B"]
     lin.close()
     eng.close()
+
+
+# ---- many scenes in one launch ----------------------------------------------
+
+def _batch(sizes, seed0=1000):
+    print("batch of %d scenes" % len(sizes), flush=True)
+    b = radgpu.SceneBatch(sizes)
+    E_all, rho_all, Fs = [], [], []
+    for s, n in enumerate(sizes):
+        sc = scenes.patch_room(seed0 + s, n)
+        with radgpu.Engine("-triangle", str(n), "-quiet", "1") as e:
+            e.set_scene(sc).prepare_host().upload()
+            b.from_scene(s, e)
+            with radgpu.LinearSystem(n) as lin:
+                lin.from_scene(e)
+                Fs.append(lin.matrix())
+        E, rho = lo.emission_reflectance(sc.frgb, sc.brgb)
+        E_all.append(E)
+        rho_all.append(rho)
+    return b, Fs, np.vstack(E_all), np.vstack(rho_all)
+
+
+def test_batch_graph_equals_sequential_and_dense_lu():
+    sizes = [700, 64, 1500, 300, 2100, 999]
+    b, Fs, E, rho = _batch(sizes)
+    for prec in ("f16", "bf16"):
+        Bg, relg, info = b.solve(rho, E, prec=prec, graph=True, tol=1e-11, outer=12, sweeps=6)
+        print(prec, "graph", info, relg, flush=True)
+        Bs, rels, info_s = b.solve(rho, E, prec=prec, graph=False, tol=1e-11, outer=12, sweeps=6)
+        print(prec, "sequential", info_s, flush=True)
+        assert np.array_equal(Bg, Bs) and np.array_equal(relg, rels)
+        assert info["above_tol"] == 0 and np.all(relg <= 1e-11)
+        # replaying the cached graph gives the same answer
+        Bg2, _, info2 = b.solve(rho, E, prec=prec, graph=True, tol=1e-11, outer=12, sweeps=6)
+        assert info2["build_ms"] == 0 and np.array_equal(Bg, Bg2)
+        off = b.offsets
+        for s, F in enumerate(Fs):
+            ref = lo.dense_solve(F, rho[off[s]:off[s + 1]], E[off[s]:off[s + 1]])
+            assert rel(Bg[off[s]:off[s + 1]], ref) < 1e-9, (prec, s)
+    b.close()
+
+
+def test_batch_scenes_stop_at_tolerance():
+    sizes = [400, 900]
+    b, Fs, E, rho = _batch(sizes, seed0=3000)
+    B, r, info = b.solve(rho, E, prec="f16", graph=True, tol=1e-4, outer=10, sweeps=4)
+    assert np.all(r <= 1e-4) and info["above_tol"] == 0
+    B2, r2, info2 = b.solve(rho, E, prec="f16", graph=True, tol=1e-14, outer=2, sweeps=2)
+    assert info2["above_tol"] == 2 and np.all(r2 > 1e-14)
+    b.close()
