@@ -83,6 +83,39 @@ def dense_reference(F, rho, E, tmp):
     return np.load(os.path.join(tmp, "B.npy"))
 
 
+def batch_leg(count=256):
+    import radgpu
+    from radgpu import scenes
+    sizes = scenes.batch_sizes(count, 500, 8000, seed=4)
+    t0 = time.perf_counter()
+    b = radgpu.SceneBatch(sizes)
+    E_all, rho_all, ff_ms = [], [], 0.0
+    for s, n in enumerate(sizes):
+        sc = scenes.patch_room(1000 + s, n)
+        with radgpu.Engine("-triangle", str(n), "-quiet", "1") as e:
+            e.set_scene(sc).prepare_host().upload()
+            ff_ms += b.from_scene(s, e)
+        E, rho = scenes.emission_reflectance(sc.frgb, sc.brgb)
+        E_all.append(E)
+        rho_all.append(rho)
+    E, rho = np.vstack(E_all), np.vstack(rho_all)
+    log("batch: %d scenes, %d patches, built in %.1f s (form factors %.0f ms on the device)" % (
+        count, sum(sizes), time.perf_counter() - t0, ff_ms))
+    res = {"scenes": count, "patches": int(sum(sizes)), "sizes": "log-uniform [500, 8000], seed 4; rooms seeds 1000+i",
+           "matrix_entries": int(sum(n * n for n in sizes)), "form_factor_ms": ff_ms}
+    for prec in ("f16", "bf16"):
+        b.solve(rho, E, prec=prec, graph=True, tol=1e-6, outer=6, sweeps=8)  # capture + warm
+        Bg, relg, ig = b.solve(rho, E, prec=prec, graph=True, tol=1e-6, outer=6, sweeps=8)
+        Bs, rels, is_ = b.solve(rho, E, prec=prec, graph=False, tol=1e-6, outer=6, sweeps=8)
+        res[prec] = {"graph_ms": ig["ms"], "graph_nodes": ig["launches"], "sequential_ms": is_["ms"],
+                     "sequential_launches": is_["launches"], "speedup_graph_vs_sequential": is_["ms"] / ig["ms"],
+                     "identical": bool(np.array_equal(Bg, Bs)), "max_rel_residual": float(relg.max()),
+                     "scenes_above_tol": ig["above_tol"]}
+        log("batch %s: %s" % (prec, res[prec]))
+    b.close()
+    return res
+
+
 def linear_leg(n=32768, n_adj=16384):
     """Explicit-matrix solvers on the patch room: the mixed-precision gather
     (FP16 / BF16 MFMA, FP64 refinement) against plain FP32 progressive
@@ -148,6 +181,9 @@ def linear_leg(n=32768, n_adj=16384):
     log("batched 192: %s" % info)
     lin.close()
     eng.close()
+    # many scenes: 256 rooms of log-uniform size in one graph launch vs the
+    # same kernels launched scene after scene
+    out["batched_scenes"] = batch_leg()
     # adjoint at n_adj
     sc = scenes.patch_room(5, n_adj)
     eng = radgpu.Engine("-triangle", str(n_adj), "-quiet", "1")

@@ -125,7 +125,8 @@ __device__ __forceinline__ void gemm_tile(const E *__restrict__ A, const E *__re
 
 // One wave's share of the FP64 product with an FP32 matrix on the FP64 MFMA
 // (v_mfma_f64_16x16x4_f64: lane l supplies A[l%16][l/16] and B[l/16][l%16],
-// receives D[4*(l/16)+j][l%16]): rows 16*rb0 .. 16*(rb0+RT)-1, columns
+// receives D[l/16 + 4*j][l%16], j = 0..3 -- rows interleaved, unlike the
+// 16-bit MFMAs): rows 16*rb0 .. 16*(rb0+RT)-1, columns
 // 16*cb0 .. 16*(cb0+CT)-1, k in [k0, k1) (a multiple of 32).  Per 32 k each
 // lane loads 8 consecutive F entries (converted exactly to FP64) and 8
 // consecutive X entries, then runs 8 MFMAs, step s pairing k = kk + 8*(l/16)
@@ -178,7 +179,7 @@ __device__ __forceinline__ void fx64_tile(const float *__restrict__ F, const dou
 		for (int c = 0; c < CT; c++)
 #pragma unroll
 			for (int j = 0; j < 4; j++) {
-				const int row = (rb0 + r) * 16 + (lane >> 4) * 4 + j;
+				const int row = (rb0 + r) * 16 + (lane >> 4) + 4 * j;
 				part[(size_t)row * rp + (cb0 + c) * 16 + lr] = acc[r][c][j];
 			}
 }
