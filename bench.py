@@ -48,12 +48,65 @@ def algorithmic_bytes(cnt):
     return 40 * cnt["rays"] + 32 * cnt["cells"] + 8 * cnt["levels"] + 76 * cnt["tests"]
 
 
+def reference_drop_in(cores):
+    """The shipped reference binary (oracle/_ref/rad, built from the
+    reference's sources by oracle/ref.mk) against the engine's drop-in `rad`
+    on the reference's own CPU-runnable case (the seeded C1 room): the
+    "Radiosity Method took" span of each (InitRadiosity .. final .fin) and
+    whether the .fin files are byte-identical."""
+    import hashlib
+    import re
+    import subprocess
+    from radgpu import scenes
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "rad")
+    gpu_bin = os.path.join(ROOT, "hw-radiosity_amd", "_bin", "rad")
+    if not os.path.exists(ref_bin):
+        return {"skipped": "oracle/_ref/rad not built"}
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        tri = os.path.join(tmp, "c1.tri")
+        scenes.write_tri(tri, scenes.room_scene(1))
+        for name, opts in (("c1 (-converge 1200 -delta_ff 5e-6 -adapt_area 5 -triangle 20000)",
+                            ["-converge", "1200", "-delta_ff", "5e-6", "-adapt_area", "5", "-triangle", "20000"]),
+                           ("c1 at -converge 100", ["-converge", "100", "-triangle", "20000"])):
+            row = {}
+            for who, exe in (("reference", ref_bin), ("engine", gpu_bin)):
+                prefix = os.path.join(tmp, who)
+                t0 = time.perf_counter()
+                try:
+                    r = subprocess.run([exe] + opts + ["-jobs", str(cores), tri, "-o", prefix], capture_output=True,
+                                       text=True, timeout=300)
+                except (OSError, subprocess.TimeoutExpired) as e:
+                    row[who] = {"error": str(e)}
+                    continue
+                wall = time.perf_counter() - t0
+                m = re.search(r"Radiosity Method took \S*?([0-9.]+)", re.sub(r"\x1b\[[0-9;]*m", "", r.stderr))
+                it = re.search(r"#Iterations (\d+)", r.stderr)
+                fin = prefix + ".fin"
+                row[who] = {"rc": r.returncode, "wall_s": wall, "method_s": float(m.group(1)) if m else None,
+                            "iterations": int(it.group(1)) if it else None,
+                            "fin_md5": hashlib.md5(open(fin, "rb").read()).hexdigest() if os.path.exists(fin) else None}
+            if "method_s" in row.get("reference", {}) and "method_s" in row.get("engine", {}):
+                a, b = row["reference"]["method_s"], row["engine"]["method_s"]
+                row["speedup_method_span"] = a / b if a and b else None
+                row["fin_identical"] = row["reference"]["fin_md5"] == row["engine"]["fin_md5"]
+            out[name] = row
+            log("drop-in %s: %s" % (name, row))
+    return out
+
+
 def cpu_baseline(scene, seconds, cores):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     o = ol.Oracle(*(OPTS + ["-quiet", "1", "-jobs", str(cores)]))
     o.load(scene)
     o.build_tree()
+    # InitRadiosity's O(N^2) neighbour pass as the reference runs it: a
+    # sample of rows, extrapolated to all of them
+    rows = 400
+    t0 = time.perf_counter()
+    o.neighbor_rows(rows)
+    init_est = (time.perf_counter() - t0) * o.trinum / rows
     o.init_radiosity()
     o.prepartition()
     t0 = time.perf_counter()
@@ -65,7 +118,7 @@ def cpu_baseline(scene, seconds, cores):
             break
     dt = time.perf_counter() - t0
     c = o.counters()
-    return {"rays": int(c[ol.CNT_RAYS]), "seconds": dt, "shots": shots}
+    return {"rays": int(c[ol.CNT_RAYS]), "seconds": dt, "shots": shots, "init_est_s": init_est, "rows": rows}
 
 
 def dense_reference(F, rho, E, tmp):
@@ -310,6 +363,14 @@ def main():
                                          "discarded full-source rays" % (cb["shots"], cb["seconds"]),
                                "time_to_converge_est_s": per_shot * shots_all / args.steps}
         out["speedup_time_to_converge_device"] = out["cpu_baseline"]["time_to_converge_est_s"] / (total_s / args.steps)
+        out["cpu_baseline"]["init_neighbour_pass_est_s"] = cb["init_est_s"]
+        out["cpu_baseline"]["init_sample"] = ("the reference's exact O(N^2) neighbour pass (rad.cpp:295-334, OpenMP "
+                                              "over j) on the first %d rows, extrapolated to all %d" %
+                                              (cb["rows"], scene.n))
+        est = cb["init_est_s"] + out["cpu_baseline"]["time_to_converge_est_s"]
+        out["cpu_baseline"]["end_to_end_est_s"] = est
+        out["speedup_end_to_end_vs_cpu_est"] = est / e2e_s
+        out["reference_drop_in_c1"] = reference_drop_in(cores)
     print(json.dumps(out), flush=True)
 
 

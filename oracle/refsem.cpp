@@ -1231,6 +1231,32 @@ static float centroid_ff(rs_ctx *c, int i, int j, int vis, long long *k)
 	return occ ? 0.0f : ff;
 }
 
+// Cost sample of InitRadiosity's exact neighbour pass as the reference runs
+// it (rad.cpp:295-334: i serial, j in an OpenMP parallel for, every pair's
+// vertex comparisons): rows i in [0, rows), matches counted, nothing
+// written.  Returns the matches; the caller times the call.
+long long rs_neighbor_rows(rs_ctx *c, int rows)
+{
+	const int n = c->trinum;
+	long long matches = 0;
+	for (int i = 0; i < rows && i < n; i++) {
+		const Rec &tp = c->T[i];
+#pragma omp parallel for reduction(+ : matches)
+		for (int j = 0; j < n; j++) {
+			if (j == i) continue;
+			const Rec &tn = c->T[j];
+			for (int v = 0; v < 3; v++)
+				for (int m = 0; m < 3; m++) {
+					if (!rs_ctx::veq(tp.p[v], tn.p[m])) continue;
+					if (rs_ctx::veq(tp.p[(v + 1) % 3], tn.p[(m + 1) % 3]) ||
+					    rs_ctx::veq(tp.p[(v + 1) % 3], tn.p[(m + 2) % 3]))
+						matches++;
+				}
+		}
+	}
+	return matches;
+}
+
 int rs_ff_centroid(rs_ctx *c, int vis, int n, const int *pairs, float *out)
 {
 	c->build_fine();

@@ -91,6 +91,10 @@ enum { RS_VIS_SEGMENT = 0, RS_VIS_REFERENCE = 1, RS_VIS_NONE = 2, RS_VIS_BRUTE =
 int rs_ff_centroid(rs_ctx *, int vis, int n, const int *pairs, float *out);
 /* ...and whole rows (n = trinum entries each), scaled to sum <= 1. */
 int rs_ff_rows(rs_ctx *, int vis, int nrows, const int *rows, float *out);
+/* Cost sample of InitRadiosity's O(N^2) neighbour pass (rad.cpp:295-334):
+ * the reference's comparisons for rows i < rows, OpenMP over j as the
+ * reference runs it, nothing written; returns the edge matches. */
+long long rs_neighbor_rows(rs_ctx *, int rows);
 void rs_counters(rs_ctx *, long long *out);
 void rs_reset_counters(rs_ctx *);
 

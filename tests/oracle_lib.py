@@ -49,6 +49,8 @@ def lib():
             f = getattr(L, name)
             f.restype = ip
             f.argtypes = args
+        L.rs_neighbor_rows.argtypes = [vp, ip]
+        L.rs_neighbor_rows.restype = ctypes.c_longlong
         L.rs_counters.argtypes = [vp, vp]
         L.rs_reset_counters.argtypes = [vp]
         _lib = L
@@ -167,6 +169,9 @@ class Oracle:
         out = np.zeros((len(rows), self.trinum), np.float32)
         lib().rs_ff_rows(self.h, int(vis), len(rows), _ptr(rows), _ptr(out))
         return out
+
+    def neighbor_rows(self, rows):
+        return lib().rs_neighbor_rows(self.h, int(rows))
 
     def counters(self):
         out = np.zeros(8, np.int64)
