@@ -47,6 +47,94 @@ __global__ __launch_bounds__(kB) void k_gemm(const E *__restrict__ A, const E *_
 	gemm_tile<E, RT, CT>(A, XT, part + (size_t)s * ld * rp, ld, rp, (blockIdx.x * 4 + w) * RT, kb0, kb0 + (kchunk >> 5));
 }
 
+// Wide right-hand sides (CT = 8 or 12 column tiles): the B fragments of
+// each group of U k-blocks are loaded once per block, cooperatively, into LDS
+// (double-buffered) and read there by the four waves, so the B operand costs
+// one L2 read per block instead of one per wave.  A wave owns 16*RT rows;
+// its A fragments for the next group stream from HBM while the current group
+// feeds the MFMAs.  Needs (k-blocks per chunk) % (2U) == 0 (PER is whole
+// for U * CT a multiple of 4).
+template <typename E, int RT, int CT>
+__global__ __launch_bounds__(kB) void k_gemm_lds(const E *__restrict__ A, const E *__restrict__ XT,
+                                                 float *__restrict__ part, int ld, int rp, int kchunk)
+{
+	typedef typename VecOf<E>::type V;
+	constexpr int U = CT >= 12 ? 2 : 4;     // k-blocks per group (registers: 4 x 12 accumulators)
+	constexpr int PER = U * CT * 64 / kB;  // B fragments (16 bytes) per thread per group
+	__shared__ V sb[2][U * CT * 64];
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const int rb0 = (blockIdx.x * 4 + w) * RT;
+	const int s = blockIdx.y, kb0 = s * (kchunk >> 5), kb1 = kb0 + (kchunk >> 5);
+	const int nkb = ld >> 5;
+	const V *Ap = (const V *)A + lane;
+	const V *Bg = (const V *)XT;
+	f4 acc[RT][CT];
+#pragma unroll
+	for (int r = 0; r < RT; r++)
+#pragma unroll
+		for (int c = 0; c < CT; c++) acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
+	V a0[U][RT], a1[U][RT], bs[PER];
+	// group g covers k-blocks kb .. kb+U-1; LDS slot (u * CT + c) * 64 + lane
+	auto load_b = [&](int kb) {
+#pragma unroll
+		for (int m = 0; m < PER; m++) {
+			const int q = threadIdx.x + m * kB;
+			const int l2 = q & 63, uc = q >> 6, u = uc / CT, c = uc % CT;
+			bs[m] = Bg[((size_t)c * nkb + kb + u) * 64 + l2];
+		}
+	};
+	auto store_b = [&](int buf) {
+#pragma unroll
+		for (int m = 0; m < PER; m++) sb[buf][threadIdx.x + m * kB] = bs[m];
+	};
+	auto load_a = [&](V (&a)[U][RT], int kb) {
+#pragma unroll
+		for (int u = 0; u < U; u++)
+#pragma unroll
+			for (int r = 0; r < RT; r++) a[u][r] = __builtin_nontemporal_load(Ap + ((size_t)(rb0 + r) * nkb + kb + u) * 64);
+	};
+	auto mma = [&](const V (&a)[U][RT], int buf) {
+#pragma unroll
+		for (int u = 0; u < U; u++)
+#pragma unroll
+			for (int c = 0; c < CT; c++) {
+				const V b = sb[buf][(u * CT + c) * 64 + lane];
+#pragma unroll
+				for (int r = 0; r < RT; r++) acc[r][c] = mfma(a[u][r], b, acc[r][c]);
+			}
+	};
+	load_b(kb0);
+	load_a(a0, kb0);
+	store_b(0);
+	__syncthreads();
+	for (int kb = kb0; kb < kb1; kb += 2 * U) {
+		// group kb (buffer 0, a0) computes while group kb + U loads
+		load_b(kb + U);
+		load_a(a1, kb + U);
+		mma(a0, 0);
+		store_b(1);
+		__syncthreads();
+		const bool more = kb + 2 * U < kb1;
+		if (more) {
+			load_b(kb + 2 * U);
+			load_a(a0, kb + 2 * U);
+		}
+		mma(a1, 1);
+		if (more) store_b(0);
+		__syncthreads();
+	}
+	const int lr = lane & 15;
+#pragma unroll
+	for (int r = 0; r < RT; r++)
+#pragma unroll
+		for (int c = 0; c < CT; c++)
+#pragma unroll
+			for (int j = 0; j < 4; j++) {
+				const int row = (rb0 + r) * 16 + (lane >> 4) * 4 + j;
+				part[((size_t)s * ld + row) * rp + c * 16 + lr] = acc[r][c][j];
+			}
+}
+
 // d = rhs + rho o (inv_s * sum_s part); XT <- d (fragment order).
 // Padding of XT (rows >= nrhs, k >= n) is never written and stays zero.
 template <typename E>
@@ -363,14 +451,17 @@ int launch_gemm(int rt, int ct, const E *A, const E *XT, float *part, int ld, in
 		GR(1)
 	} else if (rt == 2) {
 		GR(2)
-	} else if (rt == 4 && ct <= 8) {
+	} else if (rt == 4 && ct <= 4) {
 		switch (ct) {
 		case 1: G(4, 1); break;
 		case 2: G(4, 2); break;
 		case 4: G(4, 4); break;
-		case 8: G(4, 8); break;
 		default: return -1;
 		}
+	} else if (rt == 4 && ct == 8) {
+		hipLaunchKernelGGL((k_gemm_lds<E, 4, 8>), grid, blk, 0, st, A, XT, part, ld, rp, kchunk);
+	} else if (rt == 4 && ct == 12) {
+		hipLaunchKernelGGL((k_gemm_lds<E, 4, 12>), grid, blk, 0, st, A, XT, part, ld, rp, kchunk);
 	} else {
 		return -1;
 	}
@@ -381,8 +472,9 @@ int launch_gemm(int rt, int ct, const E *A, const E *XT, float *part, int ld, in
 
 bool valid_tile(int rt, int ct, int S, int ld)
 {
-	if (rt == 4 && ct > 8) return false;
-	const int u2 = 2 * (ct <= 4 ? 4 : 2);  // k-steps per pipelined loop trip (GemmUnroll)
+	// k-blocks per pipelined loop trip: 2 * GemmUnroll<ct>::U, or 2 * U of
+	// the LDS kernel (rt = 4 with 8 or 12 column tiles: U = 4 or 2)
+	const int u2 = (rt == 4 && ct >= 8) ? (ct >= 12 ? 4 : 8) : 2 * (ct <= 4 ? 4 : 2);
 	return ld % (64 * rt) == 0 && ld % (32 * S) == 0 && (ld / 32 / S) % u2 == 0;
 }
 

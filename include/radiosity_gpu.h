@@ -168,7 +168,8 @@ rad_status rad_lin_set_areas(rad_lin *lin, const double *area); /* power sums an
  * by timing every candidate on this matrix; gather does it on first use. */
 rad_status rad_lin_autotune(rad_lin *lin, int nrhs, int prec, int *rt, int *splitk);
 /* Forces the tile for nrhs columns instead of the autotuner's choice:
- * rt in {1, 2, 4} (rt = 4 only up to 128 columns), splitk in {1, 2, 4, 8},
+ * rt in {1, 2, 4} (rt = 4 with more than 64 columns shares the right-hand
+ * sides through LDS), splitk in {1, 2, 4, 8},
  * subject to the matrix size. */
 rad_status rad_lin_set_tile(rad_lin *lin, int nrhs, int prec, int rt, int splitk);
 /* Average device time of one low-precision product F X, reps launches. */

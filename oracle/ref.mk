@@ -10,6 +10,8 @@
 #   REF-NOFMA  the same with -ffp-contract=off (source-order arithmetic)
 # refdump links the reference's own translation units with oracle/refdump.cpp
 # (a dump harness), renaming the reference's main() with -Dmain=radref_main.
+# rad_oldfmt is the shipped build with the reference's own OLD_FORMAT switch
+# (rad.cpp:150) defined on the command line: the per-vertex colour output.
 
 REF     ?= /root/reference
 HERE    := $(patsubst %/,%,$(dir $(lastword $(MAKEFILE_LIST))))
@@ -23,7 +25,7 @@ SRCS    := $(REF)/src/shade.cpp $(REF)/src/raycast.cpp $(REF)/src/rad.cpp
 HDRS    := $(wildcard $(REF)/include/*.h)
 
 .PHONY: all clean
-all: $(OUT)/rad $(OUT)/rad_nofma $(OUT)/refdump $(OUT)/refdump_nofma
+all: $(OUT)/rad $(OUT)/rad_nofma $(OUT)/rad_oldfmt $(OUT)/refdump $(OUT)/refdump_nofma
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -33,6 +35,9 @@ $(OUT)/rad: $(SRCS) $(HDRS) | $(OUT)
 
 $(OUT)/rad_nofma: $(SRCS) $(HDRS) | $(OUT)
 	$(CXX) $(NOFMA_FL) $(SRCS) -o $@ -lm
+
+$(OUT)/rad_oldfmt: $(SRCS) $(HDRS) | $(OUT)
+	$(CXX) $(FMA_FL) -DOLD_FORMAT $(SRCS) -o $@ -lm
 
 $(OUT)/%.fma.o: $(REF)/src/%.cpp $(HDRS) | $(OUT)
 	$(CXX) $(FMA_FL) -Dmain=radref_main -c $< -o $@

@@ -9,6 +9,11 @@ each stage, .fin md5s, iteration counts, and per-ray / per-pair vectors.
 
 Reference builds used: oracle/_ref/refdump (REF-FMA, shipped flags) and
 oracle/_ref/refdump_nofma (-ffp-contract=off); see oracle/ref.mk.
+
+    python tests/golden/make_goldens.py --old-format
+
+adds only the .fin md5s of oracle/_ref/rad_oldfmt (the shipped build with
+the reference's OLD_FORMAT switch defined) to goldens.json.
 """
 from __future__ import annotations
 
@@ -167,5 +172,42 @@ def main():
     shutil.rmtree(tmp)
 
 
+OLD_FORMAT_CASES = [("c1", "c1_seed1.tri", C1_OPTS), ("room_t5000", "room.tri", ["-triangle", "5000"]),
+                    ("box_t2000", "box.tri", ["-triangle", "2000"])]
+
+
+def old_format():
+    """.fin of the OLD_FORMAT build (per-vertex colours, rad.cpp:150-159)."""
+    path = os.path.join(HERE, "goldens.json")
+    with open(path) as fh:
+        out = json.load(fh)
+    tmp = tempfile.mkdtemp(prefix="goldens_")
+    c1 = os.path.join(tmp, "c1_seed1.tri")
+    scenes.write_tri(c1, scenes.room_scene(1))
+    inputs = {"c1_seed1.tri": c1, "room.tri": os.path.join(REF_MODELS, "room.tri"),
+              "box.tri": os.path.join(REF_MODELS, "box.tri")}
+    exe = os.path.join(ROOT, "oracle", "_ref", "rad_oldfmt")
+    res = {}
+    for case, model, opts in OLD_FORMAT_CASES:
+        prefix = os.path.join(tmp, case)
+        subprocess.run([exe] + opts + ["-jobs", "1", "-write_cycle", "1000000", model_path(inputs, model), "-o", prefix],
+                       capture_output=True, text=True, check=True, timeout=600)
+        fin = open(prefix + ".fin", "rb").read()
+        res[case] = {"input": model, "opts": opts, "fin_md5": hashlib.md5(fin).hexdigest(),
+                     "fin_bytes": len(fin)}
+        print("old_format", case, res[case], flush=True)
+    out["old_format"] = res
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+    shutil.rmtree(tmp)
+
+
+def model_path(inputs, model):
+    return inputs[model]
+
+
 if __name__ == "__main__":
-    main()
+    if "--old-format" in sys.argv:
+        old_format()
+    else:
+        main()

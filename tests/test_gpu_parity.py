@@ -105,6 +105,17 @@ def test_cli_drop_in(goldens, inputs, tmp_path):
         ["c1.10", "c1.20"]
 
 
+@pytest.mark.parametrize("case", ["c1", "room_t5000", "box_t2000"])
+def test_cli_old_format(case, goldens, inputs, tmp_path):
+    """-old_format 1 writes the OLD_FORMAT build's per-vertex .fin."""
+    print("case", case, flush=True)
+    e = goldens["old_format"][case]
+    r = radgpu.run_cli(e["opts"] + ["-old_format", "1", "-write_cycle", "1000000", inputs[e["input"]],
+                                    "-o", str(tmp_path / "o")], timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert hashlib.md5(open(tmp_path / "o.fin", "rb").read()).hexdigest() == e["fin_md5"]
+
+
 @pytest.mark.parametrize("select", ["high", "low"])
 def test_hall_subset_matches_oracle(select):
     """The hall generator (yawed boxes, pre-partition at scale) with 40 boxes:

@@ -64,6 +64,20 @@ def test_full_run_matches_reference(key, goldens, inputs, tmp_path):
     assert hashlib.md5(fin).hexdigest() == e["fin_md5"]
 
 
+@pytest.mark.parametrize("case", ["c1", "room_t5000", "box_t2000"])
+def test_old_format_matches_reference(case, goldens, inputs, tmp_path):
+    """The OLD_FORMAT build's per-vertex .fin (rad.cpp:150-159), from the
+    reference compiled with its own switch (oracle/ref.mk rad_oldfmt)."""
+    e = goldens["old_format"][case]
+    o = ol.Oracle(*(e["opts"] + ["-old_format", "1", "-quiet", "1", "-write_cycle", "1000000"]))
+    o.load_file(inputs[e["input"]])
+    o.build_tree()
+    o.run(str(tmp_path / "out"))
+    fin = open(tmp_path / "out.fin", "rb").read()
+    assert len(fin) == e["fin_bytes"]
+    assert hashlib.md5(fin).hexdigest() == e["fin_md5"]
+
+
 def test_neighbor_search_hash_equals_exact(inputs):
     """The hashed neighbour search reproduces the O(N^2) scan in loop order."""
     for m in ("box.tri", "room.tri", "blocks.tri", "c1_seed1.tri"):
