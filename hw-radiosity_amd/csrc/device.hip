@@ -109,7 +109,7 @@ struct Loop {
 	int shots;
 	int nsrc;
 	int ncand;  // candidates after the last update: a lower bound on the shots left
-	int pad;
+	int nrays;  // rays of the current source (length of the compacted ray list)
 	SrcDesc src[kMaxSrc];
 };
 
@@ -517,12 +517,12 @@ __device__ void add_block_counters(const int (&vals_in)[8], long long *bcnt)
 // where no ray is cast.
 template <bool F>
 __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigned char *live, float *ffbuf,
-                                                long long *bcnt, int nblocks)
+                                                long long *bcnt, int nblocks, int *raycnt)
 {
 	if (L->done || k >= L->nsrc) return;
 	const RecvMap m(nblocks);
 	const SrcDesc &s = L->src[k];
-	int ref_rays = 0;
+	int ref_rays = 0, nray = 0;
 #pragma unroll 1
 	for (int g = 0; g < kGroups; g++) {
 		const int r = m.slot(g);
@@ -562,27 +562,77 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 		}
 		ffbuf[2 * sv] = f0;
 		ffbuf[2 * sv + 1] = f1;
+		nray += (f0 != 0.0f) + (f1 != 0.0f);
+	}
+	// rays of this block's contiguous lane range [vb * 6 * kRecvPerBlock, +6 * kRecvPerBlock)
+	__shared__ int wray[kWavesPerBlock];
+	for (int off = 32; off > 0; off >>= 1) nray += __shfl_xor(nray, off);
+	if ((threadIdx.x & 63) == 0) wray[threadIdx.x >> 6] = nray;
+	__syncthreads();
+	if (threadIdx.x == 0 && m.vb < nblocks) {
+		int t = 0;
+		for (int w = 0; w < kWavesPerBlock; w++) t += wray[w];
+		raycnt[m.vb] = t;
 	}
 	const int vals[8] = {0, ref_rays, 0, 0, 0, 0, 0, 0};
 	add_block_counters(vals, bcnt);
 }
 
-// Source k's rays, part 2: one lane per (receiver, vertex, half) in slot
-// order, so a wave's rays start from ~11 neighbouring receivers toward the
-// same point; lanes without a ray exit at once (facing and non-facing
-// receivers come in spatial runs, so waves are mostly all busy or all idle).
+// The rays of source k as a dense list in lane order: block b (one per
+// k_gen range) places the lanes i of [b * 6 * kRecvPerBlock, +6 *
+// kRecvPerBlock) whose form-factor prefix is non-zero at offset
+// sum(raycnt[0..b)), in ascending i; the last block stores the total.
+__global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float *ffbuf, int nlanes,
+                                                    const int *raycnt, int nblocks, int *rays)
+{
+	if (L->done || k >= L->nsrc) return;
+	__shared__ int wsum[kWavesPerBlock];
+	__shared__ int base;
+	const int b = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	int off = 0;
+	for (int j = threadIdx.x; j < b; j += kBlock) off += raycnt[j];
+	for (int o = 32; o > 0; o >>= 1) off += __shfl_xor(off, o);
+	if (lane == 0) wsum[w] = off;
+	__syncthreads();
+	if (threadIdx.x == 0) base = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+	__syncthreads();
+	constexpr int chunk = 6 * kRecvPerBlock;
+	const int i0 = b * chunk, i1 = min(nlanes, i0 + chunk);
+	int run = base;
+	for (int r0 = i0; r0 < i1; r0 += kBlock) {
+		const int i = r0 + threadIdx.x;
+		const bool ray = i < i1 && ffbuf[i] != 0.0f;
+		const unsigned long long bal = __ballot(ray);
+		const int rank = __popcll(bal & ((1ull << lane) - 1));
+		__syncthreads();  // wsum is reused each round
+		if (lane == 0) wsum[w] = __popcll(bal);
+		__syncthreads();
+		int before = run;
+		for (int q = 0; q < w; q++) before += wsum[q];
+		if (ray) rays[before + rank] = i;
+		run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+	}
+	if (b == nblocks - 1 && threadIdx.x == 0) L->nrays = run;
+}
+
+// Source k's rays, part 2: one lane per ray of the compacted list, which
+// keeps slot order, so a wave's rays start from neighbouring receivers
+// toward the same point and every lane has a ray.
 // RayHitted (raycast.cpp:335-360); a ray that does not reach the source
 // zeroes its form factor (shade.cpp:124-126).  XCD-contiguous block order.
 template <bool F>
 __global__ __launch_bounds__(kBlock) void k_trace(DScene S, DRecv R, Loop *L, int k, float *ffbuf, long long *bcnt,
-                                                  int nblocks)
+                                                  const int *rays)
 {
 	if (L->done || k >= L->nsrc) return;
-	const int xcd = blockIdx.x & 7, per = (nblocks + 7) >> 3;
+	// the compacted list spread XCD-contiguously over the blocks that have work
+	const int nr = L->nrays, nb = (nr + kBlock - 1) / kBlock, per = (nb + 7) >> 3;
+	const int xcd = blockIdx.x & 7;
 	const int vb = xcd * per + (blockIdx.x >> 3);
-	const int i = vb * kBlock + threadIdx.x;
+	const int j = vb * kBlock + threadIdx.x;
 	Cnt c = {0, 0, 0, 0, 0, 0, 0};
-	if (vb < nblocks && i < 6 * R.nslots && ffbuf[i] != 0.0f) {
+	if ((int)(blockIdx.x >> 3) < per && j < nr) {
+		const int i = rays[j];
 		const SrcDesc &s = L->src[k];
 		const int sv = i >> 1, h = i & 1, r = sv / 3;
 		const float px = R.vx[sv], py = R.vy[sv], pz = R.vz[sv];
@@ -945,6 +995,7 @@ struct DeviceEngine::Impl {
 	DevBuf<unsigned long long> cand;
 	DevBuf<unsigned char> live;
 	DevBuf<float> ffbuf;
+	DevBuf<int> raycnt, rays;  // per-k_gen-block ray counts, compacted ray list
 	DScene S;
 	DRecv R;
 	Loop *host_loop = nullptr;  // pinned mirror of the loop header
@@ -1086,7 +1137,9 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	std::vector<long long> zeros((size_t)std::max(grid, tgrid) * 8, 0);
 	if (d.bcnt.put(zeros.data(), zeros.size(), st, err)) return -1;
 	if (!d.lowest && d.parallel_src > 1 && d.cand.alloc((size_t)n + 1, err)) return -1;
-	if (d.live.alloc((size_t)n + 1, err) || d.ffbuf.alloc((size_t)n * 6 + 2, err)) return -1;
+	if (d.live.alloc((size_t)n + 1, err) || d.ffbuf.alloc((size_t)n * 6 + 2, err) ||
+	    d.raycnt.alloc((size_t)d.nblocks + 1, err) || d.rays.alloc((size_t)n * 6 + 1, err))
+		return -1;
 	std::memset(d.host_loop, 0, sizeof(Loop));
 	if (d.loop.put(d.host_loop, 1, st, err)) return -1;
 	const float thr_t = smallest_float_at_least(0.0001);
@@ -1123,7 +1176,8 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 	const int start = d.host_loop->shots;
 	const int grid = ((d.nblocks + 7) / 8) * 8;
 	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
-	if (d.bcnt.n < (size_t)std::max(grid, tgrid) * 8 || d.ffbuf.n < (size_t)d.R.nslots * 6 || d.live.n < (size_t)d.R.nslots) {
+	if (d.bcnt.n < (size_t)std::max(grid, tgrid) * 8 || d.ffbuf.n < (size_t)d.R.nslots * 6 || d.live.n < (size_t)d.R.nslots ||
+	    d.raycnt.n < (size_t)d.nblocks || d.rays.n < (size_t)d.R.nslots * 6) {
 		err = "device buffers smaller than the launch grids";
 		return -1;
 	}
@@ -1153,11 +1207,13 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 			const int nk = multi ? d.parallel_src : 1;
 			for (int k = 0; grid > 0 && k < nk; k++) {
 				const dim3 g(grid), blk(kBlock), tg(tgrid);
-				if (d.fma) hipLaunchKernelGGL(k_gen<true>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks);
-				else hipLaunchKernelGGL(k_gen<false>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks);
+				if (d.fma) hipLaunchKernelGGL(k_gen<true>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
+				else hipLaunchKernelGGL(k_gen<false>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
+				hipLaunchKernelGGL(k_compact, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, 6 * d.R.nslots, d.raycnt.p,
+				                   d.nblocks, d.rays.p);
 				if (timing && k == 0) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv)], st), "event");
-				if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, blk, 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.trace_nblocks);
-				else hipLaunchKernelGGL(k_trace<false>, tg, blk, 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.trace_nblocks);
+				if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, blk, 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
+				else hipLaunchKernelGGL(k_trace<false>, tg, blk, 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
 				if (timing && k == 0) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv) + 1], st), "event");
 #define UPD(F, FS, LO) hipLaunchKernelGGL((k_update<F, FS, LO>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.converge, d.bcnt.p, d.nblocks)
 				if (multi) { if (d.fma) UPD(true, false, false); else UPD(false, false, false); }
