@@ -164,12 +164,12 @@ rad_status rad_upload(rad_ctx *c)
 {
 	if (!c) return RAD_EINVAL;
 	if (!c->prepared) return fail(c, RAD_ESTATE, "rad_prepare_host first");
-	const auto t0 = Clock::now();
-	if (!c->dev) {
+	if (!c->dev) {  // HIP start-up and the context's stream: not part of the upload time
 		DeviceEngine *d = DeviceEngine::create(c->err);
 		if (!d) return RAD_EDEVICE;
 		c->dev.reset(d);
 	}
+	const auto t0 = Clock::now();
 	c->dev->timing = c->scene.o.kernel_timing != 0;
 	if (c->dev->upload(c->scene, c->tree, c->err)) return RAD_EDEVICE;
 	c->ms_upload = ms_since(t0);

@@ -58,6 +58,7 @@ constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
 constexpr int kGroups = 2;        // receiver groups per block
+constexpr int kMinQueue = 8;      // iterations queued per host check at least (up to -batch)
 constexpr int kRecvPerBlock = kGroups * kWavesPerBlock * kRecvPerWave;
 constexpr int kMaxSrc = 256;
 constexpr int kMaxWalk = 1 << 24;
@@ -1188,8 +1189,10 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 		// A candidate's unshot energy only grows until it is shot, so the
 		// candidate count bounds the shots left from below: queue no more
 		// iterations than that, and none run idle after convergence.
+		// Below kMinQueue the bound is not used: a host round trip costs more
+		// than the few launches that may find nothing left to do.
 		const int per_iter = multi ? d.parallel_src : 1;
-		const int left = std::max(1, (d.host_loop->ncand + per_iter - 1) / per_iter);
+		const int left = std::max(kMinQueue, (d.host_loop->ncand + per_iter - 1) / per_iter);
 		const int b = std::min(std::min(d.batch, left), max_shots - queued);
 		HIP_OK(hipEventRecord(d.ev0, st), "event");
 		for (int i = 0; i < b; i++) {

@@ -107,7 +107,8 @@ rad_status rad_ff_pairs(rad_ctx *ctx, int n, const int *pairs, float *ff3);
 /* Device counters since rad_upload (RAD_CNT_*). */
 rad_status rad_counters(rad_ctx *ctx, long long *out);
 
-/* Timings: out[0] host setup ms (load to prepare), out[1] upload ms,
+/* Timings: out[0] host setup ms (load to prepare), out[1] upload ms (HIP
+ * start-up excluded),
  * out[2] device ms of the shooting loop, out[3] shooting iterations
  * launched, out[4] device ms of the ray-tracing kernel alone and out[5] the
  * number of such kernels timed (both 0 unless -kernel_timing 1), out[6]
