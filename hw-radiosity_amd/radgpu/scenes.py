@@ -187,7 +187,8 @@ def patch_room(seed: int = 1, n: int = 32768) -> Scene:
     triangle count 2 * cells stays <= n; then the largest triangle (ties: the
     lowest index) is bisected at the midpoint of its longest edge until there
     are n.  Ceiling cells whose centre lies in room_scene's light rectangle
-    (x in [3.75, 6.25], y in [40/13, 64/13]) emit Brgb 255 255 255."""
+    (x in [3.75, 6.25], y in [40/13, 64/13]) emit Brgb 255 255 255 (at very
+    coarse grids, n below a few hundred, no centre may fall inside it)."""
     if n < 64:
         raise ValueError("patch_room needs n >= 64")
     rng = SplitMix64(seed)
