@@ -313,3 +313,18 @@ def test_batch_scenes_stop_at_tolerance():
     B2, r2, info2 = b.solve(rho, E, prec="f16", graph=True, tol=1e-14, outer=2, sweeps=2)
     assert info2["above_tol"] == 2 and np.all(r2 > 1e-14)
     b.close()
+
+
+def test_benchmark_dense_reference_matches_cpu_lu(sys2k, tmp_path):
+    """radgpu.dense_ref (torch LU on the GPU, the benchmark's measuring
+    stick at 32768 patches) agrees with the oracle's CPU LU."""
+    import subprocess
+    eng, lin, sc, E, rho, area, F = sys2k
+    np.save(tmp_path / "F.npy", F)
+    np.save(tmp_path / "rho.npy", rho)
+    np.save(tmp_path / "E.npy", E)
+    env = dict(os.environ, PYTHONPATH=os.path.join(ROOT, "hw-radiosity_amd"))
+    r = subprocess.run([sys.executable, "-m", "radgpu.dense_ref", str(tmp_path)], env=env, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert rel(np.load(tmp_path / "B.npy"), lo.dense_solve(F, rho, E)) < 1e-12
