@@ -1,9 +1,8 @@
 // Device kernels and buffers of the shooting loop (gfx950).
 //
 // Layout in HBM
-//   octree: cell_rec[2 * cell], cell in [0, gt^3) (TreeNodeNum tabulated):
-//       the leaf's record {g0.xyz, id}, {g1.xyz, count | depth<<24}, id = its
-//       first list entry (-(leaf+1) when empty)
+//   octree: cell_leaf[gt^3] (TreeNodeNum tabulated), leaf_box[2*l] =
+//       {g0.xyz, first item}, {g1.xyz, count | depth<<24}
 //   leaf lists, leaf-major with the triangle inline (the list order is the
 //       reference's ascending original index): lnc[2*k] = {n.xyz, c.x},
 //       lnc[2*k+1] = {c.y, c.z, original id, -}, lse[3*k+s] = side plane s.
@@ -71,7 +70,8 @@ constexpr float kGuardDown = 1.0f - 0x1p-19f;
 struct DScene {
 	const float4 *__restrict__ lnc;
 	const float4 *__restrict__ lse;
-	const float4 *__restrict__ cell_rec;  // 2 per lookup cell: its leaf's {g0, id}, {g1, count|depth<<24}
+	const int *__restrict__ cell_leaf;
+	const float4 *__restrict__ leaf_box;
 	float g0x, g0y, g0z, g1x, g1y, g1z;
 	double mgl, inv_mgl;
 	int gt;
@@ -178,46 +178,25 @@ __device__ __forceinline__ int cell_coord(float q, float g0, const DScene &S)
 // raycast.cpp:283-333) and return the nearest hit of the first leaf with any
 // hit; else step q to just past the leaf's nearest exit plane (NextPoint,
 // raycast.cpp:235-251).
-//
-// Each lookup cell holds its leaf's record itself ({g0, id}, {g1,
-// count|depth<<24}; id = the leaf's first list entry, or -(leaf+1) for an
-// empty leaf, so ids are unique), so a lookup is one 32-byte load.  The exit
-// point depends only on the ray and the leaf's box, so the next cell's record
-// is requested before the current leaf's triangles are tested: its latency
-// hides behind the tests.
-constexpr int kNoLeaf = (int)0x80000000;
-enum { LOC_OK = 0, LOC_OUT = 1, LOC_STALL = 2 };
-
-__device__ __forceinline__ int locate(const DScene &S, float qx, float qy, float qz, float4 &r0, float4 &r1)
-{
-	if (qx < S.g0x || qx > S.g1x || qy < S.g0y || qy > S.g1y || qz < S.g0z || qz > S.g1z) return LOC_OUT;
-	const int gx = cell_coord(qx, S.g0x, S), gy = cell_coord(qy, S.g0y, S), gz = cell_coord(qz, S.g0z, S);
-	// outside the tabulated descent: cannot happen for q inside [G0, G1]
-	if ((unsigned)gx >= (unsigned)S.gt || (unsigned)gy >= (unsigned)S.gt || (unsigned)gz >= (unsigned)S.gt)
-		return LOC_STALL;
-	const size_t cell = ((size_t)gx * S.gt + gy) * S.gt + gz;
-	r0 = S.cell_rec[2 * cell];
-	r1 = S.cell_rec[2 * cell + 1];
-	return LOC_OK;
-}
-
 template <bool F>
 __device__ int walk(const DScene &S, float px, float py, float pz, float vx, float vy, float vz, int ban, Cnt &c)
 {
 	float t = 0.f, qx = px, qy = py, qz = pz;
-	int prev = kNoLeaf;
+	int prev = -1;
 	// Brent's cycle detection over the leaves searched: the walk's next leaf
 	// is a function of the leaf it leaves (the hit test and the exit point
 	// use only the ray and that leaf's box), so a leaf searched twice starts
 	// a cycle the reference never leaves.
-	int tort = kNoLeaf, power = 1, lam = 0;
+	int tort = -1, power = 1, lam = 0;
 	const bool use_x = fabsf(vx) > S.thr_v, use_y = fabsf(vy) > S.thr_v, use_z = fabsf(vz) > S.thr_v;
-	float4 b0, b1;
-	int loc = locate(S, qx, qy, qz, b0, b1);
 	for (int steps = 0;;) {
-		if (loc == LOC_OUT) return -1;
-		if (loc == LOC_STALL) { c.stalled++; return -2; }
-		const int leaf = __float_as_int(b0.w);
+		if (qx < S.g0x || qx > S.g1x || qy < S.g0y || qy > S.g1y || qz < S.g0z || qz > S.g1z) return -1;
+		const int gx = cell_coord(qx, S.g0x, S), gy = cell_coord(qy, S.g0y, S), gz = cell_coord(qz, S.g0z, S);
+		if ((unsigned)gx >= (unsigned)S.gt || (unsigned)gy >= (unsigned)S.gt || (unsigned)gz >= (unsigned)S.gt) {
+			c.stalled++;  // outside the tabulated descent: cannot happen for q inside [G0, G1]
+			return -2;
+		}
+		const int leaf = S.cell_leaf[(gx * S.gt + gy) * S.gt + gz];
 		if (++steps > kMaxWalk) { c.stalled++; return -2; }
 		if (leaf == prev) {
 			c.precision++;
@@ -227,34 +206,15 @@ __device__ int walk(const DScene &S, float px, float py, float pz, float vx, flo
 			qx = axpy<F>(t, vx, px);
 			qy = axpy<F>(t, vy, py);
 			qz = axpy<F>(t, vz, pz);
-			loc = locate(S, qx, qy, qz, b0, b1);
 			continue;
 		}
-		const int cd = __float_as_int(b1.w), cnt = cd & 0xffffff;
+		const float4 b0 = S.leaf_box[2 * leaf], b1 = S.leaf_box[2 * leaf + 1];
+		const int begin = __float_as_int(b0.w), cd = __float_as_int(b1.w), cnt = cd & 0xffffff;
 		c.cells++;
 		c.levels += cd >> 24;
-		// NextPoint first (independent of the tests), and the next record
-		float tm = 1000000.0f;
-		if (use_x) {
-			const float s = vx > 0 ? (b1.x - px) / vx : (b0.x - px) / vx;
-			tm = s < tm ? s : tm;
-		}
-		if (use_y) {
-			const float s = vy > 0 ? (b1.y - py) / vy : (b0.y - py) / vy;
-			tm = s < tm ? s : tm;
-		}
-		if (use_z) {
-			const float s = vz > 0 ? (b1.z - pz) / vz : (b0.z - pz) / vz;
-			tm = s < tm ? s : tm;
-		}
-		tm = (float)((double)tm + 0.1);
-		const float nx = axpy<F>(tm, vx, px), ny = axpy<F>(tm, vy, py), nz = axpy<F>(tm, vz, pz);
-		float4 nb0, nb1;
-		const int nloc = locate(S, nx, ny, nz, nb0, nb1);
-		// the leaf's list
 		float tmin = 1000000.0f, tmin_hi = 1000000.0f * kGuardUp;
 		int best = -1;
-		const float4 *nc = S.lnc + 2 * (size_t)(cnt > 0 ? leaf : 0);
+		const float4 *nc = S.lnc + 2 * (size_t)begin;
 		float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
 		if (cnt > 0) {
 			a = nc[0];
@@ -282,15 +242,18 @@ __device__ int walk(const DScene &S, float px, float py, float pz, float vx, flo
 					// TriHitted: t < 0.0001 (in double) or t >= t_min rejects
 					if (!(tt < S.thr_t) && !(tt >= tmin)) {
 						const float hx = axpy<F>(tt, vx, px), hy = axpy<F>(tt, vy, py), hz = axpy<F>(tt, vz, pz);
-						// the three side planes in one round trip, tested in order
-						const float4 *se = S.lse + 3 * ((size_t)leaf + e);
-						const float4 s0 = se[0], s1 = se[1], s2 = se[2];
-						if (!(dot3<F>(hx, hy, hz, s0.x, s0.y, s0.z) + s0.w < 0.0f) &&
-						    !(dot3<F>(hx, hy, hz, s1.x, s1.y, s1.z) + s1.w < 0.0f) &&
-						    !(dot3<F>(hx, hy, hz, s2.x, s2.y, s2.z) + s2.w < 0.0f)) {
-							tmin = tt;
-							tmin_hi = tt * kGuardUp;
-							best = id;
+						const float4 *se = S.lse + 3 * ((size_t)begin + e);
+						const float4 s0 = se[0];
+						if (!(dot3<F>(hx, hy, hz, s0.x, s0.y, s0.z) + s0.w < 0.0f)) {
+							const float4 s1 = se[1];
+							if (!(dot3<F>(hx, hy, hz, s1.x, s1.y, s1.z) + s1.w < 0.0f)) {
+								const float4 s2 = se[2];
+								if (!(dot3<F>(hx, hy, hz, s2.x, s2.y, s2.z) + s2.w < 0.0f)) {
+									tmin = tt;
+									tmin_hi = tt * kGuardUp;
+									best = id;
+								}
+							}
 						}
 					}
 				}
@@ -302,13 +265,24 @@ __device__ int walk(const DScene &S, float px, float py, float pz, float vx, flo
 		if (leaf == tort) { c.stalled++; return -2; }
 		if (++lam == power) { tort = leaf; power <<= 1; lam = 0; }
 		prev = leaf;
+		float tm = 1000000.0f;
+		if (use_x) {
+			const float s = vx > 0 ? (b1.x - px) / vx : (b0.x - px) / vx;
+			tm = s < tm ? s : tm;
+		}
+		if (use_y) {
+			const float s = vy > 0 ? (b1.y - py) / vy : (b0.y - py) / vy;
+			tm = s < tm ? s : tm;
+		}
+		if (use_z) {
+			const float s = vz > 0 ? (b1.z - pz) / vz : (b0.z - pz) / vz;
+			tm = s < tm ? s : tm;
+		}
+		tm = (float)((double)tm + 0.1);
+		qx = axpy<F>(tm, vx, px);
+		qy = axpy<F>(tm, vy, py);
+		qz = axpy<F>(tm, vz, pz);
 		t = tm;
-		qx = nx;
-		qy = ny;
-		qz = nz;
-		loc = nloc;
-		b0 = nb0;
-		b1 = nb1;
 	}
 }
 
@@ -1011,7 +985,8 @@ float largest_float_at_most(double x)
 
 struct DeviceEngine::Impl {
 	hipStream_t stream = nullptr;
-	DevBuf<float4> lnc, lse, cell_rec, nrm, cA, rho, acc, dacc, dB;
+	DevBuf<float4> lnc, lse, leaf_box, nrm, cA, rho, acc, dacc, dB;
+	DevBuf<int> cell_leaf;
 	DevBuf<float> vx, vy, vz;
 	DevBuf<int> fine_leaf, seg_begin;  // segment-visibility index (explicit matrix)
 	DevBuf<float4> seg_nc, seg_se;
@@ -1120,25 +1095,11 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 			return -1;
 		}
 	}
-	// each lookup cell carries its leaf's record; the leaf id is the first
-	// list entry, or -(leaf + 1) for an empty leaf (ids stay unique)
-	std::vector<float4> cell_rec(t.cell_leaf.size() * 2);
-	for (size_t cidx = 0; cidx < t.cell_leaf.size(); cidx++) {
-		const int leaf = t.cell_leaf[cidx];
+	for (int leaf : t.cell_leaf)
 		if (leaf < 0 || leaf >= t.n_leaves) {
 			err = "octree cell table entry out of range";
 			return -1;
 		}
-		const float *b = &t.leaf_box[(size_t)leaf * 8];
-		int begin, cd;
-		std::memcpy(&begin, &b[3], 4);
-		std::memcpy(&cd, &b[7], 4);
-		const int id = (cd & 0xffffff) > 0 ? begin : -(leaf + 1);
-		float idf;
-		std::memcpy(&idf, &id, 4);
-		cell_rec[2 * cidx] = make_float4(b[0], b[1], b[2], idf);
-		cell_rec[2 * cidx + 1] = make_float4(b[4], b[5], b[6], b[7]);
-	}
 	std::vector<float> vx((size_t)n * 3), vy((size_t)n * 3), vz((size_t)n * 3);
 	std::vector<float4> nrm(n), cA(n), rho(n), acc((size_t)n * 3), dacc((size_t)n * 3), dB(n);
 	for (int i = 0; i < n; i++) {
@@ -1160,7 +1121,8 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	}
 	hipStream_t st = d.stream;
 	if (d.lnc.put(lnc.data(), lnc.size(), st, err) || d.lse.put(lse.data(), lse.size(), st, err) ||
-	    d.cell_rec.put(cell_rec.data(), cell_rec.size(), st, err) ||
+	    d.cell_leaf.put(t.cell_leaf.data(), t.cell_leaf.size(), st, err) ||
+	    d.leaf_box.put((const float4 *)t.leaf_box.data(), t.leaf_box.size() / 4, st, err) ||
 	    d.vx.put(vx.data(), vx.size(), st, err) || d.vy.put(vy.data(), vy.size(), st, err) ||
 	    d.vz.put(vz.data(), vz.size(), st, err) || d.nrm.put(nrm.data(), nrm.size(), st, err) ||
 	    d.cA.put(cA.data(), cA.size(), st, err) || d.rho.put(rho.data(), rho.size(), st, err) ||
@@ -1182,7 +1144,7 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	std::memset(d.host_loop, 0, sizeof(Loop));
 	if (d.loop.put(d.host_loop, 1, st, err)) return -1;
 	const float thr_t = smallest_float_at_least(0.0001);
-	d.S = DScene{d.lnc.p, d.lse.p, d.cell_rec.p, s.G0[0], s.G0[1], s.G0[2], s.G1[0], s.G1[1], s.G1[2],
+	d.S = DScene{d.lnc.p, d.lse.p, d.cell_leaf.p, d.leaf_box.p, s.G0[0], s.G0[1], s.G0[2], s.G1[0], s.G1[1], s.G1[2],
 	             (double)t.min_grid, 1.0 / (double)t.min_grid, t.gt, thr_t, thr_t * kGuardDown,
 	             largest_float_at_most(1e-6)};
 	d.R = DRecv{d.vx.p, d.vy.p, d.vz.p, d.nrm.p, d.cA.p, d.rho.p, d.acc.p, d.dacc.p, d.dB.p, n};

@@ -123,61 +123,21 @@ __device__ __forceinline__ void gemm_tile(const E *__restrict__ A, const E *__re
 			}
 }
 
-template <int RT, int CT> struct Fx64Regs {
-	float a[RT][8];
-	double b[CT][8];
-};
-
-template <int RT, int CT>
-__device__ __forceinline__ void fx64_load(Fx64Regs<RT, CT> &g, const float *__restrict__ F,
-                                          const double *__restrict__ xT, int ld, int rb0, int cb0, int kk, int lr, int g8)
-{
-	typedef float fv4 __attribute__((ext_vector_type(4)));
-#pragma unroll
-	for (int r = 0; r < RT; r++) {
-		const fv4 *pa = (const fv4 *)(F + (size_t)((rb0 + r) * 16 + lr) * ld + kk + g8);
-		const fv4 u = __builtin_nontemporal_load(pa), v = __builtin_nontemporal_load(pa + 1);
-		g.a[r][0] = u[0]; g.a[r][1] = u[1]; g.a[r][2] = u[2]; g.a[r][3] = u[3];
-		g.a[r][4] = v[0]; g.a[r][5] = v[1]; g.a[r][6] = v[2]; g.a[r][7] = v[3];
-	}
-#pragma unroll
-	for (int c = 0; c < CT; c++) {
-		const double2 *pb = (const double2 *)(xT + (size_t)((cb0 + c) * 16 + lr) * ld + kk + g8);
-#pragma unroll
-		for (int q = 0; q < 4; q++) {
-			const double2 t = pb[q];
-			g.b[c][2 * q] = t.x;
-			g.b[c][2 * q + 1] = t.y;
-		}
-	}
-}
-
-template <int RT, int CT>
-__device__ __forceinline__ void fx64_mma(const Fx64Regs<RT, CT> &g, double __attribute__((ext_vector_type(4))) (&acc)[RT][CT])
-{
-#pragma unroll
-	for (int st = 0; st < 8; st++)
-#pragma unroll
-		for (int r = 0; r < RT; r++)
-#pragma unroll
-			for (int c = 0; c < CT; c++)
-				acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)g.a[r][st], g.b[c][st], acc[r][c], 0, 0, 0);
-}
-
 // One wave's share of the FP64 product with an FP32 matrix on the FP64 MFMA
 // (v_mfma_f64_16x16x4_f64: lane l supplies A[l%16][l/16] and B[l/16][l%16],
 // receives D[l/16 + 4*j][l%16], j = 0..3 -- rows interleaved, unlike the
-// 16-bit MFMAs): rows 16*rb0 .. 16*(rb0+RT)-1, columns 16*cb0 ..
-// 16*(cb0+CT)-1, k in [k0, k1) (a multiple of 32).  Per 32 k each lane loads
-// 8 consecutive F entries (converted exactly to FP64) and 8 consecutive X
-// entries, then runs 8 MFMAs, step s pairing k = kk + 8*(l/16) + s in both
-// operands, so every k is summed once.  Two register sets: the next 32 k load
-// while the current feed the MFMAs.  part is row-major, rp columns.
+// 16-bit MFMAs): rows 16*rb0 .. 16*(rb0+RT)-1, columns
+// 16*cb0 .. 16*(cb0+CT)-1, k in [k0, k1) (a multiple of 32).  Per 32 k each
+// lane loads 8 consecutive F entries (converted exactly to FP64) and 8
+// consecutive X entries, then runs 8 MFMAs, step s pairing k = kk + 8*(l/16)
+// + s in both operands, so every k is summed once.  part is row-major, rp
+// columns.
 template <int RT, int CT>
 __device__ __forceinline__ void fx64_tile(const float *__restrict__ F, const double *__restrict__ xT,
                                           double *__restrict__ part, int ld, int rp, int rb0, int cb0, int k0, int k1)
 {
 	typedef double d4 __attribute__((ext_vector_type(4)));
+	typedef float fv4 __attribute__((ext_vector_type(4)));
 	const int lane = threadIdx.x & 63;
 	const int lr = lane & 15, g8 = (lane >> 4) * 8;
 	d4 acc[RT][CT];
@@ -185,14 +145,33 @@ __device__ __forceinline__ void fx64_tile(const float *__restrict__ F, const dou
 	for (int r = 0; r < RT; r++)
 #pragma unroll
 		for (int c = 0; c < CT; c++) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
-	Fx64Regs<RT, CT> g0, g1;
-	fx64_load(g0, F, xT, ld, rb0, cb0, k0, lr, g8);
-	for (int kk = k0; kk < k1; kk += 64) {
-		const bool has1 = kk + 32 < k1;
-		if (has1) fx64_load(g1, F, xT, ld, rb0, cb0, kk + 32, lr, g8);
-		fx64_mma(g0, acc);
-		if (kk + 64 < k1) fx64_load(g0, F, xT, ld, rb0, cb0, kk + 64, lr, g8);
-		if (has1) fx64_mma(g1, acc);
+	for (int kk = k0; kk < k1; kk += 32) {
+		float a[RT][8];
+		double b[CT][8];
+#pragma unroll
+		for (int r = 0; r < RT; r++) {
+			const fv4 *pa = (const fv4 *)(F + (size_t)((rb0 + r) * 16 + lr) * ld + kk + g8);
+			const fv4 u = __builtin_nontemporal_load(pa), v = __builtin_nontemporal_load(pa + 1);
+			a[r][0] = u[0]; a[r][1] = u[1]; a[r][2] = u[2]; a[r][3] = u[3];
+			a[r][4] = v[0]; a[r][5] = v[1]; a[r][6] = v[2]; a[r][7] = v[3];
+		}
+#pragma unroll
+		for (int c = 0; c < CT; c++) {
+			const double2 *pb = (const double2 *)(xT + (size_t)((cb0 + c) * 16 + lr) * ld + kk + g8);
+#pragma unroll
+			for (int q = 0; q < 4; q++) {
+				const double2 t = pb[q];
+				b[c][2 * q] = t.x;
+				b[c][2 * q + 1] = t.y;
+			}
+		}
+#pragma unroll
+		for (int st = 0; st < 8; st++)
+#pragma unroll
+			for (int r = 0; r < RT; r++)
+#pragma unroll
+				for (int c = 0; c < CT; c++)
+					acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[r][st], b[c][st], acc[r][c], 0, 0, 0);
 	}
 #pragma unroll
 	for (int r = 0; r < RT; r++)
