@@ -1192,7 +1192,10 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 		// Below kMinQueue the bound is not used: a host round trip costs more
 		// than the few launches that may find nothing left to do.
 		const int per_iter = multi ? d.parallel_src : 1;
-		const int left = std::max(kMinQueue, (d.host_loop->ncand + per_iter - 1) / per_iter);
+		// No candidate left: the next iteration is the one that finds none.
+		const int left = d.host_loop->ncand == 0 && queued > 0
+		                     ? 1
+		                     : std::max(kMinQueue, (d.host_loop->ncand + per_iter - 1) / per_iter);
 		const int b = std::min(std::min(d.batch, left), max_shots - queued);
 		HIP_OK(hipEventRecord(d.ev0, st), "event");
 		for (int i = 0; i < b; i++) {
