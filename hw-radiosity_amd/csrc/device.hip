@@ -853,12 +853,81 @@ __device__ bool seg_occluded(const DFine &G, float px, float py, float pz, float
 	}
 }
 
-// Row i (one block): F[i][j] = CalFF's value from the centroid of i to the
-// whole patch j (shade.cpp:107-123) when the segment between the centroids is
-// clear; 0 on the diagonal and for logical slots; the row scaled by
-// (float)(1 / sum) when its double sum (fixed reduction order) exceeds 1.
+// Visibility of every unordered pair, once: tile (ti <= tj) of 64 x 64 pairs
+// per block; wave w takes rows ti*64 + 16w .. +15, lane l column tj*64 + l,
+// so a wave's 64 rays at a time leave one centroid for 64 neighbouring ones.
+// The segment runs from the lower slot index to the higher; pairs that do
+// not face each other (CalFF's sign test, which is symmetric) are left 0.
+// vis[i * nt + j/64] bit j%64 = 1 when i and j see each other; the block
+// writes its tile and, transposed through LDS, the mirror tile.
+__device__ __forceinline__ int tri_tile(int t, int nt, int &tj)
+{
+	// t -> (ti, tj) over ti <= tj, row by row: row r starts at
+	// o(r) = r (2 nt - r + 1) / 2; the root of o(r) = t, then exact fix-up
+	const double b = 2.0 * nt + 1.0;
+	int ti = (int)((b - __builtin_sqrt(b * b - 8.0 * t)) * 0.5);
+	auto o = [nt](long long r) { return r * (2LL * nt - r + 1) / 2; };
+	while (ti > 0 && o(ti) > t) ti--;
+	while (ti + 1 < nt && o(ti + 1) <= t) ti++;
+	tj = ti + (int)(t - o(ti));
+	return ti;
+}
+
 template <bool F>
-__global__ __launch_bounds__(kBlock) void k_ff_seg(DFine G, DRecv R, int n, int ld, float *Fm, long long *bcnt)
+__global__ __launch_bounds__(kBlock) void k_vis_tiles(DFine G, DRecv R, int n, int nt,
+                                                      unsigned long long *vis, long long *bcnt)
+{
+	__shared__ unsigned long long rowbits[64];
+	int tj;
+	const int ti = tri_tile(blockIdx.x, nt, tj);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const int j = tj * 64 + lane;
+	Cnt c = {0, 0, 0, 0, 0, 0, 0};
+	float4 nj = make_float4(0.f, 0.f, 0.f, 0.f), cj = nj;
+	int pj = -1;
+	if (j < n) {
+		nj = R.nrm[j];
+		cj = R.cA[j];
+		pj = __float_as_int(nj.w);
+	}
+	for (int q = 0; q < 16; q++) {
+		const int i = ti * 64 + w * 16 + q;  // wave-uniform
+		bool see = false;
+		if (i < n && j < n && i < j && pj >= 0) {
+			const float4 ni = R.nrm[i], ci = R.cA[i];
+			const int pi = __float_as_int(ni.w);
+			const float dx = cj.x - ci.x, dy = cj.y - ci.y, dz = cj.z - ci.z;
+			const float ds = dot3<F>(dx, dy, dz, nj.x, nj.y, nj.z);
+			const float dd = -dot3<F>(dx, dy, dz, ni.x, ni.y, ni.z);
+			if (pi >= 0 && (ds < 0) == (dd < 0) && ds != 0.0f && dd != 0.0f)
+				see = !seg_occluded<F>(G, ci.x, ci.y, ci.z, dx, dy, dz, pi, pj, c);
+		}
+		const unsigned long long bits = __ballot(see);
+		if (lane == 0) rowbits[w * 16 + q] = bits;
+	}
+	__syncthreads();
+	if (threadIdx.x < 64) {
+		const int r = threadIdx.x;
+		if (ti * 64 + r < n) vis[(size_t)(ti * 64 + r) * nt + tj] = rowbits[r];
+		// mirror: row tj*64 + r, bits over the tile's rows
+		unsigned long long m = 0;
+		for (int q = 0; q < 64; q++) m |= ((rowbits[q] >> r) & 1ull) << q;
+		if (tj * 64 + r < n) {
+			if (ti == tj) m |= rowbits[r];  // diagonal tile: both halves in one word
+			vis[(size_t)(tj * 64 + r) * nt + ti] = m;
+		}
+	}
+	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
+	add_block_counters(vals, bcnt);
+}
+
+// Row i (one block): F[i][j] = CalFF's value from the centroid of i to the
+// whole patch j (shade.cpp:107-123) when i and j see each other (vis); 0 on
+// the diagonal and for logical slots; the row scaled by (float)(1 / sum)
+// when its double sum (fixed reduction order) exceeds 1.
+template <bool F>
+__global__ __launch_bounds__(kBlock) void k_ff_seg(DRecv R, int n, int ld, int nt, const unsigned long long *vis,
+                                                   float *Fm)
 {
 	__shared__ double wsum[kWavesPerBlock];
 	const int i = blockIdx.x;
@@ -867,21 +936,19 @@ __global__ __launch_bounds__(kBlock) void k_ff_seg(DFine G, DRecv R, int n, int 
 	const int pi = __float_as_int(ni.w);
 	const float nlen_i = __builtin_sqrtf(dot3<F>(ni.x, ni.y, ni.z, ni.x, ni.y, ni.z));
 	float *row = Fm + (size_t)i * ld;
-	Cnt c = {0, 0, 0, 0, 0, 0, 0};
+	const unsigned long long *vrow = vis + (size_t)i * nt;
 	double sum = 0.0;
 	for (int j = threadIdx.x; j < n; j += kBlock) {
 		float ff = 0.0f;
 		const float4 nj = R.nrm[j], cj = R.cA[j];
 		const int pj = __float_as_int(nj.w);
-		if (j != i && pi >= 0 && pj >= 0) {
+		if (j != i && pi >= 0 && pj >= 0 && ((vrow[j >> 6] >> (j & 63)) & 1ull)) {
 			SrcPatch sp;
 			sp.c[0] = cj.x; sp.c[1] = cj.y; sp.c[2] = cj.z;
 			sp.n[0] = nj.x; sp.n[1] = nj.y; sp.n[2] = nj.z;
 			sp.area = cj.w;
 			sp.nlen = __builtin_sqrtf(dot3<F>(nj.x, nj.y, nj.z, nj.x, nj.y, nj.z));
 			ff = ff_prefix<F>(sp, ni.x, ni.y, ni.z, nlen_i, ci.x, ci.y, ci.z);
-			if (ff != 0.0f && seg_occluded<F>(G, ci.x, ci.y, ci.z, cj.x - ci.x, cj.y - ci.y, cj.z - ci.z, pi, pj, c))
-				ff = 0.0f;
 		}
 		row[j] = ff;
 		sum += (double)ff;
@@ -894,8 +961,6 @@ __global__ __launch_bounds__(kBlock) void k_ff_seg(DFine G, DRecv R, int n, int 
 		const float scale = (float)(1.0 / total);
 		for (int j = threadIdx.x; j < n; j += kBlock) row[j] *= scale;
 	}
-	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
-	add_block_counters(vals, bcnt);
 }
 
 template <bool F>
@@ -989,6 +1054,7 @@ struct DeviceEngine::Impl {
 	DevBuf<int> cell_leaf;
 	DevBuf<float> vx, vy, vz;
 	DevBuf<int> fine_leaf, seg_begin;  // segment-visibility index (explicit matrix)
+	DevBuf<unsigned long long> vis;    // pair visibility bits of the explicit matrix
 	DevBuf<float4> seg_nc, seg_se;
 	bool have_seg = false;
 	DevBuf<Loop> loop;
@@ -1365,8 +1431,9 @@ int DeviceEngine::form_factors(float *dF, int n, int ld, int vis, FlatOctree &t,
 		HIP_OK(hipStreamSynchronize(st), "segment index upload");
 		d.have_seg = true;
 	}
-	if (d.bcnt.n < (size_t)n * 8) {
-		std::vector<long long> zeros((size_t)n * 8, 0);
+	const size_t nblk_ff = std::max<size_t>((size_t)n, (size_t)((n + 63) / 64) * ((n + 63) / 64 + 1) / 2);
+	if (d.bcnt.n < nblk_ff * 8) {  // the largest grid of the two kernels indexes bcnt
+		std::vector<long long> zeros(nblk_ff * 8, 0);
 		if (d.bcnt.put(zeros.data(), zeros.size(), st, err)) return -1;
 		HIP_OK(hipStreamSynchronize(st), "counter upload");
 	}
@@ -1383,8 +1450,13 @@ int DeviceEngine::form_factors(float *dF, int n, int ld, int vis, FlatOctree &t,
 			G.h[k] = (s.G1[k] - s.G0[k]) / (float)t.fine_r;
 			G.inv[k] = (float)t.fine_r / (s.G1[k] - s.G0[k]);
 		}
-		if (d.fma) hipLaunchKernelGGL(k_ff_seg<true>, dim3(n), dim3(kBlock), 0, st, G, d.R, n, ld, dF, d.bcnt.p);
-		else hipLaunchKernelGGL(k_ff_seg<false>, dim3(n), dim3(kBlock), 0, st, G, d.R, n, ld, dF, d.bcnt.p);
+		const int nt = (n + 63) / 64;
+		const long long tiles = (long long)nt * (nt + 1) / 2;
+		if (d.vis.alloc((size_t)n * nt, err)) return -1;
+		if (d.fma) hipLaunchKernelGGL(k_vis_tiles<true>, dim3((unsigned)tiles), dim3(kBlock), 0, st, G, d.R, n, nt, d.vis.p, d.bcnt.p);
+		else hipLaunchKernelGGL(k_vis_tiles<false>, dim3((unsigned)tiles), dim3(kBlock), 0, st, G, d.R, n, nt, d.vis.p, d.bcnt.p);
+		if (d.fma) hipLaunchKernelGGL(k_ff_seg<true>, dim3(n), dim3(kBlock), 0, st, d.R, n, ld, nt, d.vis.p, dF);
+		else hipLaunchKernelGGL(k_ff_seg<false>, dim3(n), dim3(kBlock), 0, st, d.R, n, ld, nt, d.vis.p, dF);
 	} else {
 		if (d.fma) hipLaunchKernelGGL(k_ff_rows<true>, dim3(n), dim3(kBlock), 0, st, d.S, d.R, n, ld, dF, d.bcnt.p);
 		else hipLaunchKernelGGL(k_ff_rows<false>, dim3(n), dim3(kBlock), 0, st, d.S, d.R, n, ld, dF, d.bcnt.p);

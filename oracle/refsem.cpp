@@ -1221,13 +1221,16 @@ static float centroid_ff(rs_ctx *c, int i, int j, int vis, long long *k)
 		                : c->cal_ff<false>(w, s.parent, d.n, d.parent, d.c, k);
 	const float ff = c->o.fma ? c->ff_value<true>(w, d.n, d.c) : c->ff_value<false>(w, d.n, d.c);
 	if (ff == 0.0f || vis == RS_VIS_NONE) return ff;
-	const float v[3] = {s.c[0] - d.c[0], s.c[1] - d.c[1], s.c[2] - d.c[2]};
+	// one test per unordered pair: the segment runs from the lower slot
+	// index to the higher, so visibility is symmetric
+	const Rec &lo = i < j ? d : s, &hi = i < j ? s : d;
+	const float v[3] = {hi.c[0] - lo.c[0], hi.c[1] - lo.c[1], hi.c[2] - lo.c[2]};
 	bool occ;
 	if (vis == RS_VIS_BRUTE)
-		occ = c->o.fma ? c->seg_brute<true>(d.c, v, d.parent, s.parent) : c->seg_brute<false>(d.c, v, d.parent, s.parent);
+		occ = c->o.fma ? c->seg_brute<true>(lo.c, v, lo.parent, hi.parent) : c->seg_brute<false>(lo.c, v, lo.parent, hi.parent);
 	else
-		occ = c->o.fma ? c->seg_occluded<true>(d.c, v, d.parent, s.parent, k)
-		               : c->seg_occluded<false>(d.c, v, d.parent, s.parent, k);
+		occ = c->o.fma ? c->seg_occluded<true>(lo.c, v, lo.parent, hi.parent, k)
+		               : c->seg_occluded<false>(lo.c, v, lo.parent, hi.parent, k);
 	return occ ? 0.0f : ff;
 }
 

@@ -81,7 +81,8 @@ int rs_ff_pairs(rs_ctx *, int n, const int *pairs, float *out3);
 /* Explicit gathering matrix: F[i][j] = CalFF's value from the centroid of i
  * to the whole patch j, times its visibility:
  *   RS_VIS_SEGMENT    no other original crosses the open segment between the
- *                     centroids, t in (1e-4, 1 - 1e-4) (grid walk with
+ *                     centroids, t in (1e-4, 1 - 1e-4), the segment taken from
+ *                     the lower slot index to the higher (grid walk with
  *                     geometric point location; not a reference function)
  *   RS_VIS_REFERENCE  RayHitted from the centroid returns j's logical index
  *   RS_VIS_NONE       always visible (checks)

@@ -210,9 +210,12 @@ def test_centroid_rows_reciprocity_and_clamp(c1_small):
     assert far.sum() > 1000
     rel = np.abs(lhs - lhs.T)[far] / lhs[far]
     assert np.median(rel) < 0.05
-    # segment visibility is symmetric
-    vis = raw > 0
-    assert (vis != vis.T).mean() < 1e-4
+    # segment visibility is symmetric: wherever both directions have a
+    # non-zero unoccluded value, both are visible or both are blocked
+    none = o.ff_centroid(_all_pairs(n), ol.VIS_NONE).reshape(n, n)
+    both = (none > 0) & (none.T > 0)
+    assert both.sum() > 1000
+    assert np.array_equal((raw > 0) & both, (raw.T > 0) & both)
     # the reference's RayHitted from a centroid is a different (and, on
     # this octree, mostly blind) visibility: it reaches far fewer patches
     ref = o.ff_centroid(_all_pairs(n), ol.VIS_REFERENCE).reshape(n, n)
