@@ -152,9 +152,10 @@ const char *rad_lin_last_error(const rad_lin *lin);
  * the centroid of i to the whole patch j (shade.cpp:107-123) times a
  * visibility:
  *   RAD_VIS_SEGMENT    nothing crosses the open segment between the two
- *                      centroids (parameter in (1e-4, 1 - 1e-4)), found by a
- *                      grid walk over complete octree leaf lists; needs a
- *                      scene that pre-partition left unsplit
+ *                      centroids (parameter in (1e-4, 1 - 1e-4), walked from
+ *                      the lower slot index to the higher, so symmetric),
+ *                      found by a grid walk over complete octree leaf lists;
+ *                      needs a scene that pre-partition left unsplit
  *   RAD_VIS_REFERENCE  RayHitted from the centroid returns j
  *                      (raycast.cpp:335-360), blind spots included
  * *ms (may be NULL) = device time of the form-factor kernel.  Replaces the
