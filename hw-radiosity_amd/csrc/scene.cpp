@@ -1,8 +1,10 @@
 // Host stages: ReadTriangle, InitRadiosity (with a sort-based neighbour
-// search) and PrePartitionTriangles.  These are serial, index-order
+// search) and PrePartitionTriangles.  These are index-order
 // dependent steps of the reference; they run once per solve on the host and
 // the result is uploaded to the device.
 #include "scene.h"
+
+#include <parallel/algorithm>
 
 #include <algorithm>
 #include <cstdarg>
@@ -58,8 +60,7 @@ int HostScene::set_triangles(int n, const float *p9, const int *f3, const int *b
 {
 	if (n < 0) { err = "negative triangle count"; return -1; }
 	const long cap = std::max<long>(n, std::min(o.max_tri, o.triangle)) + 16;
-	pool.assign(cap, Patch());
-	std::memset(pool.data(), 0, sizeof(Patch) * pool.size());
+	if (!pool.reset((size_t)cap)) { err = "cannot allocate the triangle pool"; return -1; }
 	float lo[3] = {1000000.0f, 1000000.0f, 1000000.0f};
 	float hi[3] = {-1000000.0f, -1000000.0f, -1000000.0f};
 	for (int i = 0; i < n; i++) {
@@ -152,33 +153,44 @@ bool point_key(const float *p, VertexRef &r)
 
 inline bool same_vertex(const float *a, const float *b) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; }
 
-template <bool F> void link_neighbors(std::vector<Patch> &T, int n)
+// Keep the greatest order seen for one (slot, edge): 0 means none (an order
+// is never 0: it holds two different slots).
+inline void keep_max(uint64_t *cell, uint64_t order)
+{
+	uint64_t cur = __atomic_load_n(cell, __ATOMIC_RELAXED);
+	while (cur < order &&
+	       !__atomic_compare_exchange_n(cell, &cur, order, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+	}
+}
+
+template <bool F> void link_neighbors(PatchPool &T, int n)
 {
 	using A = Arith<F>;
-	std::vector<VertexRef> refs;
-	refs.reserve((size_t)n * 3);
+	std::vector<VertexRef> refs((size_t)n * 3);
+	std::vector<uint8_t> ok((size_t)n * 3);
+#pragma omp parallel for schedule(static)
 	for (int j = 0; j < n; j++)
 		for (int c = 0; c < 3; c++) {
-			VertexRef r;
-			if (point_key(T[j].p[c], r)) {
-				r.code = 3 * j + c;
-				refs.push_back(r);
-			}
+			VertexRef &r = refs[3 * (size_t)j + c];
+			ok[3 * (size_t)j + c] = point_key(T[j].p[c], r);
+			r.code = 3 * j + c;
 		}
-	std::sort(refs.begin(), refs.end());
-	struct Write { uint64_t order; int value; };
-	std::vector<Write> last((size_t)n * 3, Write{0, -1});
-	std::vector<uint8_t> written((size_t)n * 3, 0);
-	auto record = [&](int slot, int edge, uint64_t order, int value) {
-		const size_t s = (size_t)slot * 3 + edge;
-		if (!written[s] || order > last[s].order) {
-			last[s] = Write{order, value};
-			written[s] = 1;
-		}
-	};
-	for (size_t a = 0; a < refs.size();) {
-		size_t b = a + 1;
-		while (b < refs.size() && refs[b].same_point(refs[a])) b++;
+	size_t m = 0;  // drop NaN corners (never equal to anything)
+	for (size_t k = 0; k < refs.size(); k++)
+		if (ok[k]) refs[m++] = refs[k];
+	refs.resize(m);
+	__gnu_parallel::sort(refs.begin(), refs.end());
+	std::vector<size_t> group;  // starts of the runs of one point
+	for (size_t a = 0; a < refs.size(); a++)
+		if (a == 0 || !refs[a].same_point(refs[a - 1])) group.push_back(a);
+	group.push_back(refs.size());
+	// order = (i, j, v, n) packed so that a later write of the reference's
+	// loop is a greater number: i in bits 34.., j in bits 4..33, 3v + n below
+	std::vector<uint64_t> best((size_t)n * 3, 0);
+	const long ng = (long)group.size() - 1;
+#pragma omp parallel for schedule(dynamic, 512)
+	for (long g = 0; g < ng; g++) {
+		const size_t a = group[g], b = group[g + 1];
 		// every ordered pair (i, v) x (j, n) with i != j among refs[a, b)
 		for (size_t u = a; u < b; u++) {
 			const int i = refs[u].code / 3, v = refs[u].code % 3;
@@ -194,15 +206,19 @@ template <bool F> void link_neighbors(std::vector<Patch> &T, int n)
 				else continue;
 				const float cosine = A::dot(tp.n, tn.n) / A::len(tp.n) / A::len(tn.n);
 				if (!((double)cosine > 0.85)) continue;
-				record(i, v, order, j);
-				record(j, other_edge, order, i);
+				keep_max(&best[(size_t)i * 3 + v], order);             // i.neighbor[v] = j
+				keep_max(&best[(size_t)j * 3 + other_edge], order);    // j.neighbor[..] = i
 			}
 		}
-		a = b;
 	}
+#pragma omp parallel for schedule(static)
 	for (int s = 0; s < n; s++)
-		for (int e = 0; e < 3; e++)
-			if (written[(size_t)s * 3 + e]) T[s].neighbor[e] = last[(size_t)s * 3 + e].value;
+		for (int e = 0; e < 3; e++) {
+			const uint64_t order = best[(size_t)s * 3 + e];
+			if (!order) continue;
+			const int i = (int)(order >> 34), j = (int)((order >> 4) & 0x3fffffffu);
+			T[s].neighbor[e] = s == i ? j : i;
+		}
 }
 
 }  // namespace
@@ -263,7 +279,7 @@ namespace {
 struct Splitter {
 	HostScene &S;
 	explicit Splitter(HostScene &s) : S(s) {}
-	std::vector<Patch> &T() { return S.pool; }
+	PatchPool &T() { return S.pool; }
 
 	bool pool_full(int need) const { return S.trinum + need >= std::min(S.o.max_tri, S.o.triangle); }
 	int take_slot()

@@ -1,6 +1,7 @@
 // Host stages of the engine: input, InitRadiosity and pre-partition.
 #pragma once
 
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -8,9 +9,36 @@
 
 namespace rgpu {
 
+// The slot pool: zeroed memory from calloc, so a capacity of a million
+// slots (236 MB) costs only the pages that pre-partition actually touches.
+class PatchPool {
+public:
+	PatchPool() = default;
+	PatchPool(const PatchPool &) = delete;
+	PatchPool &operator=(const PatchPool &) = delete;
+	~PatchPool() { std::free(p_); }
+	// count zeroed slots; false when the allocation fails
+	bool reset(size_t count)
+	{
+		std::free(p_);
+		p_ = static_cast<Patch *>(std::calloc(count, sizeof(Patch)));
+		n_ = p_ ? count : 0;
+		return p_ != nullptr;
+	}
+	Patch &operator[](size_t i) { return p_[i]; }
+	const Patch &operator[](size_t i) const { return p_[i]; }
+	Patch *data() { return p_; }
+	const Patch *data() const { return p_; }
+	size_t size() const { return n_; }
+
+private:
+	Patch *p_ = nullptr;
+	size_t n_ = 0;
+};
+
 struct HostScene {
 	Options o;
-	std::vector<Patch> pool;  // slots; [0, n_orig) are the originals
+	PatchPool pool;  // slots; [0, n_orig) are the originals
 	int trinum = 0;
 	int n_orig = 0;
 	float G0[3] = {0, 0, 0}, G1[3] = {0, 0, 0};

@@ -140,3 +140,23 @@ def test_number_format_known_answers(tmp_path):
             want = last
         assert txt == want, (float(x), txt, want)
         last = want
+
+
+def test_hall_host_stages_match_oracle():
+    """The 200k-triangle hall (the benchmark scene): records after
+    InitRadiosity and pre-partition, neighbour links included, identical to
+    the oracle's (its neighbour search is an independent hashed pass)."""
+    from radgpu import scenes
+    s = scenes.hall_scene(2)
+    opts = ["-converge", "1200", "-triangle", "1000000", "-quiet", "1"]
+    with radgpu.Engine(*opts) as eng:
+        rec, g = eng.set_scene(s).prepare_host().records()
+    o = ol.Oracle(*(opts + ["-jobs", "8"]))
+    o.load(s)
+    o.init_radiosity()
+    o.prepartition()
+    orec, og = o.dump()
+    assert len(rec) == len(orec) == 253728
+    assert (rec["neighbor"] >= 0).mean() > 0.9
+    assert rec.tobytes() == orec.tobytes()
+    assert np.array_equal(g, og)
