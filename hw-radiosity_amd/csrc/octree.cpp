@@ -89,6 +89,26 @@ template <bool F> bool overlaps(const Patch &t, const float *g0, const float *g1
 	return true;
 }
 
+// A sufficient condition for CrossOver to say no: the triangle's bounding
+// box misses the node's box by more than rounding can move a point that
+// InterpVector computes on one of its edges (such a point lies within the
+// edge's range up to a few ulps; the margin is ~100 ulps of the larger
+// coordinate plus an absolute floor).  Every test CrossOver makes compares
+// such points, or vertices, with the box's planes, so beyond the margin each
+// test lands on the same side it would for the bounding box.
+struct TriBox {
+	float lo[3], hi[3];
+};
+inline bool surely_apart(const TriBox &b, const float *g0, const float *g1)
+{
+	for (int k = 0; k < 3; k++) {
+		const float m0 = 1e-5f * (std::fabs(g0[k]) + std::fabs(b.hi[k])) + 1e-30f;
+		const float m1 = 1e-5f * (std::fabs(g1[k]) + std::fabs(b.lo[k])) + 1e-30f;
+		if (b.hi[k] < g0[k] - m0 || b.lo[k] > g1[k] + m1) return true;
+	}
+	return false;
+}
+
 struct BNode {
 	float g0[3], g1[3];
 	int sub[8];
@@ -105,6 +125,7 @@ struct Subtree {
 
 template <bool F> struct Builder {
 	const Patch *pool;
+	const TriBox *box;
 	int task_depth;
 
 	int build(Subtree &st, const std::vector<int> &idx, int level, const float *g0, const float *g1)
@@ -150,7 +171,7 @@ template <bool F> struct Builder {
 					std::vector<int> sel;
 					sel.reserve(idx.size());
 					for (int t : idx)
-						if (overlaps<F>(pool[t], q0[c], q1[c])) sel.push_back(t);
+						if (!surely_apart(box[t], q0[c], q1[c]) && overlaps<F>(pool[t], q0[c], q1[c])) sel.push_back(t);
 					build(*kid[c], sel, level + 1, q0[c], q1[c]);
 				}
 			}
@@ -176,7 +197,7 @@ template <bool F> struct Builder {
 				std::vector<int> sel;
 				sel.reserve(idx.size());
 				for (int t : idx)
-					if (overlaps<F>(pool[t], q0[c], q1[c])) sel.push_back(t);
+					if (!surely_apart(box[t], q0[c], q1[c]) && overlaps<F>(pool[t], q0[c], q1[c])) sel.push_back(t);
 				nd.sub[c] = build(st, sel, level + 1, q0[c], q1[c]);
 			}
 		}
@@ -194,11 +215,20 @@ FlatOctree build_octree(const Patch *pool, int n, const float *G0, const float *
 	std::vector<int> all(n);
 	for (int i = 0; i < n; i++) all[i] = i;
 	const int nthreads = threads > 0 ? threads : omp_get_max_threads();
+	std::vector<TriBox> box(n);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+	for (int i = 0; i < n; i++)
+		for (int k = 0; k < 3; k++) {
+			const float a = pool[i].p[0][k], b = pool[i].p[1][k], c = pool[i].p[2][k];
+			const bool nan = std::isnan(a) || std::isnan(b) || std::isnan(c);  // never rejected early
+			box[i].lo[k] = nan ? -INFINITY : std::fmin(a, std::fmin(b, c));
+			box[i].hi[k] = nan ? INFINITY : std::fmax(a, std::fmax(b, c));
+		}
 #pragma omp parallel num_threads(nthreads)
 #pragma omp single
 	{
-		if (fma) Builder<true>{pool, 2}.build(st, all, 0, G0, G1);
-		else Builder<false>{pool, 2}.build(st, all, 0, G0, G1);
+		if (fma) Builder<true>{pool, box.data(), 3}.build(st, all, 0, G0, G1);
+		else Builder<false>{pool, box.data(), 3}.build(st, all, 0, G0, G1);
 	}
 	FlatOctree T;
 	T.n_nodes = (int)st.nodes.size();

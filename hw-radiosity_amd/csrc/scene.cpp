@@ -63,6 +63,7 @@ int HostScene::set_triangles(int n, const float *p9, const int *f3, const int *b
 	if (!pool.reset((size_t)cap)) { err = "cannot allocate the triangle pool"; return -1; }
 	float lo[3] = {1000000.0f, 1000000.0f, 1000000.0f};
 	float hi[3] = {-1000000.0f, -1000000.0f, -1000000.0f};
+#pragma omp parallel for schedule(static)
 	for (int i = 0; i < n; i++) {
 		Patch &t = pool[i];
 		std::memcpy(t.p, p9 + 9 * (size_t)i, sizeof t.p);
@@ -70,14 +71,17 @@ int HostScene::set_triangles(int n, const float *p9, const int *f3, const int *b
 		std::memcpy(t.Brgb, b3 + 3 * (size_t)i, sizeof t.Brgb);
 		if (o.fma) derive_fields<true>(t);
 		else derive_fields<false>(t);
+		t.parent = i;
+		t.neighbor[0] = t.neighbor[1] = t.neighbor[2] = -1;
+	}
+	for (int i = 0; i < n; i++) {  // in index order, as the reference's min/max
+		const Patch &t = pool[i];
 		for (int v = 0; v < 3; v++)
 			for (int k = 0; k < 3; k++) {
 				const float x = t.p[v][k];
 				if (x < lo[k]) lo[k] = x;   // std::min(lo, x)
 				if (hi[k] < x) hi[k] = x;   // std::max(hi, x)
 			}
-		t.parent = i;
-		t.neighbor[0] = t.neighbor[1] = t.neighbor[2] = -1;
 	}
 	trinum = n_orig = n;
 	float m = 0;

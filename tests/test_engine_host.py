@@ -160,3 +160,17 @@ def test_hall_host_stages_match_oracle():
     assert (rec["neighbor"] >= 0).mean() > 0.9
     assert rec.tobytes() == orec.tobytes()
     assert np.array_equal(g, og)
+
+
+def test_hall_tree_matches_oracle():
+    """The benchmark scene's octree: node count, list entries, grid size and
+    MinGridLen equal to the oracle's (the ray parity tests check the lists)."""
+    from radgpu import scenes
+    s = scenes.hall_scene(2)
+    o = ol.Oracle("-quiet", "1", "-triangle", "1000000", "-jobs", "8")
+    o.load(s)
+    o.build_tree()
+    with radgpu.Engine("-quiet", "1", "-triangle", "1000000") as eng:
+        eng.set_scene(s)
+        a, b = eng.tree_info(), o.tree_info()
+    assert a[:3] == b[:3] and struct.pack("f", a[3]) == struct.pack("f", b[3])
