@@ -49,6 +49,7 @@
 #include <vector>
 
 #include "device.h"
+#include "pack.h"
 
 namespace rgpu {
 namespace {
@@ -1010,17 +1011,21 @@ __global__ __launch_bounds__(kBlock) void k_ff_pairs(DScene S, DRecv R, const in
 		}                                                                         \
 	} while (0)
 
+// A device array of n elements; the allocation is kept and reused while
+// later sizes fit in it.
 template <typename T> struct DevBuf {
 	T *p = nullptr;
-	size_t n = 0;
+	size_t n = 0, cap = 0;
 	~DevBuf() { if (p) (void)hipFree(p); }
 	int alloc(size_t count, std::string &err)
 	{
+		n = count;
+		if (count == 0 || (p && count <= cap)) return 0;
 		if (p) (void)hipFree(p);
 		p = nullptr;
-		n = count;
-		if (count == 0) return 0;
+		cap = 0;
 		HIP_OK(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+		cap = count;
 		return 0;
 	}
 	int put(const T *src, size_t count, hipStream_t s, std::string &err)
@@ -1138,20 +1143,13 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	// leaf lists with the originals inline, as the octree sees them: the
 	// post-pre-partition contents of slots [0, n_orig)
 	const size_t ni = t.items.size();
-	std::vector<float4> lnc(ni * 2), lse(ni * 3);
-	for (size_t k = 0; k < ni; k++) {
-		const int id = t.items[k];
-		if (id < 0 || id >= n0) {
+	for (size_t k = 0; k < ni; k++)
+		if (t.items[k] < 0 || t.items[k] >= n0) {
 			err = "octree item out of range";
 			return -1;
 		}
-		const Patch &p = s.pool[id];
-		float idf;
-		std::memcpy(&idf, &id, 4);
-		lnc[2 * k] = make_float4(p.n[0], p.n[1], p.n[2], p.c[0]);
-		lnc[2 * k + 1] = make_float4(p.c[1], p.c[2], idf, 0.f);
-		for (int q = 0; q < 3; q++) lse[3 * k + q] = make_float4(p.se[q][0], p.se[q][1], p.se[q][2], p.se[q][3]);
-	}
+	std::vector<float4> lnc(ni * 2), lse(ni * 3);
+	pack_leaf_items(t.items.data(), ni, s.pool.data(), (float *)lnc.data(), (float *)lse.data());
 	for (int l = 0; l < t.n_leaves; l++) {
 		int begin, cd;
 		std::memcpy(&begin, &t.leaf_box[(size_t)l * 8 + 3], 4);
@@ -1168,23 +1166,8 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 		}
 	std::vector<float> vx((size_t)n * 3), vy((size_t)n * 3), vz((size_t)n * 3);
 	std::vector<float4> nrm(n), cA(n), rho(n), acc((size_t)n * 3), dacc((size_t)n * 3), dB(n);
-	for (int i = 0; i < n; i++) {
-		const Patch &p = s.pool[i];
-		for (int v = 0; v < 3; v++) {
-			vx[3 * i + v] = p.p[v][0];
-			vy[3 * i + v] = p.p[v][1];
-			vz[3 * i + v] = p.p[v][2];
-			acc[3 * i + v] = make_float4(p.accB[v][0], p.accB[v][1], p.accB[v][2], 0.f);
-			dacc[3 * i + v] = make_float4(p.deltaaccB[v][0], p.deltaaccB[v][1], p.deltaaccB[v][2], 0.f);
-		}
-		int par = p.parent;
-		float parf;
-		std::memcpy(&parf, &par, 4);
-		nrm[i] = make_float4(p.n[0], p.n[1], p.n[2], parf);
-		cA[i] = make_float4(p.c[0], p.c[1], p.c[2], p.area);
-		rho[i] = make_float4((float)p.Frgb[0], (float)p.Frgb[1], (float)p.Frgb[2], 0.f);
-		dB[i] = make_float4(p.deltaB[0], p.deltaB[1], p.deltaB[2], 0.f);
-	}
+	pack_receivers(s.pool.data(), n, vx.data(), vy.data(), vz.data(), (float *)nrm.data(), (float *)cA.data(),
+	               (float *)rho.data(), (float *)acc.data(), (float *)dacc.data(), (float *)dB.data());
 	hipStream_t st = d.stream;
 	if (d.lnc.put(lnc.data(), lnc.size(), st, err) || d.lse.put(lse.data(), lse.size(), st, err) ||
 	    d.cell_leaf.put(t.cell_leaf.data(), t.cell_leaf.size(), st, err) ||
