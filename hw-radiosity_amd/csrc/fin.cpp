@@ -1,9 +1,13 @@
 #include "fin.h"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <string>
 #include <vector>
 
 namespace rgpu {
@@ -57,12 +61,86 @@ int format_truncated(float x, char *out)
 	return (int)(o - out);
 }
 
-const char *FinWriter::text(float x)
+namespace {
+
+// One record in the reference's order: "Triangle", the mean colour, then per
+// vertex its coordinates and the normal, written at o (at most kRecordMax
+// bytes); returns the end.  last = the text of the last value that
+// converted (the reference prints its static buffer again for a value that
+// does not convert); updated in place.
+constexpr int kRecordMax = 1024;
+
+inline char *put_text(char *o, const char *s)
+{
+	const size_t l = std::strlen(s);
+	std::memcpy(o, s, l);
+	return o + l;
+}
+
+char *format_record(const Patch &t, bool old_format, char *last, char *o)
+{
+	o = put_text(o, "Triangle \n");
+	if (old_format) {
+		for (int v = 0; v < 3; v++) {
+			o += std::snprintf(o, 400, "%f %f %f ", t.p[v][0], t.p[v][1], t.p[v][2]);
+			o += std::snprintf(o, 64, "%i %i %i 0 0 0 \n", (int)t.accB[v][0], (int)t.accB[v][1], (int)t.accB[v][2]);
+		}
+		return o;
+	}
+	int rgb[3];
+	for (int c = 0; c < 3; c++) {
+		float m = 0;
+		for (int v = 0; v < 3; v++) m += t.accB[v][c];
+		m /= 3;
+		if (255.0f < m) m = 255.0f;  // std::min(m, 255.0f)
+		if (m < 0.0f) m = 0.0f;      // std::max(m, 0.0f)
+		if (std::isnan(m)) m = 0;
+		rgb[c] = (int)m;
+	}
+	o += std::snprintf(o, 64, "%d %d %d 0 0 0\n", rgb[0], rgb[1], rgb[2]);
+	auto text = [last](float x) {
+		char tmp[48];
+		if (format_truncated(x, tmp)) std::strcpy(last, tmp);
+		return (const char *)last;
+	};
+	// the normal text is formatted before the vertices and reused
+	char nrm[160];
+	char *e = nrm;
+	for (int k = 0; k < 3; k++) {
+		e = put_text(e, text(t.n[k]));
+		*e++ = k < 2 ? ' ' : '\n';
+	}
+	const size_t nl = (size_t)(e - nrm);
+	for (int v = 0; v < 3; v++) {
+		for (int k = 0; k < 3; k++) {
+			o = put_text(o, text(t.p[v][k]));
+			*o++ = ' ';
+		}
+		std::memcpy(o, nrm, nl);
+		o += nl;
+	}
+	return o;
+}
+
+// The state `last` after formatting record t, given it before: the last of
+// its values (formatting order n[0..2], p[0][0..2], .., p[2][0..2]) that
+// converts, if any.
+void last_after(const Patch &t, char *last)
 {
 	char tmp[48];
-	if (format_truncated(x, tmp)) std::strcpy(last_, tmp);
-	return last_;
+	for (int q = 8; q >= 0; q--)
+		if (format_truncated(t.p[q / 3][q % 3], tmp)) {
+			std::strcpy(last, tmp);
+			return;
+		}
+	for (int k = 2; k >= 0; k--)
+		if (format_truncated(t.n[k], tmp)) {
+			std::strcpy(last, tmp);
+			return;
+		}
 }
+
+}  // namespace
 
 int FinWriter::write(const std::string &path, const Patch *pool, int trinum, bool old_format, std::string &err)
 {
@@ -71,51 +149,39 @@ int FinWriter::write(const std::string &path, const Patch *pool, int trinum, boo
 		err = "Error: output file " + path + " open failed";
 		return -1;
 	}
-	std::vector<char> buf(1 << 20);
-	std::setvbuf(f, buf.data(), _IOFBF, buf.size());
-	char line[256];
-	for (int i = 0; i < trinum; i++) {
-		const Patch &t = pool[i];
-		if (t.parent < 0) continue;  // logical slots are not written
-		std::fputs("Triangle \n", f);
-		if (old_format) {
-			for (int v = 0; v < 3; v++) {
-				std::fprintf(f, "%f %f %f ", t.p[v][0], t.p[v][1], t.p[v][2]);
-				std::fprintf(f, "%i %i %i 0 0 0 \n", (int)t.accB[v][0], (int)t.accB[v][1], (int)t.accB[v][2]);
+	// records are formatted in chunks in parallel; each chunk starts from the
+	// `last` state that the records before it leave
+	const int kChunk = 4096;
+	const int nch = (trinum + kChunk - 1) / kChunk;
+	std::vector<std::array<char, 48>> start(nch);
+	for (int c = 0; c < nch; c++) {
+		std::memcpy(start[c].data(), last_, sizeof last_);
+		if (old_format) continue;
+		for (int i = std::min(trinum, (c + 1) * kChunk) - 1; i >= c * kChunk; i--)
+			if (pool[i].parent >= 0) {
+				last_after(pool[i], last_);
+				break;
 			}
-			continue;
-		}
-		int rgb[3];
-		for (int c = 0; c < 3; c++) {
-			float m = 0;
-			for (int v = 0; v < 3; v++) m += t.accB[v][c];
-			m /= 3;
-			if (255.0f < m) m = 255.0f;  // std::min(m, 255.0f)
-			if (m < 0.0f) m = 0.0f;      // std::max(m, 0.0f)
-			if (std::isnan(m)) m = 0;
-			rgb[c] = (int)m;
-		}
-		std::fprintf(f, "%d %d %d 0 0 0\n", rgb[0], rgb[1], rgb[2]);
-		// the normal text is formatted before the vertices and reused
-		std::string nrm;
-		for (int k = 0; k < 3; k++) {
-			nrm += text(t.n[k]);
-			nrm += k < 2 ? " " : "\n";
-		}
-		for (int v = 0; v < 3; v++) {
-			int len = 0;
-			for (int k = 0; k < 3; k++) {
-				const char *s = text(t.p[v][k]);
-				const size_t l = std::strlen(s);
-				std::memcpy(line + len, s, l);
-				len += (int)l;
-				line[len++] = ' ';
-			}
-			std::fwrite(line, 1, len, f);
-			std::fputs(nrm.c_str(), f);
-		}
 	}
-	std::fclose(f);
+	bool ok = true;
+	// chunks are formatted in parallel and written in order as they finish
+#pragma omp parallel for ordered schedule(dynamic, 1)
+	for (int c = 0; c < nch; c++) {
+		char last[48];
+		std::memcpy(last, start[c].data(), sizeof last);
+		std::unique_ptr<char[]> buf(new char[(size_t)kChunk * kRecordMax]);  // not zeroed
+		char *o = buf.get();
+		for (int i = c * kChunk; i < std::min(trinum, (c + 1) * kChunk); i++)
+			if (pool[i].parent >= 0) o = format_record(pool[i], old_format, last, o);  // logical slots are not written
+		const size_t len = (size_t)(o - buf.get());
+#pragma omp ordered
+		if (ok) ok = std::fwrite(buf.get(), 1, len, f) == len;
+	}
+	ok = (std::fclose(f) == 0) && ok;
+	if (!ok) {
+		err = "Error: output file " + path + " write failed";
+		return -1;
+	}
 	return 0;
 }
 

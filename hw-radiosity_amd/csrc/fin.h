@@ -24,7 +24,6 @@ private:
 	// The reference formats into one static buffer and prints it even when
 	// a value is not converted, so such a value repeats the previous text.
 	char last_[48] = {0};
-	const char *text(float x);
 };
 
 }  // namespace rgpu
