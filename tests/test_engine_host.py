@@ -174,3 +174,34 @@ def test_hall_tree_matches_oracle():
         eng.set_scene(s)
         a, b = eng.tree_info(), o.tree_info()
     assert a[:3] == b[:3] and struct.pack("f", a[3]) == struct.pack("f", b[3])
+
+
+def test_fin_unconvertible_values_across_chunks(tmp_path):
+    """Values the reference's formatter does not convert (a degenerate
+    triangle's NaN normal) print the previous value's text again; the engine
+    formats records in parallel chunks of 4096, so degenerate records at and
+    around the chunk starts must carry the text over exactly as the oracle's
+    sequential writer does."""
+    from radgpu import scenes
+    rng = np.random.default_rng(12)
+    n = 9000
+    base = rng.uniform(0, 50, (n, 1, 3)).astype(np.float32)
+    p = (base + rng.uniform(0, 1, (n, 3, 3))).astype(np.float32)
+    for i in (0, 4095, 4096, 4097, 8191, 8192, n - 1):
+        p[i, 1] = p[i, 0]
+        p[i, 2] = p[i, 0]          # zero area: normal 0/0 = NaN
+    s = scenes.Scene(p=p, frgb=rng.integers(0, 255, (n, 3)).astype(np.int32),
+                     brgb=np.zeros((n, 3), np.int32))
+    opts = ["-triangle", "20000", "-quiet", "1"]
+    o = ol.Oracle(*opts)
+    o.load(s)
+    o.write_fin(str(tmp_path / "o.fin"))
+    with radgpu.Engine(*opts) as eng:
+        eng.set_scene(s)
+        eng.write_fin(str(tmp_path / "e.fin"))
+    a, b = open(tmp_path / "o.fin", "rb").read(), open(tmp_path / "e.fin", "rb").read()
+    assert b"nan" not in a.lower() and a.count(b"Triangle") == n
+    recs = a.split(b"Triangle \n")[1:]
+    prev_last = recs[4095].split()[-4]           # record 4095's p[2][2] (before its normal)
+    assert recs[4096].split(b"\n")[1].split()[3:6] == [prev_last] * 3  # 4096's normal repeats it
+    assert a == b
