@@ -92,6 +92,10 @@ struct DRecv {
 	float4 *dacc;
 	float4 *dB;
 	int nslots;
+	// receiver processing order: order[p] = the slot at position p (slots
+	// along a Z-order curve of their centroids, so that a block's, a wave's
+	// and an XCD's rays start close together); results do not depend on it
+	const int *__restrict__ order;
 };
 
 struct SrcPatch {
@@ -527,8 +531,8 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 	int ref_rays = 0, nray = 0;
 #pragma unroll 1
 	for (int g = 0; g < kGroups; g++) {
-		const int r = m.slot(g);
 		const bool valid = m.valid(g, R.nslots, nblocks);
+		const int r = valid ? R.order[m.slot(g)] : -1;
 		bool lv;
 		if (k == 0) {
 			int parent = -1;
@@ -580,11 +584,12 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 	add_block_counters(vals, bcnt);
 }
 
-// The rays of source k as a dense list in lane order: block b (one per
-// k_gen range) places the lanes i of [b * 6 * kRecvPerBlock, +6 *
-// kRecvPerBlock) whose form-factor prefix is non-zero at offset
-// sum(raycnt[0..b)), in ascending i; the last block stores the total.
-__global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float *ffbuf, int nlanes,
+// The rays of source k as a dense list in receiver-position order: block b
+// (one per k_gen range) takes positions q of [b * 6 * kRecvPerBlock, +6 *
+// kRecvPerBlock), lane i = 6 * order[q / 6] + q % 6, and places those whose
+// form-factor prefix is non-zero at offset sum(raycnt[0..b)), in ascending
+// q; the last block stores the total.
+__global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float *ffbuf, const int *order, int nlanes,
                                                     const int *raycnt, int nblocks, int *rays)
 {
 	if (L->done || k >= L->nsrc) return;
@@ -602,8 +607,9 @@ __global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float 
 	const int i0 = b * chunk, i1 = min(nlanes, i0 + chunk);
 	int run = base;
 	for (int r0 = i0; r0 < i1; r0 += kBlock) {
-		const int i = r0 + threadIdx.x;
-		const bool ray = i < i1 && ffbuf[i] != 0.0f;
+		const int q = r0 + threadIdx.x;
+		const int i = q < i1 ? 6 * order[q / 6] + q % 6 : 0;
+		const bool ray = q < i1 && ffbuf[i] != 0.0f;
 		const unsigned long long bal = __ballot(ray);
 		const int rank = __popcll(bal & ((1ull << lane) - 1));
 		__syncthreads();  // wsum is reused each round
@@ -618,8 +624,8 @@ __global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float 
 }
 
 // Source k's rays, part 2: one lane per ray of the compacted list, which
-// keeps slot order, so a wave's rays start from neighbouring receivers
-// toward the same point and every lane has a ray.
+// keeps the receivers' spatial order, so a wave's rays start from
+// neighbouring receivers toward the same point and every lane has a ray.
 // RayHitted (raycast.cpp:335-360); a ray that does not reach the source
 // zeroes its form factor (shade.cpp:124-126).  XCD-contiguous block order.
 template <bool F>
@@ -1068,6 +1074,7 @@ struct DeviceEngine::Impl {
 	DevBuf<unsigned char> live;
 	DevBuf<float> ffbuf;
 	DevBuf<int> raycnt, rays;  // per-k_gen-block ray counts, compacted ray list
+	DevBuf<int> order;         // receiver processing order (DRecv::order)
 	DScene S;
 	DRecv R;
 	Loop *host_loop = nullptr;  // pinned mirror of the loop header
@@ -1168,7 +1175,10 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	std::vector<float4> nrm(n), cA(n), rho(n), acc((size_t)n * 3), dacc((size_t)n * 3), dB(n);
 	pack_receivers(s.pool.data(), n, vx.data(), vy.data(), vz.data(), (float *)nrm.data(), (float *)cA.data(),
 	               (float *)rho.data(), (float *)acc.data(), (float *)dacc.data(), (float *)dB.data());
+	std::vector<int> order(n);
+	spatial_order(s.pool.data(), n, s.G0, s.G1, order.data());
 	hipStream_t st = d.stream;
+	if (d.order.put(order.data(), order.size(), st, err)) return -1;
 	if (d.lnc.put(lnc.data(), lnc.size(), st, err) || d.lse.put(lse.data(), lse.size(), st, err) ||
 	    d.cell_leaf.put(t.cell_leaf.data(), t.cell_leaf.size(), st, err) ||
 	    d.leaf_box.put((const float4 *)t.leaf_box.data(), t.leaf_box.size() / 4, st, err) ||
@@ -1196,7 +1206,7 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	d.S = DScene{d.lnc.p, d.lse.p, d.cell_leaf.p, d.leaf_box.p, s.G0[0], s.G0[1], s.G0[2], s.G1[0], s.G1[1], s.G1[2],
 	             (double)t.min_grid, 1.0 / (double)t.min_grid, t.gt, thr_t, thr_t * kGuardDown,
 	             largest_float_at_most(1e-6)};
-	d.R = DRecv{d.vx.p, d.vy.p, d.vz.p, d.nrm.p, d.cA.p, d.rho.p, d.acc.p, d.dacc.p, d.dB.p, n};
+	d.R = DRecv{d.vx.p, d.vy.p, d.vz.p, d.nrm.p, d.cA.p, d.rho.p, d.acc.p, d.dacc.p, d.dB.p, n, d.order.p};
 	if (d.lowest || d.parallel_src <= 1) {
 		const int sb = (n + kBlock - 1) / kBlock;
 		if (sb > 0) {
@@ -1264,7 +1274,7 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 				const dim3 g(grid), blk(kBlock), tg(tgrid);
 				if (d.fma) hipLaunchKernelGGL(k_gen<true>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
 				else hipLaunchKernelGGL(k_gen<false>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
-				hipLaunchKernelGGL(k_compact, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, 6 * d.R.nslots, d.raycnt.p,
+				hipLaunchKernelGGL(k_compact, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, d.order.p, 6 * d.R.nslots, d.raycnt.p,
 				                   d.nblocks, d.rays.p);
 				if (timing && k == 0) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv)], st), "event");
 				if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, blk, 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);

@@ -19,4 +19,8 @@ void pack_leaf_items(const int *items, size_t ni, const Patch *pool, float *lnc,
 void pack_receivers(const Patch *pool, int n, float *vx, float *vy, float *vz, float *nrm, float *cA, float *rho,
                     float *acc, float *dacc, float *dB);
 
+// order[0..n) = the slots sorted by the Z-order (Morton) code of their
+// centroids on a 2^10 grid over [G0, G1], ties by slot; a permutation.
+void spatial_order(const Patch *pool, int n, const float *G0, const float *G1, int *order);
+
 }  // namespace rgpu
