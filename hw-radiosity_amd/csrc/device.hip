@@ -166,15 +166,23 @@ __device__ __forceinline__ int wave_sum(int x)
 	for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
 	return x;
 }
-// (int)(((double)q - G0) / MinGridLen) of TreeNodeNum (raycast.cpp:263)
+// (int)(((double)q - G0) / MinGridLen) of TreeNodeNum (raycast.cpp:263),
+// from the product with the reciprocal unless its fraction is within 1e-9
+// of an integer; there the exact quotient decides.  The division sits
+// behind an asm barrier so that the compiler keeps it in a branch (it is
+// rare) instead of computing it for every lane and selecting.
 __device__ __forceinline__ int cell_coord(float q, float g0, const DScene &S)
 {
 	const double x = (double)q - (double)g0;
 	const double qd = x * S.inv_mgl;
 	const double fl = floor(qd);
 	const double fr = qd - fl;
-	if (fr > 1e-9 && fr < 1.0 - 1e-9) return (int)fl;
-	return (int)(x / S.mgl);
+	int g = (int)fl;
+	if (!(fr > 1e-9 && fr < 1.0 - 1e-9)) {
+		asm volatile("" ::: "memory");
+		g = (int)(x / S.mgl);
+	}
+	return g;
 }
 
 // ---- RayHitted ------------------------------------------------------------
