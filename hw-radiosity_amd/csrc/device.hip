@@ -57,6 +57,12 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
+#ifndef RGPU_TRACE_BLOCK
+#define RGPU_TRACE_BLOCK 64
+#endif
+// k_trace's block: one wave, so that a wave whose rays walk far holds only
+// its own slot (rays' costs are heavy-tailed)
+constexpr int kTraceBlock = RGPU_TRACE_BLOCK;
 constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
 constexpr int kGroups = 2;        // receiver groups per block
 constexpr int kMinQueue = 8;      // iterations queued per host check at least (up to -batch)
@@ -508,9 +514,9 @@ struct RecvMap {
 	}
 };
 
-__device__ void add_block_counters(const int (&vals_in)[8], long long *bcnt)
+template <int W = kWavesPerBlock> __device__ void add_block_counters(const int (&vals_in)[8], long long *bcnt)
 {
-	__shared__ int wcnt[kWavesPerBlock][8];
+	__shared__ int wcnt[W][8];
 	int vals[8];
 	for (int k = 0; k < 8; k++) vals[k] = wave_sum(vals_in[k]);
 	if ((threadIdx.x & 63) == 0)
@@ -519,7 +525,7 @@ __device__ void add_block_counters(const int (&vals_in)[8], long long *bcnt)
 	if (threadIdx.x == 0)
 		for (int k = 0; k < 8; k++) {
 			long long sum = 0;
-			for (int w = 0; w < kWavesPerBlock; w++) sum += wcnt[w][k];
+			for (int w = 0; w < W; w++) sum += wcnt[w][k];
 			if (sum) bcnt[(size_t)blockIdx.x * 8 + k] += sum;
 		}
 }
@@ -637,15 +643,15 @@ __global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float 
 // RayHitted (raycast.cpp:335-360); a ray that does not reach the source
 // zeroes its form factor (shade.cpp:124-126).  XCD-contiguous block order.
 template <bool F>
-__global__ __launch_bounds__(kBlock) void k_trace(DScene S, DRecv R, Loop *L, int k, float *ffbuf, long long *bcnt,
-                                                  const int *rays)
+__global__ __launch_bounds__(kTraceBlock) void k_trace(DScene S, DRecv R, Loop *L, int k, float *ffbuf,
+                                                       long long *bcnt, const int *rays)
 {
 	if (L->done || k >= L->nsrc) return;
 	// the compacted list spread XCD-contiguously over the blocks that have work
-	const int nr = L->nrays, nb = (nr + kBlock - 1) / kBlock, per = (nb + 7) >> 3;
+	const int nr = L->nrays, nb = (nr + kTraceBlock - 1) / kTraceBlock, per = (nb + 7) >> 3;
 	const int xcd = blockIdx.x & 7;
 	const int vb = xcd * per + (blockIdx.x >> 3);
-	const int j = vb * kBlock + threadIdx.x;
+	const int j = vb * kTraceBlock + threadIdx.x;
 	Cnt c = {0, 0, 0, 0, 0, 0, 0};
 	if ((int)(blockIdx.x >> 3) < per && j < nr) {
 		const int i = rays[j];
@@ -659,7 +665,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DScene S, DRecv R, Loop *L, in
 		if (walk<F>(S, px, py, pz, sp.c[0] - px, sp.c[1] - py, sp.c[2] - pz, ban, c) != s.log) ffbuf[i] = 0.0f;
 	}
 	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
-	add_block_counters(vals, bcnt);
+	add_block_counters<kTraceBlock / kWave>(vals, bcnt);
 }
 
 // Source k, part 3: CalRadiosity (shade.cpp:244-259) per receiver, its
@@ -1199,7 +1205,7 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	// grid sizes first: every kernel that adds into bcnt indexes it by its
 	// hardware block index, so bcnt must cover the largest grid
 	d.nblocks = (n + kRecvPerBlock - 1) / kRecvPerBlock;
-	d.trace_nblocks = (int)(((size_t)n * 6 + kBlock - 1) / kBlock);
+	d.trace_nblocks = (int)(((size_t)n * 6 + kTraceBlock - 1) / kTraceBlock);
 	const int grid = ((d.nblocks + 7) / 8) * 8;  // a whole number of blocks per XCD
 	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
 	std::vector<long long> zeros((size_t)std::max(grid, tgrid) * 8, 0);
@@ -1285,8 +1291,8 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 				hipLaunchKernelGGL(k_compact, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, d.order.p, 6 * d.R.nslots, d.raycnt.p,
 				                   d.nblocks, d.rays.p);
 				if (timing && k == 0) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv)], st), "event");
-				if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, blk, 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
-				else hipLaunchKernelGGL(k_trace<false>, tg, blk, 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
+				if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
+				else hipLaunchKernelGGL(k_trace<false>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
 				if (timing && k == 0) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv) + 1], st), "event");
 #define UPD(F, FS, LO) hipLaunchKernelGGL((k_update<F, FS, LO>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.converge, d.bcnt.p, d.nblocks)
 				if (multi) { if (d.fma) UPD(true, false, false); else UPD(false, false, false); }
