@@ -57,12 +57,9 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
-#ifndef RGPU_TRACE_BLOCK
-#define RGPU_TRACE_BLOCK 64
-#endif
 // k_trace's block: one wave, so that a wave whose rays walk far holds only
 // its own slot (rays' costs are heavy-tailed)
-constexpr int kTraceBlock = RGPU_TRACE_BLOCK;
+constexpr int kTraceBlock = 64;
 constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
 constexpr int kGroups = 2;        // receiver groups per block
 constexpr int kMinQueue = 8;      // iterations queued per host check at least (up to -batch)
