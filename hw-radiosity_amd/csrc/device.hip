@@ -1238,6 +1238,10 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 	hipStream_t st = d.stream;
 	*shots = 0;
 	*converged = 0;
+	if (d.R.nslots == 0) {  // no triangles: no candidate, as listCanditate finds none
+		*converged = 1;
+		return 0;
+	}
 	HIP_OK(hipMemcpyAsync(d.host_loop, d.loop.p, 64, hipMemcpyDeviceToHost, st), "loop read");
 	HIP_OK(hipStreamSynchronize(st), "loop read");
 	if (d.host_loop->done) {

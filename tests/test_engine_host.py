@@ -205,3 +205,25 @@ def test_fin_unconvertible_values_across_chunks(tmp_path):
     prev_last = recs[4095].split()[-4]           # record 4095's p[2][2] (before its normal)
     assert recs[4096].split(b"\n")[1].split()[3:6] == [prev_last] * 3  # 4096's normal repeats it
     assert a == b
+
+
+@pytest.mark.parametrize("name", ["empty", "lone_light", "facing_pair", "no_light", "room_zero_area"])
+def test_edge_scene_host_stages_match_oracle(name):
+    """Host stages of the edge-case scenes (no triangles, a lone light, no
+    light, zero-area triangles with NaN normals) equal the oracle's."""
+    from edge_scenes import edge_scenes
+    s = edge_scenes()[name]
+    opts = ["-converge", "1200", "-triangle", "20000", "-quiet", "1"]
+    with radgpu.Engine(*opts) as eng:
+        a = eng.set_scene(s).tree_info()
+        rec, g = eng.prepare_host().records()
+    o = ol.Oracle(*opts)
+    o.load(s)
+    o.build_tree()
+    b = o.tree_info()
+    o.init_radiosity()
+    o.prepartition()
+    orec, og = o.dump()
+    assert a[:3] == b[:3] and struct.pack("f", a[3]) == struct.pack("f", b[3])
+    assert len(rec) == len(orec) and rec.tobytes() == orec.tobytes()
+    assert g.tobytes() == og.tobytes()
