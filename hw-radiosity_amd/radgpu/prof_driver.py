@@ -1,7 +1,9 @@
 """Profiling driver: one hall-200k solve (or, with --linear, the explicit
 matrix workload: the 32768-patch room's form factors, FP16 gathers with 3
-and 192 right-hand sides, 2000 plain shots), then a plain 256 MiB device copy
-that calibrates the FETCH_SIZE / WRITE_SIZE counters in the same run.
+and 192 right-hand sides, 2000 plain shots; with --gemm, the same matrix and
+five FP16 sweeps with 192 right-hand sides at the tile RT = 4, split-K 2),
+then a plain 256 MiB device copy that calibrates the FETCH_SIZE / WRITE_SIZE
+counters in the same run.
 
     rocprofv3 --pmc FETCH_SIZE -d <dir> -o p -- python3 hw-radiosity_amd/radgpu/prof_driver.py [--linear]
 """
@@ -36,9 +38,24 @@ def linear():
     e.close()
 
 
+def gemm():
+    n = 32768
+    sc = scenes.patch_room(1, n)
+    e = radgpu.Engine("-triangle", str(n), "-quiet", "1")
+    e.set_scene(sc).prepare_host().upload()
+    lin = radgpu.LinearSystem(n)
+    print("form factors %.1f ms" % lin.from_scene(e), flush=True)
+    lin.set_tile(192, "f16", 4, 2)
+    print("gemm 192 f16 %.3f ms" % lin.gemm_ms(192, "f16", reps=5), flush=True)
+    lin.close()
+    e.close()
+
+
 def main():
     if "--linear" in sys.argv:
         linear()
+    elif "--gemm" in sys.argv:
+        gemm()
     else:
         s = scenes.hall_scene(2)
         e = radgpu.Engine("-converge", "1200", "-triangle", "1000000", "-quiet", "1")
