@@ -49,21 +49,24 @@ __global__ __launch_bounds__(kB) void k_gemm(const E *__restrict__ A, const E *_
 
 // Wide right-hand sides (CT = 8 or 12 column tiles): the B fragments of
 // each group of U k-blocks are loaded once per block, cooperatively, into LDS
-// (double-buffered) and read there by the four waves, so the B operand costs
+// (double-buffered) and read there by the W waves, so the B operand costs
 // one L2 read per block instead of one per wave.  A wave owns 16*RT rows;
 // its A fragments for the next group stream from HBM while the current group
-// feeds the MFMAs.  Needs (k-blocks per chunk) % (2U) == 0 (PER is whole
-// for U * CT a multiple of 4).
-template <typename E, int RT, int CT>
-__global__ __launch_bounds__(kB) void k_gemm_lds(const E *__restrict__ A, const E *__restrict__ XT,
-                                                 float *__restrict__ part, int ld, int rp, int kchunk)
+// feeds the MFMAs.  W = 8 with RT = 2 keeps two waves on each SIMD, so one
+// computes while the other waits.  Needs (k-blocks per chunk) % (2U) == 0
+// (PER is whole for U * CT a multiple of W).
+template <typename E, int RT, int CT, int W>
+__global__ __launch_bounds__(64 * W) void k_gemm_lds(const E *__restrict__ A, const E *__restrict__ XT,
+                                                     float *__restrict__ part, int ld, int rp, int kchunk)
 {
 	typedef typename VecOf<E>::type V;
-	constexpr int U = CT >= 12 ? 2 : 4;     // k-blocks per group (registers: 4 x 12 accumulators)
-	constexpr int PER = U * CT * 64 / kB;  // B fragments (16 bytes) per thread per group
+	constexpr int NT = 64 * W;
+	constexpr int U = CT >= 12 ? 2 : 4;  // k-blocks per group (registers: RT x CT accumulators)
+	constexpr int PER = U * CT * 64 / NT;     // B fragments (16 bytes) per thread per group
+	static_assert(U * CT * 64 % NT == 0, "B fragments split evenly over the block");
 	__shared__ V sb[2][U * CT * 64];
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-	const int rb0 = (blockIdx.x * 4 + w) * RT;
+	const int rb0 = (blockIdx.x * W + w) * RT;
 	const int s = blockIdx.y, kb0 = s * (kchunk >> 5), kb1 = kb0 + (kchunk >> 5);
 	const int nkb = ld >> 5;
 	const V *Ap = (const V *)A + lane;
@@ -78,14 +81,14 @@ __global__ __launch_bounds__(kB) void k_gemm_lds(const E *__restrict__ A, const 
 	auto load_b = [&](int kb) {
 #pragma unroll
 		for (int m = 0; m < PER; m++) {
-			const int q = threadIdx.x + m * kB;
+			const int q = threadIdx.x + m * NT;
 			const int l2 = q & 63, uc = q >> 6, u = uc / CT, c = uc % CT;
 			bs[m] = Bg[((size_t)c * nkb + kb + u) * 64 + l2];
 		}
 	};
 	auto store_b = [&](int buf) {
 #pragma unroll
-		for (int m = 0; m < PER; m++) sb[buf][threadIdx.x + m * kB] = bs[m];
+		for (int m = 0; m < PER; m++) sb[buf][threadIdx.x + m * NT] = bs[m];
 	};
 	auto load_a = [&](V (&a)[U][RT], int kb) {
 #pragma unroll
@@ -449,6 +452,10 @@ int launch_gemm(int rt, int ct, const E *A, const E *XT, float *part, int ld, in
 	}
 	if (rt == 1) {
 		GR(1)
+	} else if (rt == 2 && ct == 8) {
+		hipLaunchKernelGGL((k_gemm_lds<E, 2, 8, 8>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
+	} else if (rt == 2 && ct == 12) {
+		hipLaunchKernelGGL((k_gemm_lds<E, 2, 12, 8>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
 	} else if (rt == 2) {
 		GR(2)
 	} else if (rt == 4 && ct <= 4) {
@@ -459,9 +466,9 @@ int launch_gemm(int rt, int ct, const E *A, const E *XT, float *part, int ld, in
 		default: return -1;
 		}
 	} else if (rt == 4 && ct == 8) {
-		hipLaunchKernelGGL((k_gemm_lds<E, 4, 8>), grid, blk, 0, st, A, XT, part, ld, rp, kchunk);
+		hipLaunchKernelGGL((k_gemm_lds<E, 4, 8, 4>), grid, blk, 0, st, A, XT, part, ld, rp, kchunk);
 	} else if (rt == 4 && ct == 12) {
-		hipLaunchKernelGGL((k_gemm_lds<E, 4, 12>), grid, blk, 0, st, A, XT, part, ld, rp, kchunk);
+		hipLaunchKernelGGL((k_gemm_lds<E, 4, 12, 4>), grid, blk, 0, st, A, XT, part, ld, rp, kchunk);
 	} else {
 		return -1;
 	}
@@ -473,9 +480,12 @@ int launch_gemm(int rt, int ct, const E *A, const E *XT, float *part, int ld, in
 bool valid_tile(int rt, int ct, int S, int ld)
 {
 	// k-blocks per pipelined loop trip: 2 * GemmUnroll<ct>::U, or 2 * U of
-	// the LDS kernel (rt = 4 with 8 or 12 column tiles: U = 4 or 2)
-	const int u2 = (rt == 4 && ct >= 8) ? (ct >= 12 ? 4 : 8) : 2 * (ct <= 4 ? 4 : 2);
-	return ld % (64 * rt) == 0 && ld % (32 * S) == 0 && (ld / 32 / S) % u2 == 0;
+	// the LDS kernels (8 or 12 column tiles: U = 4 or 2); rows per block:
+	// 64 rt, or 256 for the eight-wave LDS kernel (rt = 2)
+	const bool lds = ct >= 8 && (rt == 4 || rt == 2);
+	const int u2 = lds ? (ct >= 12 ? 4 : 8) : 2 * (ct <= 4 ? 4 : 2);
+	const int rows = (lds && rt == 2) ? 256 : 64 * rt;
+	return ld % rows == 0 && ld % (32 * S) == 0 && (ld / 32 / S) % u2 == 0;
 }
 
 double power_sum(const std::vector<float> &a, const double *v, int n, int nrhs)

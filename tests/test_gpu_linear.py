@@ -124,7 +124,7 @@ def test_mfma_product_matches_fp64_of_rounded_operands(nrhs):
         X16 = X.astype(np.float32).astype(np.float16).astype(np.float64)
         for prec, Fr, Xr, sc in (("f16", F16, X16, s), ("bf16", _to_bf16(F), _to_bf16(X.astype(np.float32)), 1.0)):
             want = (Fr @ Xr) / sc
-            for rt, sk in ((1, 1), (2, 2), (4, 1), (4, 2), (1, 8)):
+            for rt, sk in ((1, 1), (2, 1), (2, 2), (4, 1), (4, 2), (1, 8)):
                 try:
                     lin.set_tile(nrhs, prec, rt, sk)
                 except radgpu.RadError:
