@@ -981,6 +981,42 @@ __global__ __launch_bounds__(kBlock) void k_ff_seg(DRecv R, int n, int ld, int n
 	}
 }
 
+// The prepared scene's slot records (the 236-byte reference layout, copied
+// to the device as they are) gathered into the device layouts: k_pack_items
+// fills the octree's list entries with the originals inline, k_pack_recv the
+// receivers' arrays.  Pure data movement.
+__global__ __launch_bounds__(kBlock) void k_pack_items(const Patch *__restrict__ pool, const int *__restrict__ items,
+                                                       int ni, float4 *lnc, float4 *lse)
+{
+	const int k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= ni) return;
+	const int id = items[k];
+	const Patch &p = pool[id];
+	lnc[2 * (size_t)k] = make_float4(p.n[0], p.n[1], p.n[2], p.c[0]);
+	lnc[2 * (size_t)k + 1] = make_float4(p.c[1], p.c[2], __int_as_float(id), 0.f);
+	for (int q = 0; q < 3; q++) lse[3 * (size_t)k + q] = make_float4(p.se[q][0], p.se[q][1], p.se[q][2], p.se[q][3]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_recv(const Patch *__restrict__ pool, int n, float *vx, float *vy,
+                                                      float *vz, float4 *nrm, float4 *cA, float4 *rho, float4 *acc,
+                                                      float4 *dacc, float4 *dB)
+{
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) return;
+	const Patch &p = pool[i];
+	for (int v = 0; v < 3; v++) {
+		vx[3 * i + v] = p.p[v][0];
+		vy[3 * i + v] = p.p[v][1];
+		vz[3 * i + v] = p.p[v][2];
+		acc[3 * i + v] = make_float4(p.accB[v][0], p.accB[v][1], p.accB[v][2], 0.f);
+		dacc[3 * i + v] = make_float4(p.deltaaccB[v][0], p.deltaaccB[v][1], p.deltaaccB[v][2], 0.f);
+	}
+	nrm[i] = make_float4(p.n[0], p.n[1], p.n[2], __int_as_float(p.parent));
+	cA[i] = make_float4(p.c[0], p.c[1], p.c[2], p.area);
+	rho[i] = make_float4((float)p.Frgb[0], (float)p.Frgb[1], (float)p.Frgb[2], 0.f);
+	dB[i] = make_float4(p.deltaB[0], p.deltaB[1], p.deltaB[2], 0.f);
+}
+
 template <bool F>
 __global__ __launch_bounds__(kBlock) void k_cast(DScene S, const RayIn *rays, int n, int *hit, int *cells, int *tests)
 {
@@ -1086,6 +1122,8 @@ struct DeviceEngine::Impl {
 	DevBuf<float> ffbuf;
 	DevBuf<int> raycnt, rays;  // per-k_gen-block ray counts, compacted ray list
 	DevBuf<int> order;         // receiver processing order (DRecv::order)
+	DevBuf<Patch> pool;        // the uploaded slot records (packed by k_pack_items / k_pack_recv)
+	DevBuf<int> items;         // the octree's list entries (original slots)
 	DScene S;
 	DRecv R;
 	Loop *host_loop = nullptr;  // pinned mirror of the loop header
@@ -1166,8 +1204,6 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 			err = "octree item out of range";
 			return -1;
 		}
-	std::vector<float4> lnc(ni * 2), lse(ni * 3);
-	pack_leaf_items(t.items.data(), ni, s.pool.data(), (float *)lnc.data(), (float *)lse.data());
 	for (int l = 0; l < t.n_leaves; l++) {
 		int begin, cd;
 		std::memcpy(&begin, &t.leaf_box[(size_t)l * 8 + 3], 4);
@@ -1182,23 +1218,26 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 			err = "octree cell table entry out of range";
 			return -1;
 		}
-	std::vector<float> vx((size_t)n * 3), vy((size_t)n * 3), vz((size_t)n * 3);
-	std::vector<float4> nrm(n), cA(n), rho(n), acc((size_t)n * 3), dacc((size_t)n * 3), dB(n);
-	pack_receivers(s.pool.data(), n, vx.data(), vy.data(), vz.data(), (float *)nrm.data(), (float *)cA.data(),
-	               (float *)rho.data(), (float *)acc.data(), (float *)dacc.data(), (float *)dB.data());
 	std::vector<int> order(n);
 	spatial_order(s.pool.data(), n, s.G0, s.G1, order.data());
 	hipStream_t st = d.stream;
 	if (d.order.put(order.data(), order.size(), st, err)) return -1;
-	if (d.lnc.put(lnc.data(), lnc.size(), st, err) || d.lse.put(lse.data(), lse.size(), st, err) ||
+	// the slot records as they are, then gathered on the device
+	if (d.pool.put(s.pool.data(), (size_t)n, st, err) || d.items.put(t.items.data(), ni, st, err) ||
+	    d.lnc.alloc(ni * 2, err) || d.lse.alloc(ni * 3, err) ||
 	    d.cell_leaf.put(t.cell_leaf.data(), t.cell_leaf.size(), st, err) ||
 	    d.leaf_box.put((const float4 *)t.leaf_box.data(), t.leaf_box.size() / 4, st, err) ||
-	    d.vx.put(vx.data(), vx.size(), st, err) || d.vy.put(vy.data(), vy.size(), st, err) ||
-	    d.vz.put(vz.data(), vz.size(), st, err) || d.nrm.put(nrm.data(), nrm.size(), st, err) ||
-	    d.cA.put(cA.data(), cA.size(), st, err) || d.rho.put(rho.data(), rho.size(), st, err) ||
-	    d.acc.put(acc.data(), acc.size(), st, err) || d.dacc.put(dacc.data(), dacc.size(), st, err) ||
-	    d.dB.put(dB.data(), dB.size(), st, err))
+	    d.vx.alloc((size_t)n * 3, err) || d.vy.alloc((size_t)n * 3, err) || d.vz.alloc((size_t)n * 3, err) ||
+	    d.nrm.alloc(n, err) || d.cA.alloc(n, err) || d.rho.alloc(n, err) || d.acc.alloc((size_t)n * 3, err) ||
+	    d.dacc.alloc((size_t)n * 3, err) || d.dB.alloc(n, err))
 		return -1;
+	if (ni > 0)
+		hipLaunchKernelGGL(k_pack_items, dim3((unsigned)((ni + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d.pool.p,
+		                   d.items.p, (int)ni, d.lnc.p, d.lse.p);
+	if (n > 0)
+		hipLaunchKernelGGL(k_pack_recv, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, d.pool.p, n, d.vx.p,
+		                   d.vy.p, d.vz.p, d.nrm.p, d.cA.p, d.rho.p, d.acc.p, d.dacc.p, d.dB.p);
+	HIP_OK(hipGetLastError(), "pack launch");
 	// grid sizes first: every kernel that adds into bcnt indexes it by its
 	// hardware block index, so bcnt must cover the largest grid
 	d.nblocks = (n + kRecvPerBlock - 1) / kRecvPerBlock;
