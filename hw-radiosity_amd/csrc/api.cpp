@@ -10,6 +10,7 @@
 #include "../../include/radiosity_gpu.h"
 #include "device.h"
 #include "fin.h"
+#include "viewer_json.h"
 #include "batch.h"
 #include "linear.h"
 #include "octree.h"
@@ -222,6 +223,19 @@ rad_status rad_write_fin(rad_ctx *c, const char *path)
 	rad_status st = sync_host(c);
 	if (st != RAD_OK) return st;
 	if (c->writer.write(path, c->scene.pool.data(), c->scene.trinum, c->scene.o.old_format != 0, c->err)) return RAD_EIO;
+	return RAD_OK;
+}
+
+rad_status rad_fin_to_json(rad_ctx *c, const char *fin_path, const char *json_path, int *triangles)
+{
+	if (!c || !fin_path || !json_path) return RAD_EINVAL;
+	int n = 0, tiny = 0;
+	if (fin_to_json(fin_path, json_path, n, tiny, c->err)) return RAD_EIO;
+	if (!c->scene.o.quiet) {
+		for (int k = 0; k < tiny; k++) std::fprintf(stderr, "[Warning] Triangle area is too small to show\n");
+		std::fprintf(stderr, "#Triangle %d\nModel-Converter Success\n", n);
+	}
+	if (triangles) *triangles = n;
 	return RAD_OK;
 }
 

@@ -35,7 +35,7 @@ assert RECORD_DTYPE.itemsize == 236
 RAY_DTYPE = np.dtype([("p", "<f4", 3), ("v", "<f4", 3), ("ban", "<i4"), ("target", "<i4")])
 
 EXPORTS = ("rad_create", "rad_destroy", "rad_last_error", "rad_load_tri", "rad_set_triangles",
-           "rad_prepare_host", "rad_upload", "rad_shoot", "rad_run", "rad_write_fin", "rad_records",
+           "rad_prepare_host", "rad_upload", "rad_shoot", "rad_run", "rad_write_fin", "rad_fin_to_json", "rad_records",
            "rad_tree_info", "rad_cast", "rad_ff_pairs", "rad_counters", "rad_timings",
            "rad_lin_create", "rad_lin_destroy", "rad_lin_last_error", "rad_lin_from_scene",
            "rad_lin_set_matrix", "rad_lin_get_matrix", "rad_lin_set_areas", "rad_lin_autotune", "rad_lin_set_tile",
@@ -76,6 +76,7 @@ def library() -> ctypes.CDLL:
             "rad_shoot": [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip)],
             "rad_run": [vp, cp, ctypes.POINTER(ip)],
             "rad_write_fin": [vp, cp],
+            "rad_fin_to_json": [vp, cp, cp, ctypes.POINTER(ip)],
             "rad_records": [vp, ctypes.POINTER(ip), vp, vp],
             "rad_tree_info": [vp, vp, ctypes.POINTER(ctypes.c_float)],
             "rad_cast": [vp, ip, vp, vp, vp, vp],
@@ -179,6 +180,13 @@ class Engine:
 
     def write_fin(self, path: str):
         self._ok(library().rad_write_fin(self.h, path.encode()))
+
+    def fin_to_json(self, fin_path: str, json_path: str) -> int:
+        """The viewer's model file from a .fin (the reference's tri2json);
+        returns the number of triangles converted."""
+        n = ctypes.c_int(0)
+        self._ok(library().rad_fin_to_json(self.h, fin_path.encode(), json_path.encode(), ctypes.byref(n)))
+        return n.value
 
     # ---- state / checks ------------------------------------------------
     def records(self):

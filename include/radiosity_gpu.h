@@ -85,6 +85,16 @@ rad_status rad_run(rad_ctx *ctx, const char *prefix, int *iterations);
 /* PrintOut (rad.cpp:117-242) of the current state to `path`. */
 rad_status rad_write_fin(rad_ctx *ctx, const char *path);
 
+/* The WebGL viewer's model file from a .fin, byte-identical to the
+ * reference converter's (`tri2json -i fin -o json`,
+ * WebGL_demo/model-converter/main.cpp:81-246): vertices moved into a cube
+ * of side 10 centred on the origin, normals, colours / 255, indices.  Needs
+ * no scene; ctx only carries the error text and -quiet (otherwise the
+ * converter's "#Triangle n" and success lines go to stderr).  *triangles
+ * (may be NULL) = records converted.  RAD_EIO for an unreadable .fin (the
+ * reference aborts) or an unwritable output. */
+rad_status rad_fin_to_json(rad_ctx *ctx, const char *fin_path, const char *json_path, int *triangles);
+
 /* State in the reference's 236-byte Triangle layout (rad.h:25-39), slots
  * [0, *trinum); records may be NULL to query *trinum.  g6 (may be NULL)
  * receives G0 and G1 (rad.cpp:105-110). */

@@ -12,6 +12,7 @@
 # (a dump harness), renaming the reference's main() with -Dmain=radref_main.
 # rad_oldfmt is the shipped build with the reference's own OLD_FORMAT switch
 # (rad.cpp:150) defined on the command line: the per-vertex colour output.
+# tri2json is the WebGL viewer's model converter (WebGL_demo/model-converter/main.cpp).
 
 REF     ?= /root/reference
 HERE    := $(patsubst %/,%,$(dir $(lastword $(MAKEFILE_LIST))))
@@ -25,7 +26,7 @@ SRCS    := $(REF)/src/shade.cpp $(REF)/src/raycast.cpp $(REF)/src/rad.cpp
 HDRS    := $(wildcard $(REF)/include/*.h)
 
 .PHONY: all clean
-all: $(OUT)/rad $(OUT)/rad_nofma $(OUT)/rad_oldfmt $(OUT)/refdump $(OUT)/refdump_nofma
+all: $(OUT)/rad $(OUT)/rad_nofma $(OUT)/rad_oldfmt $(OUT)/refdump $(OUT)/refdump_nofma $(OUT)/tri2json
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -50,6 +51,10 @@ $(OUT)/refdump: $(HERE)/refdump.cpp $(OUT)/shade.fma.o $(OUT)/raycast.fma.o $(OU
 
 $(OUT)/refdump_nofma: $(HERE)/refdump.cpp $(OUT)/shade.nofma.o $(OUT)/raycast.nofma.o $(OUT)/rad.nofma.o
 	$(CXX) $(NOFMA_FL) $^ -o $@ -lm
+
+# the WebGL viewer's model converter, one source file (no build file ships with it)
+$(OUT)/tri2json: $(REF)/WebGL_demo/model-converter/main.cpp | $(OUT)
+	$(CXX) -O2 -std=c++11 -w $< -o $@
 
 clean:
 	rm -rf $(OUT)

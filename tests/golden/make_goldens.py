@@ -14,6 +14,11 @@ oracle/_ref/refdump_nofma (-ffp-contract=off); see oracle/ref.mk.
 
 adds only the .fin md5s of oracle/_ref/rad_oldfmt (the shipped build with
 the reference's OLD_FORMAT switch defined) to goldens.json.
+
+    python tests/golden/make_goldens.py --viewer-json
+
+adds the md5s of the reference's WebGL model converter (oracle/_ref/tri2json)
+run on the reference's own .fin outputs and on an empty .fin.
 """
 from __future__ import annotations
 
@@ -202,6 +207,41 @@ def old_format():
     shutil.rmtree(tmp)
 
 
+VIEWER_JSON_CASES = [("c1", "c1_seed1.tri", C1_OPTS), ("room_t5000", "room.tri", ["-triangle", "5000"])]
+
+
+def viewer_json():
+    """The reference's WebGL model converter (oracle/_ref/tri2json) on the
+    reference's own .fin outputs, and on an empty .fin."""
+    path = os.path.join(HERE, "goldens.json")
+    with open(path) as fh:
+        out = json.load(fh)
+    tmp = tempfile.mkdtemp(prefix="goldens_")
+    c1 = os.path.join(tmp, "c1_seed1.tri")
+    scenes.write_tri(c1, scenes.room_scene(1))
+    inputs = {"c1_seed1.tri": c1, "room.tri": os.path.join(REF_MODELS, "room.tri")}
+    rad = os.path.join(ROOT, "oracle", "_ref", "rad")
+    conv = os.path.join(ROOT, "oracle", "_ref", "tri2json")
+    res = {}
+    for case, model, opts in VIEWER_JSON_CASES + [("empty", None, None)]:
+        prefix = os.path.join(tmp, case)
+        if model is None:
+            open(prefix + ".fin", "wb").close()
+        else:
+            subprocess.run([rad] + opts + ["-jobs", "1", "-write_cycle", "1000000", model_path(inputs, model), "-o", prefix],
+                           capture_output=True, text=True, check=True, timeout=600)
+        r = subprocess.run([conv, "-i", prefix + ".fin", "-o", prefix + ".json"], capture_output=True, text=True,
+                           check=True, timeout=600)
+        js = open(prefix + ".json", "rb").read()
+        res[case] = {"input": model, "opts": opts, "fin_md5": hashlib.md5(open(prefix + ".fin", "rb").read()).hexdigest(),
+                     "json_md5": hashlib.md5(js).hexdigest(), "json_bytes": len(js), "stderr": r.stderr}
+        print("viewer_json", case, res[case]["json_md5"], res[case]["json_bytes"], flush=True)
+    out["viewer_json"] = res
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+    shutil.rmtree(tmp)
+
+
 def model_path(inputs, model):
     return inputs[model]
 
@@ -209,5 +249,7 @@ def model_path(inputs, model):
 if __name__ == "__main__":
     if "--old-format" in sys.argv:
         old_format()
+    elif "--viewer-json" in sys.argv:
+        viewer_json()
     else:
         main()

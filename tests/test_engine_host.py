@@ -227,3 +227,68 @@ def test_edge_scene_host_stages_match_oracle(name):
     assert a[:3] == b[:3] and struct.pack("f", a[3]) == struct.pack("f", b[3])
     assert len(rec) == len(orec) and rec.tobytes() == orec.tobytes()
     assert g.tobytes() == og.tobytes()
+
+
+def _ref_converter():
+    exe = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "tri2json")
+    return exe if os.path.exists(exe) else None
+
+
+@pytest.mark.parametrize("case", ["c1", "room_t5000", "empty"])
+def test_viewer_json_matches_reference_converter(case, goldens, inputs, tmp_path):
+    """tri2json's JSON from the oracle's .fin (itself byte-identical to the
+    reference's, checked by md5) equals the reference converter's output
+    (golden md5 from the reference's own .fin)."""
+    g = goldens["viewer_json"][case]
+    fin = str(tmp_path / "x.fin")
+    if g["input"] is None:
+        open(fin, "wb").close()
+    else:
+        o = ol.Oracle(*(g["opts"] + ["-quiet", "1", "-jobs", "8"]))
+        o.load_file(inputs[g["input"]])
+        o.build_tree()
+        o.init_radiosity()
+        o.prepartition()
+        o.iterate(1 << 30)
+        o.write_fin(fin)
+    assert hashlib.md5(open(fin, "rb").read()).hexdigest() == g["fin_md5"]
+    with radgpu.Engine("-quiet", "1") as eng:
+        n = eng.fin_to_json(fin, str(tmp_path / "e.json"))
+    js = open(tmp_path / "e.json", "rb").read()
+    assert n == int(g["stderr"].split()[1])
+    assert len(js) == g["json_bytes"] and hashlib.md5(js).hexdigest() == g["json_md5"]
+    ref = _ref_converter()
+    if ref:  # the reference converter itself, where it is built
+        import subprocess
+        subprocess.run([ref, "-i", fin, "-o", str(tmp_path / "r.json")], check=True, capture_output=True, timeout=120)
+        assert open(tmp_path / "r.json", "rb").read() == js
+
+
+def test_tri2json_cli_matches_reference_messages(goldens, inputs, tmp_path):
+    """The command-line converter: same output file and stderr lines as the
+    reference tool, usage errors with status 1, unreadable records rejected."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(radgpu.LIB_PATH), "tri2json")
+    fin = str(tmp_path / "x.fin")
+    o = ol.Oracle("-quiet", "1", "-triangle", "20000")
+    o.load_file(inputs["c1_seed1.tri"])
+    o.init_radiosity()
+    o.prepartition()
+    o.write_fin(fin)
+    r = subprocess.run([exe, "-i", fin, "-o", str(tmp_path / "a.json"), "--format", "COLOR"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0 and r.stderr.endswith("Model-Converter Success\n")
+    with radgpu.Engine("-quiet", "1") as eng:
+        eng.fin_to_json(fin, str(tmp_path / "b.json"))
+    assert open(tmp_path / "a.json", "rb").read() == open(tmp_path / "b.json", "rb").read()
+    assert subprocess.run([exe, "--help"], capture_output=True, timeout=60).returncode == 0
+    assert subprocess.run([exe, "-i", fin], capture_output=True, timeout=60).returncode == 1
+    assert subprocess.run([exe, "-i", fin, "-o", str(tmp_path / "c.json"), "--format", "SIMPLE"],
+                          capture_output=True, timeout=60).returncode == 1
+    bad = tmp_path / "bad.fin"
+    bad.write_bytes(b"Triangle \n1 2 3 0 0 0\n0 0 0 0 0 1\n1 0 0 0 0 1\n")  # cut short
+    assert subprocess.run([exe, "-i", str(bad), "-o", str(tmp_path / "d.json")], capture_output=True,
+                          timeout=60).returncode == 1
+    with radgpu.Engine("-quiet", "1") as eng:
+        with pytest.raises(radgpu.RadError):
+            eng.fin_to_json(str(bad), str(tmp_path / "e.json"))

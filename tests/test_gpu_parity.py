@@ -103,6 +103,14 @@ def test_cli_drop_in(goldens, inputs, tmp_path):
     # snapshots every 10 iterations, as -write_cycle asks
     assert sorted(p.name for p in tmp_path.iterdir() if p.name.startswith("c1.") and p.name != "c1.fin") == \
         ["c1.10", "c1.20"]
+    # the viewer's model file from the engine's .fin, as the reference converter writes it
+    import subprocess
+    v = goldens["viewer_json"]["c1"]
+    exe = os.path.join(os.path.dirname(radgpu.LIB_PATH), "tri2json")
+    r = subprocess.run([exe, "-i", str(tmp_path / "c1.fin"), "-o", str(tmp_path / "c1.json")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0 and r.stderr == v["stderr"]
+    assert hashlib.md5(open(tmp_path / "c1.json", "rb").read()).hexdigest() == v["json_md5"]
 
 
 @pytest.mark.parametrize("case", ["c1", "room_t5000", "box_t2000"])
