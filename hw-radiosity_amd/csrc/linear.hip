@@ -167,6 +167,82 @@ __global__ __launch_bounds__(kB) void k_fx64m(const float *__restrict__ F, const
 	                  s * kchunk, (s + 1) * kchunk);
 }
 
+// The same FP64 product for at most 4 right-hand sides on the vector FP64
+// units, where a 16-column MFMA would spend 13 of its 16 columns on padding:
+// wave w of block b owns the kFxRows rows from kFxRows (4 b + w) over the whole K; per
+// step each lane takes 4 consecutive k of each row (one 16-byte load per
+// row, the next step's loads in flight) against x staged through LDS in
+// chunks of kFxChunk k (1024 or 256; 4 columns, FP64), then a butterfly
+// sums the lanes.
+// Writes part[row][c] (S = 1), c < 4; columns >= nrhs of xT are zero.
+constexpr int kFxRows = 8;
+template <int kFxChunk>
+__global__ __launch_bounds__(kB) void k_fx64v(const float *__restrict__ F, const double *__restrict__ xT,
+                                              double *__restrict__ part, int ld, int rp)
+{
+	typedef float fv4 __attribute__((ext_vector_type(4)));
+	__shared__ double xs[kFxChunk * 4];
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const int r0 = (blockIdx.x * 4 + w) * kFxRows;
+	double acc[kFxRows][4];
+#pragma unroll
+	for (int r = 0; r < kFxRows; r++)
+#pragma unroll
+		for (int c = 0; c < 4; c++) acc[r][c] = 0.0;
+	const float *Fr = F + (size_t)r0 * ld + 4 * lane;
+	fv4 a[kFxRows], an[kFxRows];
+#pragma unroll
+	for (int r = 0; r < kFxRows; r++) a[r] = __builtin_nontemporal_load((const fv4 *)(Fr + (size_t)r * ld));
+	for (int k0 = 0; k0 < ld; k0 += kFxChunk) {
+		__syncthreads();  // the previous chunk's reads are done
+		for (int i = threadIdx.x; i < kFxChunk * 4; i += kB) {
+			const int c = i / kFxChunk, kk = i % kFxChunk;
+			xs[kk * 4 + c] = xT[(size_t)c * ld + k0 + kk];
+		}
+		__syncthreads();
+#pragma unroll 1
+		for (int st = 0; st < kFxChunk; st += 256) {
+			const int k = k0 + st;
+			if (k + 256 < ld) {
+#pragma unroll
+				for (int r = 0; r < kFxRows; r++)
+					an[r] = __builtin_nontemporal_load((const fv4 *)(Fr + (size_t)r * ld + k + 256));
+			}
+			const double *xl = xs + (st + 4 * lane) * 4;
+#pragma unroll
+			for (int j = 0; j < 4; j++) {
+				const double x0 = xl[4 * j], x1 = xl[4 * j + 1], x2 = xl[4 * j + 2], x3 = xl[4 * j + 3];
+#pragma unroll
+				for (int r = 0; r < kFxRows; r++) {
+					const double f = (double)a[r][j];
+					acc[r][0] = __builtin_fma(f, x0, acc[r][0]);
+					acc[r][1] = __builtin_fma(f, x1, acc[r][1]);
+					acc[r][2] = __builtin_fma(f, x2, acc[r][2]);
+					acc[r][3] = __builtin_fma(f, x3, acc[r][3]);
+				}
+			}
+#pragma unroll
+			for (int r = 0; r < kFxRows; r++) a[r] = an[r];
+		}
+	}
+#pragma unroll
+	for (int r = 0; r < kFxRows; r++)
+#pragma unroll
+		for (int c = 0; c < 4; c++) {
+			double v = acc[r][c];
+			for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+			acc[r][c] = v;
+		}
+	if (lane < kFxRows * 4) {
+		const int r = lane >> 2, c = lane & 3;
+		double v = 0.0;
+#pragma unroll
+		for (int q = 0; q < kFxRows * 4; q++)
+			if (q == lane) v = acc[q >> 2][q & 3];
+		part[(size_t)(r0 + r) * rp + c] = v;
+	}
+}
+
 // out[i][c] = base[i][c] + rho[i][c] * y - beta * xT[c][i], y = the split-K
 // sum of part (FP64, fixed order); base, rho may be null (0 and 1).  Per
 // block: sum of squares and max |out|.
@@ -517,19 +593,27 @@ struct Fx64 {
 		rp = (nct + ct - 1) / ct * ct * 16;  // whole column groups: every tile the grid covers exists
 		const int tiles = (ld / (64 * rt)) * (rp / (16 * ct));
 		S = 1;
-		while (S < 8 && tiles * S < 512 && (ld / (2 * S)) % 32 == 0) S *= 2;
+		if (!vec())
+			while (S < 8 && tiles * S < 512 && (ld / (2 * S)) % 32 == 0) S *= 2;
 		nblk = ((size_t)n * nrhs + kB - 1) / kB;
 		if (part.alloc((size_t)S * ld * rp, err) || blk.alloc(std::max<size_t>(nblk, 8192) * 2, err)) return -1;
 		hb.assign(nblk * 2, 0.0);
 		return 0;
 	}
+	// at most 4 columns: the vector-unit kernel (ld is a multiple of 256:
+	// whole chunks of 1024 or 256 k, whole blocks of 4 kFxRows rows)
+	bool vec() const { return nrhs <= 4 && rp >= 4 && ld % 256 == 0 && ld % (4 * kFxRows) == 0; }
 	// out = base + rho o (F x) - beta x; returns (sum of squares, max |out|)
 	int run(const float *F, const double *xT, const double *base, const double *rho, double beta, double *out,
 	        hipStream_t st, double &ss, double &mx, bool want_stats, std::string &err)
 	{
 		const dim3 grid(ld / (64 * rt), rp / (16 * ct), S);
 		const int kchunk = ld / S;
-		if (ct == 4) hipLaunchKernelGGL((k_fx64m<2, 4>), grid, dim3(kB), 0, st, F, xT, part.p, ld, rp, kchunk);
+		if (vec() && ld % 1024 == 0)
+			hipLaunchKernelGGL(k_fx64v<1024>, dim3(ld / (4 * kFxRows)), dim3(kB), 0, st, F, xT, part.p, ld, rp);
+		else if (vec())
+			hipLaunchKernelGGL(k_fx64v<256>, dim3(ld / (4 * kFxRows)), dim3(kB), 0, st, F, xT, part.p, ld, rp);
+		else if (ct == 4) hipLaunchKernelGGL((k_fx64m<2, 4>), grid, dim3(kB), 0, st, F, xT, part.p, ld, rp, kchunk);
 		else if (ct == 2) hipLaunchKernelGGL((k_fx64m<4, 2>), grid, dim3(kB), 0, st, F, xT, part.p, ld, rp, kchunk);
 		else hipLaunchKernelGGL((k_fx64m<4, 1>), grid, dim3(kB), 0, st, F, xT, part.p, ld, rp, kchunk);
 		hipLaunchKernelGGL(k_resid, dim3((unsigned)nblk), dim3(kB), 0, st, part.p, S, ld, rp, n, nrhs, xT, base, rho,
