@@ -30,6 +30,19 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def _random_system(n, seed, cols, scale=0.9):
+    """A float32 F with zero diagonal and rows summing to `scale`, rho in
+    [0.2, 0.8], sparse emission with row 0 lit in every column."""
+    rng = np.random.default_rng(seed)
+    F = (rng.random((n, n)) * (rng.random((n, n)) < 0.7)).astype(np.float32)
+    np.fill_diagonal(F, 0.0)
+    F *= np.float32(scale) / F.sum(axis=1, keepdims=True).astype(np.float32)
+    rho = rng.uniform(0.2, 0.8, (n, cols))
+    E = np.where(rng.random((n, cols)) < 0.1, rng.uniform(0.5, 2.0, (n, cols)), 0.0)
+    E[0] = 1.0
+    return F, rho, E
+
+
 def scene_system(n, seed=1, vis="segment"):
     """(engine, LinearSystem, scene, E, rho, area) for patch_room(seed, n)."""
     print("scene patch_room(%d, %d) vis=%s" % (seed, n, vis), flush=True)
@@ -165,6 +178,24 @@ def test_gather_192_rhs_matches_dense_lu(sys2k):
     # one column at a time gives the same solution
     B1, _ = lin.gather(rho192[:, 5:6], E192[:, 5:6], prec="f16", tol=1e-10, max_outer=60, sweeps=6)
     assert rel(B1[:, 0], ref[:, 5]) < 1e-8
+
+
+@pytest.mark.parametrize("nrhs", [1, 3, 4, 5, 17])
+def test_gather_column_counts_on_ragged_size(nrhs):
+    """n = 700 (padded to 768, not a multiple of 1024) with 1-17 columns:
+    the FP64 residual takes the vector-unit kernel in 256-k chunks for at most
+    4 columns and the FP64 MFMA kernel above; both reach the dense solve."""
+    n = 700
+    F, rho, E = _random_system(n, 100 + nrhs, nrhs)
+    print("gather n=%d nrhs=%d" % (n, nrhs), flush=True)
+    ref = lo.dense_solve(F.astype(np.float64), rho, E)
+    with radgpu.LinearSystem(n) as lin:
+        lin.set_matrix(F)
+        for prec in ("f16", "bf16"):
+            B, info = lin.gather(rho, E, prec=prec, tol=1e-12, max_outer=60, sweeps=6)
+            assert info["rel_res"] <= 1e-12, (prec, info)
+            assert rel(B, ref) < 1e-10, prec
+            assert np.linalg.norm(lo.residual(F.astype(np.float64), rho, E, B)) <= 1e-11 * np.linalg.norm(E)
 
 
 def test_gather_rejects_bad_arguments(sys2k):
