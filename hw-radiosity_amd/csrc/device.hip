@@ -840,13 +840,23 @@ __device__ bool seg_occluded(const DFine &G, float px, float py, float pz, float
 	c.rays++;
 	int last = -1;
 	for (;;) {
-		const int leaf = G.leaf[((size_t)cell[0] * R + cell[1]) * R + cell[2]];
+		const int leaf = G.leaf[(cell[0] * R + cell[1]) * R + cell[2]];  // R^3 < 2^31
 		if (leaf != last) {
 			last = leaf;
 			c.cells++;
-			const int e1 = G.begin[leaf + 1];
-			for (int e = G.begin[leaf]; e < e1; e++) {
-				const float4 a = G.nc[2 * (size_t)e], b = G.nc[2 * (size_t)e + 1];
+			const int e0 = G.begin[leaf], e1 = G.begin[leaf + 1];
+			// the whole 32-byte record at once, the next one in flight
+			float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nb = na;
+			if (e0 < e1) {
+				na = G.nc[2 * e0];
+				nb = G.nc[2 * e0 + 1];
+			}
+			for (int e = e0; e < e1; e++) {
+				const float4 a = na, b = nb;
+				if (e + 1 < e1) {
+					na = G.nc[2 * e + 2];
+					nb = G.nc[2 * e + 3];
+				}
 				const int id = __float_as_int(b.z);
 				if (id == ban0 || id == ban1) continue;
 				c.tests++;
@@ -855,7 +865,7 @@ __device__ bool seg_occluded(const DFine &G, float px, float py, float pz, float
 				const float tt = a2 / a1;
 				if (!(tt > kSegLo && tt < kSegHi)) continue;
 				const float hx = axpy<F>(tt, vx, px), hy = axpy<F>(tt, vy, py), hz = axpy<F>(tt, vz, pz);
-				const float4 *se = G.se + 3 * (size_t)e;
+				const float4 *se = G.se + 3 * e;
 				const float4 s0 = se[0], s1 = se[1], s2 = se[2];
 				if (!(dot3<F>(hx, hy, hz, s0.x, s0.y, s0.z) + s0.w < 0.0f) &&
 				    !(dot3<F>(hx, hy, hz, s1.x, s1.y, s1.z) + s1.w < 0.0f) &&
@@ -1462,6 +1472,10 @@ int DeviceEngine::form_factors(float *dF, int n, int ld, int vis, FlatOctree &t,
 		}
 		if (build_segment_index(t, s.pool.data(), s.n_orig, s.G0, s.G1, err)) return -1;
 		const size_t ne = t.seg_items.size();
+		if (ne >= ((size_t)1 << 29) || (size_t)t.fine_r * t.fine_r * t.fine_r >= ((size_t)1 << 31)) {
+			err = "segment index too large for 32-bit record and cell indices";  // the walk indexes in int
+			return -1;
+		}
 		std::vector<float4> nc(2 * ne), se(3 * ne);
 		for (size_t e = 0; e < ne; e++) {
 			const Patch &p = s.pool[t.seg_items[e]];
