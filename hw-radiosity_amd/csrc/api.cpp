@@ -99,7 +99,13 @@ rad_status write_snapshot(rad_ctx *c, const std::string &path, bool final_file)
 		// rad.cpp:229-232 prints OK when system() returns non-zero
 		std::puts(std::system(cmd.c_str()) ? "OK" : "Failed");
 	}
-	if (s.o.interactive) s.log("[WARN] -interactive: the WebGL viewer is not part of this engine\n");
+	if (s.o.interactive) {
+		// rad.cpp:234-240: hand the file to the WebGL demo's script (which
+		// converts it with tri2json, main.cpp:248-290) and report its status
+		const std::string cmd = "cd ./WebGL_demo && ./run.sh ../" + path + " 2>/dev/null";
+		const int ret = std::system(cmd.c_str());
+		std::fprintf(stderr, "WebGL: %s, return %d\n", cmd.c_str(), ret);
+	}
 	return RAD_OK;
 }
 

@@ -28,7 +28,7 @@ const char *kUsage =
 	"   Output Options\n"
 	"       -zip                    Compress output files with gzip.\n"
 	"       -o </<path>/file>       Output prefix (default ./test).\n"
-	"       -interactive            Accepted; the WebGL viewer is not bundled.\n"
+	"       -interactive            After each output file run ./WebGL_demo/run.sh on it.\n"
 	"   Engine Options\n"
 	"       -fma 0|1  -select high|low  -max_tri N  -old_format 0|1  -batch N  -quiet 0|1\n";
 

@@ -259,3 +259,39 @@ def test_edge_scenes_match_oracle(name):
     orec, _ = o.dump()
     assert conv and oconv and shots == oshots, (shots, oshots)
     assert rec.tobytes() == orec.tobytes(), describe_diff(rec, orec)
+
+
+def test_cli_zip_and_interactive_match_reference_binary(goldens, inputs, tmp_path):
+    """-zip and -interactive against the reference binary built from its
+    sources (oracle/_ref/rad), each run in its own directory holding a
+    WebGL_demo/run.sh that records its argument: same files (gzip'd .fin
+    with the same content), same script calls, same status lines."""
+    import gzip
+    import subprocess
+    ref = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "rad")
+    if not os.path.exists(ref):
+        pytest.skip("reference binary not built here")
+    opts = ["-converge", "1200", "-triangle", "20000", "-zip", "-interactive", "-write_cycle", "10", "-jobs", "2"]
+    runs = {}
+    for name in ("ref", "eng"):
+        d = tmp_path / name
+        (d / "WebGL_demo").mkdir(parents=True)
+        script = d / "WebGL_demo" / "run.sh"
+        script.write_text("#!/bin/bash\necho \"$1\" >> calls.txt\n")
+        script.chmod(0o755)
+        args = opts + [inputs["c1_seed1.tri"], "-o", "out"]
+        if name == "ref":
+            r = subprocess.run([ref] + args, cwd=d, capture_output=True, text=True, timeout=300)
+        else:
+            r = radgpu.run_cli(args, cwd=str(d), timeout=300)
+        assert r.returncode == 0, (name, r.stderr[-2000:])
+        files = sorted(p.name for p in d.iterdir() if p.name.startswith("out"))
+        content = {f: gzip.open(d / f).read() for f in files}
+        calls = (d / "WebGL_demo" / "calls.txt").read_text().split()
+        status = [ln for ln in (r.stdout + r.stderr).splitlines()
+                  if ln.startswith(("WebGL:", "Compress output file")) or ln in ("OK", "Failed")]
+        runs[name] = (files, content, calls, status)
+    assert runs["eng"][0] == runs["ref"][0] and runs["eng"][0] == ["out.10.gz", "out.20.gz", "out.fin.gz"]
+    assert runs["eng"][1] == runs["ref"][1]
+    assert runs["eng"][2] == runs["ref"][2] == ["../out.10", "../out.20", "../out.fin"]
+    assert sorted(runs["eng"][3]) == sorted(runs["ref"][3])
