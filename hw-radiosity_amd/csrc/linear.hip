@@ -140,7 +140,9 @@ __global__ __launch_bounds__(64 * W) void k_gemm_lds(const E *__restrict__ A, co
 
 // d = rhs + rho o (inv_s * sum_s part); XT <- d (fragment order).
 // Padding of XT (rows >= nrhs, k >= n) is never written and stays zero.
-template <typename E>
+// RHO_IN (the adjoint's sweep d <- r + F^T (rho o d)): v = rhs + inv_s * sum
+// part and XT <- rho o v instead.
+template <typename E, bool RHO_IN = false>
 __global__ void k_jacobi(const float *part, int S, int ld, int rp, int n, int nrhs, float inv_s, const float *rhs,
                          const float *rho, float *d, E *XT)
 {
@@ -149,9 +151,9 @@ __global__ void k_jacobi(const float *part, int S, int ld, int rp, int n, int nr
 		const int i = (int)(t / nrhs), c = (int)(t % nrhs);
 		float y = 0.f;
 		for (int s = 0; s < S; s++) y += part[((size_t)s * ld + i) * rp + c];
-		const float v = rhs[t] + rho[t] * (y * inv_s);
+		const float v = RHO_IN ? rhs[t] + y * inv_s : rhs[t] + rho[t] * (y * inv_s);
 		d[t] = v;
-		XT[frag_index(c, i, ld)] = (E)v;
+		XT[frag_index(c, i, ld)] = (E)(RHO_IN ? rho[t] * v : v);
 	}
 }
 
@@ -348,12 +350,13 @@ __global__ __launch_bounds__(kB) void k_ftx_finish(const T *part, int S, int n, 
 
 // r32 = r64 / s; XT (if given) <- r32 in fragment order
 template <typename E>
-__global__ void k_scale(const double *r64, double inv_s, size_t total, int nrhs, int ld, float *r32, E *XT)
+__global__ void k_scale(const double *r64, double inv_s, size_t total, int nrhs, int ld, float *r32, E *XT,
+                        const float *rho_in = nullptr)
 {
 	for (size_t t = blockIdx.x * (size_t)kB + threadIdx.x; t < total; t += (size_t)gridDim.x * kB) {
 		const float v = (float)(r64[t] * inv_s);
 		r32[t] = v;
-		if (XT) XT[frag_index((int)(t % nrhs), (int)(t / nrhs), ld)] = (E)v;
+		if (XT) XT[frag_index((int)(t % nrhs), (int)(t / nrhs), ld)] = (E)(rho_in ? rho_in[t] * v : v);
 	}
 }
 
@@ -652,11 +655,11 @@ struct LinearSystem::Impl {
 	hipStream_t st = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	Buf<float> F32, Ft, area;
-	Buf<_Float16> F16;
+	Buf<_Float16> F16, FT16;  // FT16: s16 * F^T in fragment order (the adjoint's sweeps)
 	Buf<__bf16> FB16;
 	Buf<unsigned> umax;
 	float s16 = 1.f, sb16 = 1.f;
-	bool have16 = false, haveb16 = false, havet = false;
+	bool have16 = false, haveb16 = false, havet = false, havet16 = false;
 	std::map<std::pair<int, int>, std::pair<int, int>> tiles;  // (ct, prec) -> (rt, S)
 
 	// low-precision fragment copy of F for prec (made once per matrix)
@@ -694,6 +697,26 @@ struct LinearSystem::Impl {
 			return -1;
 		}
 		HIP_OK(hipGetLastError(), "convert");
+		return 0;
+	}
+	// FP32 transpose of F (made once per matrix)
+	int ensure_t(int ld, std::string &err)
+	{
+		if (havet) return 0;
+		if (Ft.alloc((size_t)ld * ld, err)) return -1;
+		hipLaunchKernelGGL(k_transpose, dim3(ld / 32, ld / 32), dim3(kB), 0, st, F32.p, Ft.p, ld);
+		HIP_OK(hipGetLastError(), "transpose");
+		havet = true;
+		return 0;
+	}
+	// s16 * F^T in FP16 fragment order (made once per matrix)
+	int ensure_t16(int ld, std::string &err)
+	{
+		if (havet16) return 0;
+		if (ensure_lowp(PREC_F16, ld, err) || ensure_t(ld, err) || FT16.alloc((size_t)ld * ld, err)) return -1;
+		hipLaunchKernelGGL(k_to_frag<_Float16>, dim3(8192), dim3(kB), 0, st, Ft.p, ld, s16, FT16.p);
+		HIP_OK(hipGetLastError(), "convert");
+		havet16 = true;
 		return 0;
 	}
 };
@@ -748,7 +771,7 @@ LinearSystem *LinearSystem::create(int n, std::string &err)
 
 float *LinearSystem::matrix32()
 {
-	d_->have16 = d_->haveb16 = d_->havet = false;  // the caller may rewrite F
+	d_->have16 = d_->haveb16 = d_->havet = d_->havet16 = false;  // the caller may rewrite F
 	return d_->F32.p;
 }
 
@@ -1146,12 +1169,7 @@ int LinearSystem::shoot(int nrhs, const double *rho, const double *E, const doub
 	const int n = n_, ld = np_;
 	hipStream_t st = d.st;
 	static_assert(sizeof(ShotRec) % 8 == 0, "record alignment");
-	if (!d.havet) {
-		if (d.Ft.alloc((size_t)ld * ld, err)) return -1;
-		hipLaunchKernelGGL(k_transpose, dim3(ld / 32, ld / 32), dim3(kB), 0, st, d.F32.p, d.Ft.p, ld);
-		HIP_OK(hipGetLastError(), "transpose");
-		d.havet = true;
-	}
+	if (d.ensure_t(ld, err)) return -1;
 	const size_t total = (size_t)n * nrhs;
 	const int nb = (n + kB - 1) / kB;
 	Buf<float> rho32, B32, dB32;
@@ -1242,19 +1260,30 @@ int LinearSystem::adjoint(int nrhs, const double *rho, const double *B, const do
 	const int n = n_, ld = np_;
 	const size_t total = (size_t)n * nrhs;
 	hipStream_t st = d.st;
+	// the sweeps multiply by F^T in FP16 on MFMA (the gather's kernels on
+	// the transposed fragment copy); the residual stays FP64 over FP32 F
+	const int ct = pick_ct(nrhs), rp = ct * 16;
+	if (d.ensure_t16(ld, err)) return -1;
+	if (!d.tiles.count({ct, PREC_F16}) && autotune(nrhs, PREC_F16, err)) return -1;
+	int rt, Sg;
+	tile_config(nrhs, PREC_F16, &rt, &Sg);
+	const float inv = 1.0f / d.s16;
 	const int S = std::max(1, std::min(64, n / 256));
 	const int rows_per_split = (n + S - 1) / S;
 	const int fin_blocks = (int)((total + kB - 1) / kB);
 	Buf<double> rho64, g64, x64, r64, part64, blk, B64T, fb;
 	Fx64 fx;
 	if (fx.setup(n, ld, nrhs, err)) return -1;
-	Buf<float> rho32, r32, l32, ln32, part32;
+	Buf<float> rho32, r32, l32, part16;
+	Buf<_Float16> XT16;
+	const size_t xts = (size_t)rp * ld;
 	if (rho64.alloc(total, err) || g64.alloc(total, err) || x64.alloc(total, err) || r64.alloc(total, err) ||
 	    part64.alloc((size_t)S * total, err) || rho32.alloc(total, err) || r32.alloc(total, err) ||
-	    l32.alloc(total, err) || ln32.alloc(total, err) || part32.alloc((size_t)S * total, err) ||
+	    l32.alloc(total, err) || part16.alloc((size_t)Sg * ld * rp, err) || XT16.alloc(xts, err) ||
 	    blk.alloc((size_t)std::max(fin_blocks, 8192) * 2, err) || B64T.alloc((size_t)fx.rp * ld, err) ||
 	    fb.alloc(total, err))
 		return -1;
+	hipLaunchKernelGGL(k_fill<_Float16>, dim3(grid_for(xts)), dim3(kB), 0, st, XT16.p, xts, (_Float16)0.f);
 	std::vector<float> h(total);
 	for (size_t t = 0; t < total; t++) h[t] = (float)rho[t];
 	HIP_OK(hipMemcpyAsync(rho32.p, h.data(), total * 4, hipMemcpyHostToDevice, st), "upload");
@@ -1300,15 +1329,16 @@ int LinearSystem::adjoint(int nrhs, const double *rho, const double *B, const do
 		rel = enorm > 0 ? std::sqrt(ss) / enorm : 0.0;
 		if (rel <= tol || outer >= max_outer || mx == 0.0) break;
 		const int gg = grid_for(total);
+		// d <- r (scaled), then sweeps d <- r + F^T (rho o d) with XT = rho o d
 		hipLaunchKernelGGL(k_scale<_Float16>, dim3(gg), dim3(kB), 0, st, r64.p, 1.0 / mx, total, nrhs, ld, r32.p,
-		                   (_Float16 *)nullptr);
-		HIP_OK(hipMemcpyAsync(l32.p, r32.p, total * 4, hipMemcpyDeviceToDevice, st), "copy");
+		                   XT16.p, rho32.p);
 		for (int m = 0; m < sweeps; m++) {
-			hipLaunchKernelGGL(k_ftx<float>, tgrid, dim3(kB), 0, st, d.F32.p, ld, n, nrhs, l32.p, rho32.p,
-			                   rows_per_split, part32.p);
-			hipLaunchKernelGGL(k_ftx_finish<float>, dim3(fin_blocks), dim3(kB), 0, st, part32.p, S, n, nrhs, r32.p, l32.p,
-			                   0.0f, ln32.p, blk.p);
-			std::swap(l32.p, ln32.p);
+			if (launch_gemm<_Float16>(rt, ct, d.FT16.p, XT16.p, part16.p, ld, rp, Sg, st)) {
+				err = "adjoint: no kernel for this tile";
+				return -1;
+			}
+			hipLaunchKernelGGL((k_jacobi<_Float16, true>), dim3(gg), dim3(kB), 0, st, part16.p, Sg, ld, rp, n, nrhs,
+			                   inv, r32.p, rho32.p, l32.p, XT16.p);
 			total_sweeps++;
 		}
 		hipLaunchKernelGGL(k_axpy, dim3(gg), dim3(kB), 0, st, x64.p, l32.p, mx, total);

@@ -209,7 +209,8 @@ rad_status rad_lin_shoot(rad_lin *lin, int nrhs, const double *rho, const double
                          double target, int max_shots, double *B, double *info);
 
 /* Adjoint of the solve for a loss L(B): solves (I - diag(rho) F)^T lambda =
- * dL/dB (FP32 sweeps, FP64 residual refinement, no stored iterates) and
+ * dL/dB (FP16 MFMA sweeps on a transposed copy of F made on the first call,
+ * FP64 residual refinement over the FP32 F, no stored iterates) and
  * returns dL/dE = lambda and dL/drho = lambda o (F B), visibility fixed.
  * nrhs 1..4. */
 rad_status rad_lin_adjoint(rad_lin *lin, int nrhs, const double *rho, const double *B, const double *dLdB, double tol,
