@@ -483,6 +483,27 @@ rad_status rad_batch_set_matrix(rad_batch *b, int scene, const float *F)
 	return b->b->set_matrix(scene, F, b->err) ? RAD_EINVAL : RAD_OK;
 }
 
+rad_status rad_batch_autotune(rad_batch *b, int prec, int *rt, int *kc, int *grid)
+{
+	if (!b) return RAD_EINVAL;
+	if (prec != RAD_PREC_F16 && prec != RAD_PREC_BF16) {
+		b->err = "batch autotune: precision FP16 or BF16";
+		return RAD_EINVAL;
+	}
+	BatchTile t{0, 0, 0};
+	if (b->b->autotune(prec, t, b->err)) return RAD_EDEVICE;
+	if (rt) *rt = t.rt;
+	if (kc) *kc = t.kc;
+	if (grid) *grid = t.grid;
+	return RAD_OK;
+}
+
+rad_status rad_batch_set_tile(rad_batch *b, int rt, int kc, int grid)
+{
+	if (!b) return RAD_EINVAL;
+	return b->b->set_tile(BatchTile{rt, kc, grid}, b->err) ? RAD_EINVAL : RAD_OK;
+}
+
 rad_status rad_batch_solve(rad_batch *b, int prec, int use_graph, const double *rho, const double *E, double tol,
                            int outer, int sweeps, double *B, double *rel_res, double *info4)
 {

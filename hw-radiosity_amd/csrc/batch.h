@@ -12,6 +12,13 @@
 
 namespace rgpu {
 
+// Schedule of the batched sweep product: each wave owns 16*rt rows (a tile
+// is 64*rt rows of one scene), a tile spans kc of k, and grid persistent
+// blocks walk the tile list.
+struct BatchTile {
+	int rt, kc, grid;
+};
+
 class SceneBatch {
 public:
 	~SceneBatch();
@@ -22,6 +29,11 @@ public:
 	int total() const;       // sum of n
 	float *matrix32(int s);  // device, ld(s) x ld(s) row-major, zero padded
 	int set_matrix(int s, const float *F, std::string &err);
+	// Times one sweep for every candidate schedule on the current matrices
+	// and keeps the fastest for precision prec; solve does it on first use
+	// unless set_tile fixed the schedule.
+	int autotune(int prec, BatchTile &out, std::string &err);
+	int set_tile(const BatchTile &t, std::string &err);
 	// rho, E, B: (total() x 3) row-major, scenes concatenated in order.
 	// graph = true: one HIP graph launch for everything; false: the same
 	// kernels launched scene after scene (the unbatched reference schedule,

@@ -41,7 +41,7 @@ EXPORTS = ("rad_create", "rad_destroy", "rad_last_error", "rad_load_tri", "rad_s
            "rad_lin_set_matrix", "rad_lin_get_matrix", "rad_lin_set_areas", "rad_lin_autotune", "rad_lin_set_tile",
            "rad_lin_gemm_ms", "rad_lin_multiply", "rad_lin_gather", "rad_lin_shoot", "rad_lin_adjoint",
            "rad_batch_create", "rad_batch_destroy", "rad_batch_last_error", "rad_batch_from_scene",
-           "rad_batch_set_matrix", "rad_batch_solve")
+           "rad_batch_set_matrix", "rad_batch_autotune", "rad_batch_set_tile", "rad_batch_solve")
 
 PREC_F32, PREC_F16, PREC_BF16 = 0, 1, 2
 VIS_SEGMENT, VIS_REFERENCE = 0, 1
@@ -100,6 +100,8 @@ def library() -> ctypes.CDLL:
             "rad_batch_destroy": [vp],
             "rad_batch_from_scene": [vp, ip, vp, ip, vp],
             "rad_batch_set_matrix": [vp, ip, vp],
+            "rad_batch_autotune": [vp, ip, vp, vp, vp],
+            "rad_batch_set_tile": [vp, ip, ip, ip],
             "rad_batch_solve": [vp, ip, ip, vp, vp, dp, ip, ip, vp, vp, vp],
         }
         for name, args in sig.items():
@@ -399,6 +401,15 @@ class SceneBatch:
         F = np.ascontiguousarray(F, np.float32)
         assert F.shape == (self.sizes[s], self.sizes[s])
         self._ok(library().rad_batch_set_matrix(self.h, int(s), _p(F)))
+
+    def autotune(self, prec="f16"):
+        """Times every sweep-product schedule; returns (rt, kc, grid)."""
+        v = [ctypes.c_int(0) for _ in range(3)]
+        self._ok(library().rad_batch_autotune(self.h, _prec(prec), *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_tile(self, rt, kc, grid):
+        self._ok(library().rad_batch_set_tile(self.h, int(rt), int(kc), int(grid)))
 
     def solve(self, rho, E, prec="f16", graph=True, tol=1e-6, outer=4, sweeps=8):
         """rho, E: (sum n, 3).  Returns (B, rel_res per scene, info)."""

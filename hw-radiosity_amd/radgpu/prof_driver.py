@@ -1,9 +1,11 @@
 """Profiling driver: one hall-200k solve (or, with --linear, the explicit
 matrix workload: the 32768-patch room's form factors, FP16 gathers with 3
 and 192 right-hand sides, 2000 plain shots; with --gemm, the same matrix and
-five FP16 sweeps with 192 right-hand sides at the tile RT = 4, split-K 2),
-then a plain 256 MiB device copy that calibrates the FETCH_SIZE / WRITE_SIZE
-counters in the same run.
+five FP16 sweeps with 192 right-hand sides at the tile RT = 4, split-K 2;
+with --batch, 256 rooms of 500-8000 patches, the autotuned FP16 batch solve
+twice in one graph launch; with --adjoint, the 16384-patch room's FP16
+adjoint solve twice), then a plain 256 MiB device copy that calibrates the
+FETCH_SIZE / WRITE_SIZE counters in the same run.
 
     rocprofv3 --pmc FETCH_SIZE -d <dir> -o p -- python3 hw-radiosity_amd/radgpu/prof_driver.py [--linear]
 """
@@ -51,11 +53,54 @@ def gemm():
     e.close()
 
 
+def batch():
+    import numpy as np
+    sizes = scenes.batch_sizes(256, 500, 8000, seed=4)
+    b = radgpu.SceneBatch(sizes)
+    E_all, rho_all = [], []
+    for s, n in enumerate(sizes):
+        sc = scenes.patch_room(1000 + s, n)
+        with radgpu.Engine("-triangle", str(n), "-quiet", "1") as e:
+            e.set_scene(sc).prepare_host().upload()
+            b.from_scene(s, e)
+        E, rho = scenes.emission_reflectance(sc.frgb, sc.brgb)
+        E_all.append(E)
+        rho_all.append(rho)
+    E, rho = np.vstack(E_all), np.vstack(rho_all)
+    print("tile", b.autotune("f16"), flush=True)
+    for _ in range(2):
+        B, rel, info = b.solve(rho, E, prec="f16", graph=True, tol=1e-6, outer=12, sweeps=6)
+        print("batch", info, float(rel.max()), flush=True)
+    b.close()
+
+
+def adjoint():
+    import numpy as np
+    n = 16384
+    sc = scenes.patch_room(1, n)
+    e = radgpu.Engine("-triangle", str(n), "-quiet", "1")
+    e.set_scene(sc).prepare_host().upload()
+    lin = radgpu.LinearSystem(n)
+    print("form factors %.1f ms" % lin.from_scene(e), flush=True)
+    E, rho = scenes.emission_reflectance(sc.frgb, sc.brgb)
+    B, info = lin.gather(rho, E, prec="f16", tol=1e-12, max_outer=40, sweeps=8)
+    g = np.random.default_rng(3).standard_normal(B.shape)
+    for _ in range(2):
+        dE, drho, info = lin.adjoint(rho, B, g, tol=1e-8, max_outer=40, sweeps=8)
+        print("adjoint", info, flush=True)
+    lin.close()
+    e.close()
+
+
 def main():
     if "--linear" in sys.argv:
         linear()
     elif "--gemm" in sys.argv:
         gemm()
+    elif "--batch" in sys.argv:
+        batch()
+    elif "--adjoint" in sys.argv:
+        adjoint()
     else:
         s = scenes.hall_scene(2)
         e = radgpu.Engine("-converge", "1200", "-triangle", "1000000", "-quiet", "1")

@@ -233,6 +233,14 @@ const char *rad_batch_last_error(const rad_batch *batch);
  * rad_lin_from_scene), or from a host n x n row-major matrix. */
 rad_status rad_batch_from_scene(rad_batch *batch, int scene, rad_ctx *ctx, int visibility, double *ms);
 rad_status rad_batch_set_matrix(rad_batch *batch, int scene, const float *F);
+/* Picks the schedule of the batched sweep product -- rows per wave rt*16
+ * (rt in {1, 2, 4}), k per tile kc (1024..8192) and the persistent grid
+ * (512..4096 blocks) -- by timing one sweep of every candidate on this
+ * batch; solve does it on first use for each precision. */
+rad_status rad_batch_autotune(rad_batch *batch, int prec, int *rt, int *kc, int *grid);
+/* Forces that schedule instead (rt in {1, 2, 4}, kc in {1024, 2048, 4096,
+ * 8192}, grid 64..16384); a later rad_batch_autotune hands the choice back. */
+rad_status rad_batch_set_tile(rad_batch *batch, int rt, int kc, int grid);
 /* rho, E, B: (sum of n) x 3 row-major, scenes concatenated in order.
  * `outer` refinements of `sweeps` sweeps each, a scene stopping once its
  * FP64 relative residual is <= tol.  use_graph = 1: one graph launch;

@@ -336,6 +336,38 @@ def test_batch_graph_equals_sequential_and_dense_lu():
     b.close()
 
 
+@pytest.mark.parametrize("tile", [(1, 1024, 512), (2, 4096, 64), (4, 8192, 2048)])
+def test_batch_tile_schedules_agree(tile):
+    """Every sweep-product schedule (rows per wave, k per tile, persistent
+    grid) solves to the dense LU answer, graph and scene-by-scene alike;
+    k chunks of 1024 split the 2100-patch scene three ways, 8192 not at all."""
+    sizes = [700, 64, 2100, 999]
+    b, Fs, E, rho = _batch(sizes, seed0=5000)
+    b.set_tile(*tile)
+    Bg, relg, info = b.solve(rho, E, prec="f16", graph=True, tol=1e-11, outer=12, sweeps=6)
+    Bs, rels, _ = b.solve(rho, E, prec="f16", graph=False, tol=1e-11, outer=12, sweeps=6)
+    print(tile, info, flush=True)
+    assert np.array_equal(Bg, Bs) and info["above_tol"] == 0
+    off = b.offsets
+    for s, F in enumerate(Fs):
+        ref = lo.dense_solve(F, rho[off[s]:off[s + 1]], E[off[s]:off[s + 1]])
+        assert rel(Bg[off[s]:off[s + 1]], ref) < 1e-9, (tile, s)
+    b.close()
+
+
+def test_batch_autotune_and_bad_tile():
+    b, Fs, E, rho = _batch([300, 1200], seed0=6000)
+    for prec in ("f16", "bf16"):
+        rt, kc, grid = b.autotune(prec)
+        assert rt in (1, 2, 4) and kc in (1024, 2048, 4096, 8192) and 512 <= grid <= 4096
+        B, r, info = b.solve(rho, E, prec=prec, graph=True, tol=1e-10, outer=12, sweeps=6)
+        assert info["above_tol"] == 0
+    for bad in ((3, 1024, 512), (1, 512, 512), (1, 1024, 0)):
+        with pytest.raises(radgpu.RadError):
+            b.set_tile(*bad)
+    b.close()
+
+
 def test_batch_scenes_stop_at_tolerance():
     sizes = [400, 900]
     b, Fs, E, rho = _batch(sizes, seed0=3000)
