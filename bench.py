@@ -398,7 +398,9 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "note": "algorithmic bytes count every octree and triangle byte a ray reads; most "
                              "are L2/MALL hits (traffic = measured HBM bytes per launch), so frac above "
-                             "1 means the kernel is latency/divergence-bound, not HBM-bound"},
+                             "1 means the kernel is latency/divergence-bound, not HBM-bound; rocprofv3 "
+                             "SQ counters of the same workload put its VALU lane use at 0.41 "
+                             "(profiles/k_trace_hall200k_sq.md)"},
         "counters": cnt_sum,
     }
     eng.close()

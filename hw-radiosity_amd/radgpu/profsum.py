@@ -2,11 +2,15 @@
 
     python -m radgpu.profsum trace <results.db>            per-kernel durations
     python -m radgpu.profsum pmc <fetch.db> <write.db> <kernel-substring> <copy_bytes>
+    python -m radgpu.profsum sq <results.db> <kernel-substring>    SQ counters, summed
 
 The pmc form reports FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the
 kernels whose name contains the substring, each scaled by the factor that
 makes the same counter read `copy_bytes` for the plain device copy
-(__amd_rocclr_copyBuffer) profiled in the same run.
+(__amd_rocclr_copyBuffer) profiled in the same run.  The sq form sums every
+counter of the run over the kernel's dispatches and derives the VALU lane
+use, SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU): the share of a wave's
+64 lanes active over the cycles it spends in VALU instructions.
 """
 import json
 import sqlite3
@@ -53,8 +57,27 @@ def pmc(fetch_db, write_db, kernel, copy_bytes):
     return res
 
 
+def sq(db_path, kernel):
+    con = sqlite3.connect(db_path)
+    rows = con.execute("select kernel_name, counter_name, dispatch_id, sum(value) from counters_collection "
+                       "group by kernel_name, counter_name, dispatch_id").fetchall()
+    tot, disp = {}, set()
+    for name, counter, d, v in rows:
+        if kernel in name:
+            tot[counter] = tot.get(counter, 0.0) + v
+            disp.add(d)
+    res = {"kernel_substring": kernel, "dispatches": len(disp), "counters": tot}
+    if tot.get("SQ_ACTIVE_INST_VALU"):
+        res["valu_lane_use"] = tot.get("SQ_THREAD_CYCLES_VALU", 0.0) / (64.0 * tot["SQ_ACTIVE_INST_VALU"])
+    if tot.get("SQ_INSTS_VALU") and tot.get("SQ_WAVES"):
+        res["valu_insts_per_wave"] = tot["SQ_INSTS_VALU"] / tot["SQ_WAVES"]
+    return res
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "trace":
         print(trace(sys.argv[2]))
+    elif sys.argv[1] == "sq":
+        print(json.dumps(sq(sys.argv[2], sys.argv[3]), indent=1))
     else:
         print(json.dumps(pmc(sys.argv[2], sys.argv[3], sys.argv[4], float(sys.argv[5])), indent=1))
