@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/radiosity_gpu.h"
@@ -26,6 +27,16 @@ struct rad_ctx {
 	std::string err;
 	bool loaded = false, prepared = false, uploaded = false, device_newer = false;
 	double ms_setup = 0, ms_upload = 0;
+	// device start-up on its own thread while the host prepares (rad_run):
+	// the engine it made, or the error; read only after the join
+	std::thread starter;
+	DeviceEngine *started = nullptr;
+	std::string start_err;
+	~rad_ctx()
+	{
+		if (starter.joinable()) starter.join();
+		delete started;
+	}
 };
 
 struct rad_lin {
@@ -171,6 +182,12 @@ rad_status rad_upload(rad_ctx *c)
 {
 	if (!c) return RAD_EINVAL;
 	if (!c->prepared) return fail(c, RAD_ESTATE, "rad_prepare_host first");
+	if (!c->dev && c->starter.joinable()) {  // started during the host set-up
+		c->starter.join();
+		if (!c->started) return fail(c, RAD_EDEVICE, c->start_err);
+		c->dev.reset(c->started);
+		c->started = nullptr;
+	}
 	if (!c->dev) {  // HIP start-up and the context's stream: not part of the upload time
 		DeviceEngine *d = DeviceEngine::create(c->err);
 		if (!d) return RAD_EDEVICE;
@@ -198,6 +215,16 @@ rad_status rad_run(rad_ctx *c, const char *prefix, int *iterations)
 {
 	if (!c || !prefix) return RAD_EINVAL;
 	rad_status st;
+	int device = -1;
+	if (!c->dev && !c->uploaded && !c->prepared && !c->starter.joinable() &&
+	    (device = DeviceEngine::current_device()) >= 0)
+		c->starter = std::thread([c, device] {
+			c->started = DeviceEngine::create_on(device, c->start_err);
+			if (c->started && c->started->warm(c->start_err)) {
+				delete c->started;
+				c->started = nullptr;
+			}
+		});
 	if ((st = rad_prepare_host(c)) != RAD_OK) return st;
 	if (!c->uploaded && (st = rad_upload(c)) != RAD_OK) return st;
 	const HostScene &s = c->scene;

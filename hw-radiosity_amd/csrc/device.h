@@ -35,6 +35,15 @@ public:
 	~DeviceEngine();
 	// Selects the current HIP device and creates the engine's stream.
 	static DeviceEngine *create(std::string &err);
+	// The calling thread's current HIP device, or -1 when there is none.
+	static int current_device();
+	// create() on `device`, made current on the calling thread first (for a
+	// thread other than the caller's).
+	static DeviceEngine *create_on(int device, std::string &err);
+	// Loads the engine's device code (one empty launch on the stream, waited
+	// for), so that the first upload does not; may run on another thread
+	// while the host prepares.
+	int warm(std::string &err);
 	// Copies the scene (after pre-partition) and the octree to the device and
 	// selects the first source.
 	int upload(const HostScene &s, const FlatOctree &t, std::string &err);

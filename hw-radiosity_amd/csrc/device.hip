@@ -1187,6 +1187,37 @@ DeviceEngine *DeviceEngine::create(std::string &err)
 	return e;
 }
 
+int DeviceEngine::current_device()
+{
+	int d = -1;
+	return hipGetDevice(&d) == hipSuccess ? d : -1;
+}
+
+DeviceEngine *DeviceEngine::create_on(int device, std::string &err)
+{
+	if (hipSetDevice(device) != hipSuccess) {
+		err = "HIP device " + std::to_string(device) + " cannot be selected";
+		return nullptr;
+	}
+	return create(err);
+}
+
+__global__ void k_warm(int *p)
+{
+	if (p && blockIdx.x == 0 && threadIdx.x == 0) p[0] = 0;
+}
+
+int DeviceEngine::warm(std::string &err)
+{
+	int *p = nullptr;
+	HIP_OK(hipMalloc(&p, 4096), "warm");
+	hipLaunchKernelGGL(k_warm, dim3(1), dim3(64), 0, d_->stream, p);
+	const hipError_t e = hipStreamSynchronize(d_->stream);
+	(void)hipFree(p);
+	HIP_OK(e, "warm");
+	return 0;
+}
+
 int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &err)
 {
 	d_->have_seg = false;

@@ -65,21 +65,46 @@ namespace {
 
 // One record in the reference's order: "Triangle", the mean colour, then per
 // vertex its coordinates and the normal, written at o (at most kRecordMax
-// bytes); returns the end.  last = the text of the last value that
-// converted (the reference prints its static buffer again for a value that
-// does not convert); updated in place.
+// bytes); returns the end.  last (ll chars) = the text of the last value
+// that converted (the reference prints its static buffer again for a value
+// that does not convert); updated in place.
 constexpr int kRecordMax = 1024;
 
-inline char *put_text(char *o, const char *s)
+inline char *put_literal(char *o, const char *s, size_t l)
 {
-	const size_t l = std::strlen(s);
 	std::memcpy(o, s, l);
 	return o + l;
 }
 
-char *format_record(const Patch &t, bool old_format, char *last, char *o)
+inline char *put_uint(char *o, unsigned v)
 {
-	o = put_text(o, "Triangle \n");
+	char rev[10];
+	int nd = 0;
+	do {
+		rev[nd++] = (char)('0' + v % 10);
+		v /= 10;
+	} while (v);
+	while (nd) *o++ = rev[--nd];
+	return o;
+}
+
+// x's text at o, or the text of the last value that converted (last, ll
+// chars, NUL-terminated; updated when x converts); returns the end
+inline char *put_value(char *o, float x, char *last, int &ll)
+{
+	const int l = format_truncated(x, o);
+	if (l) {
+		std::memcpy(last, o, (size_t)l);
+		last[l] = 0;
+		ll = l;
+		return o + l;
+	}
+	return put_literal(o, last, (size_t)ll);
+}
+
+char *format_record(const Patch &t, bool old_format, char *last, int &ll, char *o)
+{
+	o = put_literal(o, "Triangle \n", 10);
 	if (old_format) {
 		for (int v = 0; v < 3; v++) {
 			o += std::snprintf(o, 400, "%f %f %f ", t.p[v][0], t.p[v][1], t.p[v][2]);
@@ -87,7 +112,6 @@ char *format_record(const Patch &t, bool old_format, char *last, char *o)
 		}
 		return o;
 	}
-	int rgb[3];
 	for (int c = 0; c < 3; c++) {
 		float m = 0;
 		for (int v = 0; v < 3; v++) m += t.accB[v][c];
@@ -95,29 +119,24 @@ char *format_record(const Patch &t, bool old_format, char *last, char *o)
 		if (255.0f < m) m = 255.0f;  // std::min(m, 255.0f)
 		if (m < 0.0f) m = 0.0f;      // std::max(m, 0.0f)
 		if (std::isnan(m)) m = 0;
-		rgb[c] = (int)m;
+		o = put_uint(o, (unsigned)(int)m);  // 0 .. 255, as %d prints it
+		*o++ = ' ';
 	}
-	o += std::snprintf(o, 64, "%d %d %d 0 0 0\n", rgb[0], rgb[1], rgb[2]);
-	auto text = [last](float x) {
-		char tmp[48];
-		if (format_truncated(x, tmp)) std::strcpy(last, tmp);
-		return (const char *)last;
-	};
+	o = put_literal(o, "0 0 0\n", 6);
 	// the normal text is formatted before the vertices and reused
 	char nrm[160];
 	char *e = nrm;
 	for (int k = 0; k < 3; k++) {
-		e = put_text(e, text(t.n[k]));
+		e = put_value(e, t.n[k], last, ll);
 		*e++ = k < 2 ? ' ' : '\n';
 	}
 	const size_t nl = (size_t)(e - nrm);
 	for (int v = 0; v < 3; v++) {
 		for (int k = 0; k < 3; k++) {
-			o = put_text(o, text(t.p[v][k]));
+			o = put_value(o, t.p[v][k], last, ll);
 			*o++ = ' ';
 		}
-		std::memcpy(o, nrm, nl);
-		o += nl;
+		o = put_literal(o, nrm, nl);
 	}
 	return o;
 }
@@ -164,18 +183,23 @@ int FinWriter::write(const std::string &path, const Patch *pool, int trinum, boo
 			}
 	}
 	bool ok = true;
-	// chunks are formatted in parallel and written in order as they finish
-#pragma omp parallel for ordered schedule(dynamic, 1)
-	for (int c = 0; c < nch; c++) {
-		char last[48];
-		std::memcpy(last, start[c].data(), sizeof last);
+	// chunks are formatted in parallel and written in order as they finish;
+	// each thread formats into one buffer of its own
+#pragma omp parallel
+	{
 		std::unique_ptr<char[]> buf(new char[(size_t)kChunk * kRecordMax]);  // not zeroed
-		char *o = buf.get();
-		for (int i = c * kChunk; i < std::min(trinum, (c + 1) * kChunk); i++)
-			if (pool[i].parent >= 0) o = format_record(pool[i], old_format, last, o);  // logical slots are not written
-		const size_t len = (size_t)(o - buf.get());
+#pragma omp for ordered schedule(dynamic, 1)
+		for (int c = 0; c < nch; c++) {
+			char last[48];
+			std::memcpy(last, start[c].data(), sizeof last);
+			int ll = (int)std::strlen(last);
+			char *o = buf.get();
+			for (int i = c * kChunk; i < std::min(trinum, (c + 1) * kChunk); i++)
+				if (pool[i].parent >= 0) o = format_record(pool[i], old_format, last, ll, o);  // logical slots are not written
+			const size_t len = (size_t)(o - buf.get());
 #pragma omp ordered
-		if (ok) ok = std::fwrite(buf.get(), 1, len, f) == len;
+			if (ok) ok = std::fwrite(buf.get(), 1, len, f) == len;
+		}
 	}
 	ok = (std::fclose(f) == 0) && ok;
 	if (!ok) {
