@@ -44,3 +44,20 @@ def test_cli_usage_and_open_error(tmp_path):
     assert r.returncode == 1 and "Usage" in r.stderr
     r = radgpu.run_cli([str(tmp_path / "none.tri"), "-o", str(tmp_path / "x")])
     assert r.returncode == 1 and "File open error" in r.stderr
+
+
+def test_run_without_a_device_is_a_device_error(tmp_path):
+    """The product path has no CPU fallback: with no HIP device the run stops
+    with RAD_EDEVICE (the device start-up thread does not start) and writes
+    no .fin."""
+    import pytest
+    from radgpu import scenes
+    probe = subprocess.run(["bash", "-c", "ls /dev/kfd"], capture_output=True)
+    if probe.returncode == 0:
+        pytest.skip("a GPU device node is present")
+    with radgpu.Engine("-quiet", "1") as e:
+        e.set_scene(scenes.room_scene(1))
+        with pytest.raises(radgpu.RadError) as ei:
+            e.run(str(tmp_path / "out"))
+        assert ei.value.status == radgpu.RAD_EDEVICE and "GPU only" in str(ei.value)
+    assert not os.path.exists(str(tmp_path / "out.fin"))
