@@ -464,7 +464,7 @@ __global__ __launch_bounds__(kB) void k_lsel(const float *dB, const float *area,
 
 __global__ __launch_bounds__(kB) void k_lshoot(const float *__restrict__ Ft, int ld, const float *rho, const float *area,
                                                float *B, float *dB, int n, int nrhs, const ShotRec *rin,
-                                               ShotRec *rout, int *shots, int shot)
+                                               ShotRec *rout, int *shots)
 {
 	__shared__ unsigned long long wk[4];
 	__shared__ int sb;
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(kB) void k_lshoot(const float *__restrict__ Ft, int
 		if (p > 0.f) key = power_key(p, i);
 	}
 	block_record(key, row, nrhs, rout + blockIdx.x);
-	if (blockIdx.x == 0 && threadIdx.x == 0) *shots = shot + 1;
+	if (blockIdx.x == 0 && threadIdx.x == 0) *shots += 1;  // launches on one stream: no race
 }
 
 __global__ void k_transpose(const float *F, float *Ft, int ld)
@@ -1200,6 +1200,29 @@ int LinearSystem::shoot(int nrhs, const double *rho, const double *E, const doub
 		e = std::sqrt(a / std::max(b, 1e-300));
 		return 0;
 	};
+	// shot k reads the records of shot k - 1 (parity k & 1) and writes the
+	// other set
+	auto launch_shot = [&](int k) {
+		hipLaunchKernelGGL(k_lshoot, dim3(nb), dim3(kB), 0, st, d.Ft.p, ld, rho32.p, d.area.p, B32.p, dB32.p, n, nrhs,
+		                   rec.p + (size_t)(k & 1) * nb, rec.p + (size_t)((k + 1) & 1) * nb, shots_d.p);
+	};
+	// A shot needs the previous one's argmax, so plain shooting is bound by
+	// launch latency: runs of kGraphShots shots (an even number, so parities
+	// repeat) are captured once into a HIP graph and replayed.
+	constexpr int kGraphShots = 64;
+	struct GraphGuard {
+		hipGraph_t g = nullptr;
+		hipGraphExec_t x = nullptr;
+		~GraphGuard()
+		{
+			if (x) (void)hipGraphExecDestroy(x);
+			if (g) (void)hipGraphDestroy(g);
+		}
+	} run;
+	HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "shot capture");
+	for (int k = 0; k < kGraphShots; k++) launch_shot(k);
+	HIP_OK(hipStreamEndCapture(st, &run.g), "shot capture");
+	HIP_OK(hipGraphInstantiate(&run.x, run.g, nullptr, nullptr, 0), "shot graph");
 	const auto t0 = std::chrono::steady_clock::now();
 	hipLaunchKernelGGL(k_lsel, dim3(nb), dim3(kB), 0, st, dB32.p, d.area.p, n, nrhs, rec.p);
 	double check_ms = 0.0, ms_target = -1.0, errv = -1.0;
@@ -1207,10 +1230,11 @@ int LinearSystem::shoot(int nrhs, const double *rho, const double *E, const doub
 	for (;;) {
 		const int b = std::min(chunk, max_shots - launched);
 		if (b <= 0) break;
-		for (int k = 0; k < b; k++, launched++)
-			hipLaunchKernelGGL(k_lshoot, dim3(nb), dim3(kB), 0, st, d.Ft.p, ld, rho32.p, d.area.p, B32.p, dB32.p, n,
-			                   nrhs, rec.p + (size_t)(launched & 1) * nb, rec.p + (size_t)((launched + 1) & 1) * nb,
-			                   shots_d.p, launched);
+		int k = 0;
+		if ((launched & 1) == 0)
+			for (; k + kGraphShots <= b; k += kGraphShots) HIP_OK(hipGraphLaunch(run.x, st), "shot graph launch");
+		for (; k < b; k++) launch_shot(launched + k);
+		launched += b;
 		HIP_OK(hipGetLastError(), "shoot launch");
 		HIP_OK(hipMemcpyAsync(&done_shots, shots_d.p, sizeof(int), hipMemcpyDeviceToHost, st), "state");
 		HIP_OK(hipStreamSynchronize(st), "shoot");
