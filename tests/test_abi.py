@@ -61,3 +61,12 @@ def test_run_without_a_device_is_a_device_error(tmp_path):
             e.run(str(tmp_path / "out"))
         assert ei.value.status == radgpu.RAD_EDEVICE and "GPU only" in str(ei.value)
     assert not os.path.exists(str(tmp_path / "out.fin"))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with a message before
+    touching a device."""
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
