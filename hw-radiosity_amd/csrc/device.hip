@@ -65,6 +65,7 @@ constexpr int kGroups = 2;        // receiver groups per block
 constexpr int kMinQueue = 8;      // iterations queued per host check at least (up to -batch)
 constexpr int kRecvPerBlock = kGroups * kWavesPerBlock * kRecvPerWave;
 constexpr int kMaxSrc = 256;
+constexpr int kRayBuckets = 16;   // ray keys k_compact groups by
 constexpr int kMaxWalk = 1 << 24;
 constexpr double kPiD = 3.1415926535897932;
 constexpr float kGuardUp = 1.0f + 0x1p-19f;
@@ -595,43 +596,103 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 	add_block_counters(vals, bcnt);
 }
 
-// The rays of source k as a dense list in receiver-position order: block b
-// (one per k_gen range) takes positions q of [b * 6 * kRecvPerBlock, +6 *
-// kRecvPerBlock), lane i = 6 * order[q / 6] + q % 6, and places those whose
-// form-factor prefix is non-zero at offset sum(raycnt[0..b)), in ascending
-// q; the last block stores the total.
+// The rays of source k as a dense list: block b (one per k_gen range) takes
+// positions q of [b * 6 * kRecvPerBlock, +6 * kRecvPerBlock), lane i = 6 *
+// order[q / 6] + q % 6, and places those whose form-factor prefix is
+// non-zero at offset sum(raycnt[0..b)), grouped by their origin's key (the
+// list length of the leaf holding the receiver vertex, in kRayBuckets
+// steps), smallest key first, in ascending q within a key; the last block
+// stores the total.  A ray's cost follows mostly the lists it tests, so
+// grouping the rays of one neighbourhood by that key makes each wave's 64
+// rays alike (less divergence: k_trace 272 -> 255 us on the hall); which
+// rays there are, and each one's result, do not depend on the order.
 __global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float *ffbuf, const int *order, int nlanes,
-                                                    const int *raycnt, int nblocks, int *rays)
+                                                    const int *raycnt, int nblocks, const unsigned char *rkey,
+                                                    int *rays)
 {
 	if (L->done || k >= L->nsrc) return;
+	constexpr int kRounds = (6 * kRecvPerBlock + kBlock - 1) / kBlock;
 	__shared__ int wsum[kWavesPerBlock];
-	__shared__ int base;
+	__shared__ int slot[kRounds][kWavesPerBlock][kRayBuckets];  // rays per (round, wave, key), then their first slot
+	__shared__ int ktot[kRayBuckets], kfirst[kRayBuckets];
+	__shared__ int total;
 	const int b = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	int off = 0;
 	for (int j = threadIdx.x; j < b; j += kBlock) off += raycnt[j];
 	for (int o = 32; o > 0; o >>= 1) off += __shfl_xor(off, o);
 	if (lane == 0) wsum[w] = off;
-	__syncthreads();
-	if (threadIdx.x == 0) base = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-	__syncthreads();
 	constexpr int chunk = 6 * kRecvPerBlock;
 	const int i0 = b * chunk, i1 = min(nlanes, i0 + chunk);
-	int run = base;
-	for (int r0 = i0; r0 < i1; r0 += kBlock) {
-		const int q = r0 + threadIdx.x;
+	const unsigned long long below = (1ull << lane) - 1;
+	int ri[kRounds], rk[kRounds];
+	unsigned long long rm[kRounds];
+#pragma unroll
+	for (int r = 0; r < kRounds; r++) {
+		const int q = i0 + r * kBlock + threadIdx.x;
 		const int i = q < i1 ? 6 * order[q / 6] + q % 6 : 0;
-		const bool ray = q < i1 && ffbuf[i] != 0.0f;
-		const unsigned long long bal = __ballot(ray);
-		const int rank = __popcll(bal & ((1ull << lane) - 1));
-		__syncthreads();  // wsum is reused each round
-		if (lane == 0) wsum[w] = __popcll(bal);
-		__syncthreads();
-		int before = run;
-		for (int q = 0; q < w; q++) before += wsum[q];
-		if (ray) rays[before + rank] = i;
-		run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+		const int key = (q < i1 && ffbuf[i] != 0.0f) ? rkey[i >> 1] : -1;  // -1: no ray
+		unsigned long long mine = 0;
+#pragma unroll
+		for (int kb = 0; kb < kRayBuckets; kb++) {
+			const unsigned long long m = __ballot(key == kb);
+			if (key == kb) mine = m;
+			if (lane == kb) slot[r][w][kb] = __popcll(m);
+		}
+		ri[r] = i;
+		rk[r] = key;
+		rm[r] = mine;
 	}
-	if (b == nblocks - 1 && threadIdx.x == 0) L->nrays = run;
+	__syncthreads();
+	if (threadIdx.x < kRayBuckets) {
+		int t = 0;
+		for (int r = 0; r < kRounds; r++)
+			for (int v = 0; v < kWavesPerBlock; v++) t += slot[r][v][threadIdx.x];
+		ktot[threadIdx.x] = t;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {  // smallest key first, after the rays of the blocks before
+		int run = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+		for (int kb = 0; kb < kRayBuckets; kb++) {
+			kfirst[kb] = run;
+			run += ktot[kb];
+		}
+		total = run;
+	}
+	__syncthreads();
+	if (threadIdx.x < kRayBuckets) {  // within a key: rounds, then waves, in order
+		int run = kfirst[threadIdx.x];
+		for (int r = 0; r < kRounds; r++)
+			for (int v = 0; v < kWavesPerBlock; v++) {
+				const int c = slot[r][v][threadIdx.x];
+				slot[r][v][threadIdx.x] = run;
+				run += c;
+			}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int r = 0; r < kRounds; r++)
+		if (rk[r] >= 0) rays[slot[r][w][rk[r]] + __popcll(rm[r] & below)] = ri[r];
+	if (b == nblocks - 1 && threadIdx.x == 0) L->nrays = total;
+}
+
+// Key of each receiver vertex for k_compact's grouping: the list length of
+// the leaf that holds the vertex (the first leaf every ray from it tests),
+// in steps of 8, capped at kRayBuckets - 1; 0 outside the grid.
+__global__ __launch_bounds__(kBlock) void k_ray_keys(DScene S, DRecv R, int nv, unsigned char *rkey)
+{
+	const int sv = blockIdx.x * kBlock + threadIdx.x;
+	if (sv >= nv) return;
+	const float px = R.vx[sv], py = R.vy[sv], pz = R.vz[sv];
+	int key = 0;
+	if (!(px < S.g0x || px > S.g1x || py < S.g0y || py > S.g1y || pz < S.g0z || pz > S.g1z)) {
+		const int gx = cell_coord(px, S.g0x, S), gy = cell_coord(py, S.g0y, S), gz = cell_coord(pz, S.g0z, S);
+		if ((unsigned)gx < (unsigned)S.gt && (unsigned)gy < (unsigned)S.gt && (unsigned)gz < (unsigned)S.gt) {
+			const int leaf = S.cell_leaf[(gx * S.gt + gy) * S.gt + gz];
+			const int cnt = __float_as_int(S.leaf_box[2 * leaf + 1].w) & 0xffffff;
+			key = min(kRayBuckets - 1, cnt >> 3);
+		}
+	}
+	rkey[sv] = (unsigned char)key;
 }
 
 // Source k's rays, part 2: one lane per ray of the compacted list, which
@@ -1131,6 +1192,7 @@ struct DeviceEngine::Impl {
 	DevBuf<unsigned char> live;
 	DevBuf<float> ffbuf;
 	DevBuf<int> raycnt, rays;  // per-k_gen-block ray counts, compacted ray list
+	DevBuf<unsigned char> rkey;  // per receiver vertex: k_compact's grouping key
 	DevBuf<int> order;         // receiver processing order (DRecv::order)
 	DevBuf<Patch> pool;        // the uploaded slot records (packed by k_pack_items / k_pack_recv)
 	DevBuf<int> items;         // the octree's list entries (original slots)
@@ -1298,6 +1360,10 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	             (double)t.min_grid, 1.0 / (double)t.min_grid, t.gt, thr_t, thr_t * kGuardDown,
 	             largest_float_at_most(1e-6)};
 	d.R = DRecv{d.vx.p, d.vy.p, d.vz.p, d.nrm.p, d.cA.p, d.rho.p, d.acc.p, d.dacc.p, d.dB.p, n, d.order.p};
+	if (d.rkey.alloc((size_t)n * 3 + 1, err)) return -1;
+	if (n > 0)
+		hipLaunchKernelGGL(k_ray_keys, dim3((unsigned)(((size_t)n * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d.S,
+		                   d.R, n * 3, d.rkey.p);
 	if (d.lowest || d.parallel_src <= 1) {
 		const int sb = (n + kBlock - 1) / kBlock;
 		if (sb > 0) {
@@ -1370,7 +1436,7 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 				if (d.fma) hipLaunchKernelGGL(k_gen<true>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
 				else hipLaunchKernelGGL(k_gen<false>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
 				hipLaunchKernelGGL(k_compact, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, d.order.p, 6 * d.R.nslots, d.raycnt.p,
-				                   d.nblocks, d.rays.p);
+				                   d.nblocks, d.rkey.p, d.rays.p);
 				if (timing && k == 0) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv)], st), "event");
 				if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
 				else hipLaunchKernelGGL(k_trace<false>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
