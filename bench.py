@@ -313,6 +313,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--boxes", type=int, default=400)
     ap.add_argument("--no-linear", action="store_true", help="skip the explicit-matrix leg")
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="skip the end-to-end run (so a kernel trace holds the timed steps' launches only)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -331,20 +333,22 @@ def main():
     import ctypes
     ctypes.CDLL("libamdhip64.so").hipFree(ctypes.c_void_p(0))
     # end-to-end once: InitRadiosity through the final .fin, as the reference times it
-    with tempfile.TemporaryDirectory() as tmp:
-        # the other once-per-process device costs (loading the device code, the
-        # first allocations, a cold GPU's first launches) are start-up too: one
-        # run of the small room through the same path first
-        with radgpu.Engine(*(OPTS + ["-quiet", "1"])) as warm:
-            warm.set_scene(scenes.room_scene(1))
-            warm.run(os.path.join(tmp, "warm"))
-        with radgpu.Engine(*(OPTS + ["-quiet", "1"])) as e2e:
-            e2e.set_scene(scene)
-            t1 = time.perf_counter()
-            iters = e2e.run(os.path.join(tmp, "hall"))
-            e2e_s = time.perf_counter() - t1
-            e2e_t = e2e.timings()
-    log("end-to-end solve (init .. .fin): %.3f s, %d iterations" % (e2e_s, iters))
+    e2e_s, e2e_t = None, {"setup_ms": None, "upload_ms": None}
+    if not args.no_e2e:
+        with tempfile.TemporaryDirectory() as tmp:
+            # the other once-per-process device costs (loading the device code,
+            # the first allocations, a cold GPU's first launches) are start-up
+            # too: one run of the small room through the same path first
+            with radgpu.Engine(*(OPTS + ["-quiet", "1"])) as warm:
+                warm.set_scene(scenes.room_scene(1))
+                warm.run(os.path.join(tmp, "warm"))
+            with radgpu.Engine(*(OPTS + ["-quiet", "1"])) as e2e:
+                e2e.set_scene(scene)
+                t1 = time.perf_counter()
+                iters = e2e.run(os.path.join(tmp, "hall"))
+                e2e_s = time.perf_counter() - t1
+                e2e_t = e2e.timings()
+        log("end-to-end solve (init .. .fin): %.3f s, %d iterations" % (e2e_s, iters))
 
     eng = radgpu.Engine(*(OPTS + ["-quiet", "1", "-kernel_timing", "1"]))
     eng.set_scene(scene).prepare_host()
@@ -429,7 +433,7 @@ def main():
                                               (cb["rows"], scene.n))
         est = cb["init_est_s"] + out["cpu_baseline"]["time_to_converge_est_s"]
         out["cpu_baseline"]["end_to_end_est_s"] = est
-        out["speedup_end_to_end_vs_cpu_est"] = est / e2e_s
+        out["speedup_end_to_end_vs_cpu_est"] = est / e2e_s if e2e_s else None
         out["reference_drop_in_c1"] = reference_drop_in(cores)
     print(json.dumps(out), flush=True)
 
