@@ -295,3 +295,29 @@ def test_cli_zip_and_interactive_match_reference_binary(goldens, inputs, tmp_pat
     assert runs["eng"][1] == runs["ref"][1]
     assert runs["eng"][2] == runs["ref"][2] == ["../out.10", "../out.20", "../out.fin"]
     assert sorted(runs["eng"][3]) == sorted(runs["ref"][3])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scene", ["room", "hall_subset"])
+def test_reference_ray_count_matches_oracle(scene):
+    """The engine traces only the rays whose result is used; its ref_rays
+    counter adds the full-source rays AdaptCalFF casts and discards
+    (shade.cpp:442-450), so it equals the oracle's RayHitted count over the
+    same shots."""
+    if scene == "room":
+        s, opts, shots = scenes.room_scene(1), ["-converge", "1200", "-quiet", "1"], 1 << 30
+    else:
+        s, opts, shots = scenes.hall_scene(2, n_boxes=40), ["-converge", "1200", "-triangle", "1000000", "-quiet", "1"], 12
+    print("ref rays", scene, s.n, flush=True)
+    with radgpu.Engine(*opts) as eng:
+        eng.set_scene(s).prepare_host().upload()
+        n, _ = eng.shoot(shots)
+        c = eng.counters()
+    o = ol.Oracle(*(opts + ["-jobs", "16"]))
+    o.load(s)
+    o.build_tree()
+    o.init_radiosity()
+    o.prepartition()
+    on, _ = o.iterate(shots)
+    oc = o.counters()
+    assert n == on and c["ref_rays"] == int(oc[0]) and 0 < c["rays"] <= c["ref_rays"]
