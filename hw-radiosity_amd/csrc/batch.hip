@@ -303,6 +303,9 @@ struct SceneBatch::Impl {
 	int ensure_lowp(int prec, std::string &err)
 	{
 		if ((prec == PREC_F16 && have16) || (prec == PREC_BF16 && haveb)) return 0;
+		// a captured graph bakes in the FP16 scale (its sweeps multiply by
+		// 1/s16) and reads the copy being rebuilt: capture again
+		drop_graph();
 		if (prec == PREC_F16) {
 			if (umax.alloc(1, err) || F16.alloc(fsize, err)) return -1;
 			hipLaunchKernelGGL(k_fill<unsigned>, dim3(1), dim3(kB), 0, st, umax.p, (size_t)1, 0u);

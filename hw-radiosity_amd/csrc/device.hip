@@ -1354,6 +1354,9 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	    d.raycnt.alloc((size_t)d.nblocks + 1, err) || d.rays.alloc((size_t)n * 6 + 1, err))
 		return -1;
 	std::memset(d.host_loop, 0, sizeof(Loop));
+	// -parallel_src 0: listCanditate returns no source (rad.cpp:400), so the
+	// first iteration finds nothing to shoot
+	if (!d.lowest && d.parallel_src <= 0) d.host_loop->done = 1;
 	if (d.loop.put(d.host_loop, 1, st, err)) return -1;
 	const float thr_t = smallest_float_at_least(0.0001);
 	d.S = DScene{d.lnc.p, d.lse.p, d.cell_leaf.p, d.leaf_box.p, s.G0[0], s.G0[1], s.G0[2], s.G1[0], s.G1[1], s.G1[2],

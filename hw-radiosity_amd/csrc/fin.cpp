@@ -143,20 +143,21 @@ char *format_record(const Patch &t, bool old_format, char *last, int &ll, char *
 
 // The state `last` after formatting record t, given it before: the last of
 // its values (formatting order n[0..2], p[0][0..2], .., p[2][0..2]) that
-// converts, if any.
-void last_after(const Patch &t, char *last)
+// converts.  Returns false, leaving `last` as it was, when none converts.
+bool last_after(const Patch &t, char *last)
 {
 	char tmp[48];
 	for (int q = 8; q >= 0; q--)
 		if (format_truncated(t.p[q / 3][q % 3], tmp)) {
 			std::strcpy(last, tmp);
-			return;
+			return true;
 		}
 	for (int k = 2; k >= 0; k--)
 		if (format_truncated(t.n[k], tmp)) {
 			std::strcpy(last, tmp);
-			return;
+			return true;
 		}
+	return false;
 }
 
 }  // namespace
@@ -176,11 +177,9 @@ int FinWriter::write(const std::string &path, const Patch *pool, int trinum, boo
 	for (int c = 0; c < nch; c++) {
 		std::memcpy(start[c].data(), last_, sizeof last_);
 		if (old_format) continue;
+		// the latest written record of the chunk with a value that converts
 		for (int i = std::min(trinum, (c + 1) * kChunk) - 1; i >= c * kChunk; i--)
-			if (pool[i].parent >= 0) {
-				last_after(pool[i], last_);
-				break;
-			}
+			if (pool[i].parent >= 0 && last_after(pool[i], last_)) break;
 	}
 	bool ok = true;
 	// chunks are formatted in parallel and written in order as they finish;

@@ -43,7 +43,13 @@ bool parse_option(Options &o, int argc, const char *const *argv, int &i, std::st
 	if (a == "-o") { if (!value(v)) return false; o.out = v; return true; }
 	if (a == "-interactive") { o.interactive = 1; return true; }
 	if (a == "-jobs") return as_int(o.jobs);
-	if (a == "-parallel_src") return as_int(o.parallel_src);
+	if (a == "-parallel_src") {
+		// listCanditate (rad.cpp:400) takes min(ceil(|C|/5), value) sources:
+		// 0 shoots nothing; a negative value is undefined behaviour there
+		if (!as_int(o.parallel_src)) return false;
+		if (o.parallel_src < 0) { err = "bad value for -parallel_src: must be >= 0"; return false; }
+		return true;
+	}
 	if (a == "-fma") return as_int(o.fma);
 	if (a == "-max_tri") return as_int(o.max_tri);
 	if (a == "-old_format") return as_int(o.old_format);

@@ -30,6 +30,9 @@ def test_errors_are_statuses_not_exits(tmp_path):
         radgpu.Engine("-write_cycle")          # missing value
     with pytest.raises(radgpu.RadError):
         radgpu.Engine("-nonsense", "1")
+    with pytest.raises(radgpu.RadError) as ei:
+        radgpu.Engine("-parallel_src", "-1")   # undefined behaviour in listCanditate (rad.cpp:400)
+    assert ei.value.status == radgpu.RAD_EINVAL
     with radgpu.Engine("-quiet", "1") as e:
         with pytest.raises(radgpu.RadError) as ei:
             e.load_tri(str(tmp_path / "missing.tri"))

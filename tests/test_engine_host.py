@@ -207,6 +207,40 @@ def test_fin_unconvertible_values_across_chunks(tmp_path):
     assert a == b
 
 
+def test_fin_wholly_unconvertible_record_at_chunk_end(tmp_path):
+    """A record none of whose 12 values converts (coordinates of 2^64 or
+    more and the NaN normal of a zero-area triangle) at the end of a 4096-record
+    chunk: the next chunk starts from the text of the latest earlier record
+    that does convert, as the sequential writer carries it."""
+    from radgpu import scenes
+    rng = np.random.default_rng(13)
+    n = 8400
+    base = rng.uniform(0, 50, (n, 1, 3)).astype(np.float32)
+    p = (base + rng.uniform(0, 1, (n, 3, 3))).astype(np.float32)
+    huge = np.array([1e20, 2e20, 3e20], np.float32)
+    for i in (4094, 4095, 8190, 8191):
+        p[i, :] = huge                 # every coordinate unconvertible, normal 0/0
+    for i in (4093, 4096, 8192):
+        p[i, 1] = p[i, 0]
+        p[i, 2] = p[i, 0]              # convertible coordinates, NaN normal
+    s = scenes.Scene(p=p, frgb=rng.integers(0, 255, (n, 3)).astype(np.int32),
+                     brgb=np.zeros((n, 3), np.int32))
+    opts = ["-triangle", "20000", "-quiet", "1"]
+    o = ol.Oracle(*opts)
+    o.load(s)
+    o.write_fin(str(tmp_path / "o.fin"))
+    with radgpu.Engine(*opts) as eng:
+        eng.set_scene(s)
+        eng.write_fin(str(tmp_path / "e.fin"))
+    a, b = open(tmp_path / "o.fin", "rb").read(), open(tmp_path / "e.fin", "rb").read()
+    recs = a.split(b"Triangle \n")[1:]
+    assert len(recs) == n
+    carried = recs[4093].split()[-4]   # 4093's last coordinate, before its repeated normal
+    assert recs[4095].split(b"\n")[1].split() == [carried] * 6   # 4094/4095 print it throughout
+    assert recs[4096].split(b"\n")[1].split()[3:6] == [carried] * 3  # so does 4096's normal
+    assert a == b
+
+
 @pytest.mark.parametrize("name", ["empty", "lone_light", "facing_pair", "no_light", "room_zero_area"])
 def test_edge_scene_host_stages_match_oracle(name):
     """Host stages of the edge-case scenes (no triangles, a lone light, no

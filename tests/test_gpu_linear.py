@@ -391,3 +391,38 @@ def test_benchmark_dense_reference_matches_cpu_lu(sys2k, tmp_path):
                        text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     assert rel(np.load(tmp_path / "B.npy"), lo.dense_solve(F, rho, E)) < 1e-12
+
+
+def test_batch_graph_recaptured_after_matrix_change():
+    """A cached graph bakes in the FP16 scale of the matrices it was captured
+    for; replacing the matrices by a quarter of themselves (the power-of-two
+    FP16 scale moves by 2^2) must give the dense
+    LU answer of the new system, not Jacobi sweeps with a stale scale."""
+    sizes = [300, 500]
+    rng = np.random.default_rng(41)
+    Fs = []
+    for n in sizes:
+        F, _, _ = _random_system(n, int(rng.integers(1 << 30)), 3, scale=0.9)
+        Fs.append(F)
+    rho = rng.uniform(0.2, 0.8, (sum(sizes), 3))
+    E = np.where(rng.random((sum(sizes), 3)) < 0.2, 1.0, 0.0)
+    E[0] = E[sizes[0]] = 1.0
+    b = radgpu.SceneBatch(sizes)
+    off = b.offsets
+    try:
+        for s, F in enumerate(Fs):
+            b.set_matrix(s, F)
+        B1, r1, i1 = b.solve(rho, E, prec="f16", graph=True, tol=1e-11, outer=12, sweeps=6)
+        assert i1["above_tol"] == 0
+        F2 = [Fs[0] * np.float32(0.25), Fs[1] * np.float32(0.25)]
+        for s, F in enumerate(F2):
+            b.set_matrix(s, F)
+        B2, r2, i2 = b.solve(rho, E, prec="f16", graph=True, tol=1e-11, outer=12, sweeps=6)
+        print(i1, i2, flush=True)
+        assert i2["build_ms"] > 0           # captured again
+        assert i2["above_tol"] == 0 and np.all(r2 <= 1e-11)
+        for s, F in enumerate(F2):
+            ref = lo.dense_solve(F.astype(np.float64), rho[off[s]:off[s + 1]], E[off[s]:off[s + 1]])
+            assert rel(B2[off[s]:off[s + 1]], ref) < 1e-9, s
+    finally:
+        b.close()

@@ -321,3 +321,33 @@ def test_reference_ray_count_matches_oracle(scene):
     on, _ = o.iterate(shots)
     oc = o.counters()
     assert n == on and c["ref_rays"] == int(oc[0]) and 0 < c["rays"] <= c["ref_rays"]
+
+
+def test_parallel_src_zero_shoots_nothing(inputs, tmp_path):
+    """-parallel_src 0: listCanditate takes min(ceil(|C|/5), 0) = 0 sources
+    (rad.cpp:400), so the first iteration finds none and the run ends at
+    iteration 1 -- as the reference binary built from its sources and the
+    oracle do."""
+    import re
+    import subprocess
+    opts = ["-converge", "1200", "-triangle", "20000", "-parallel_src", "0", "-write_cycle", "1000000"]
+    r = radgpu.run_cli(opts + [inputs["c1_seed1.tri"], "-o", str(tmp_path / "eng")], timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "Complete in #Iterations 1\n" in re.sub(r"\x1b\[[0-9;]*m", "", r.stderr)
+    fin = open(tmp_path / "eng.fin", "rb").read()
+    o = ol.Oracle(*(opts + ["-quiet", "1"]))
+    o.load_file(inputs["c1_seed1.tri"])
+    o.build_tree()
+    o.init_radiosity()
+    o.prepartition()
+    shots, conv = o.iterate(1 << 30)
+    assert shots == 0 and conv
+    o.write_fin(str(tmp_path / "o.fin"))
+    assert fin == open(tmp_path / "o.fin", "rb").read()
+    ref = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "rad")
+    if os.path.exists(ref):
+        rr = subprocess.run([ref] + opts + ["-jobs", "2", inputs["c1_seed1.tri"], "-o", str(tmp_path / "ref")],
+                            capture_output=True, text=True, timeout=300)
+        assert rr.returncode == 0
+        assert "Complete in #Iterations 1\n" in re.sub(r"\x1b\[[0-9;]*m", "", rr.stderr)
+        assert fin == open(tmp_path / "ref.fin", "rb").read()
