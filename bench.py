@@ -2,18 +2,23 @@
 """Benchmark: plain progressive shooting at scale on one MI355X.
 
 Workload ("hall-200k"): the seeded 100 x 100 x 30 hall with 400 yawed boxes
-and 8 emissive quads, about 206k triangles, run with -converge 1200
--delta_ff 5e-6 -adapt_area 5 -triangle 1000000.  A step is one full solve:
-the prepared scene is uploaded to HBM (outside the timed region), then the
-device-resident shooting loop runs to convergence (timed).
+and 8 emissive quads, about 206k triangles.  The headline solve runs with
+-converge 175 -delta_ff 5e-6 -adapt_area 5 -triangle 1000000: the 64 emitter
+shots, then the lit receivers shooting on (interreflection), over a thousand
+shots in all.  A step is one full solve: the prepared scene is uploaded to
+HBM (outside the timed region), then the device-resident shooting loop runs
+to convergence (timed).
 
 value = rays traced per second over the timed steps.  The JSON line also
 carries the reference-equivalent ray rate (rays the reference would cast for
 the same solve, including AdaptCalFF's discarded full-source ray), the
-time-to-converge, the roofline of the fused shoot kernel and a CPU baseline:
+time-to-converge, the roofline of the ray-tracing kernel and a CPU baseline:
 the oracle (a faithful CPU restatement of the reference, OpenMP over
 receivers) timed on this host's cores over a bounded sample of the same
-solve.
+solve, with its records compared to the engine's; the reference binary built
+from its sources timed beside the oracle and the engine's drop-in `rad` on
+the reference's own models; and the direct-light solve (-converge 1200, the
+emitters only) run to the end on both sides and compared record for record.
 """
 import argparse
 import json
@@ -28,18 +33,47 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hw-radiosity_amd"))
 
 METRIC = "rays/s + time-to-converge at 1e-3 rel-L2 vs FP64; mixed/batched vs plain, 1 GPU"
-OPTS = ["-converge", "1200", "-delta_ff", "5e-6", "-adapt_area", "5", "-triangle", "1000000",
-        "-write_cycle", "1000000"]
+COMMON = ["-delta_ff", "5e-6", "-adapt_area", "5", "-triangle", "1000000", "-write_cycle", "1000000"]
+HEADLINE_CONVERGE = "175"    # receivers shoot on after the emitters: interreflection
+DIRECT_CONVERGE = "1200"     # the emitters only (64 shots on this scene)
+OPTS = ["-converge", HEADLINE_CONVERGE] + COMMON
+DIRECT_OPTS = ["-converge", DIRECT_CONVERGE] + COMMON
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = 2500.0  # dense FP16/BF16
+# VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes per clock at 2.4 GHz (a
+# wave64 VALU instruction occupies its SIMD for 4 clocks), in G lane-ops/s
+VALU_PEAK_GLANES = 256 * 4 * 16 * 2.4
 SWEEP_CHOICES = (4, 6, 8, 12, 16)
-# measured HBM bytes per k_trace dispatch on this workload (rocprofv3 --pmc
-# FETCH_SIZE + WRITE_SIZE, copy-calibrated); see profiles/
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_trace_hall200k_pmc.json")
+# rocprofv3 summaries of k_trace on the headline workload: SQ counters per
+# dispatch (VALU thread-instructions) and HBM bytes per dispatch (FETCH_SIZE
+# + WRITE_SIZE, copy-calibrated)
+VALU_SUMMARY = os.path.join(ROOT, "profiles", "k_trace_hall200k_c175_sq.json")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_trace_hall200k_c175_pmc.json")
+MODELS = os.path.join(ROOT, "tests", "golden", "models")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_threads():
+    """Host threads for the CPU legs: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS where the machine sets it (a GPU box sets its
+    share of the host that way)."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return (min(aff, int(omp)) if omp.isdigit() and int(omp) > 0 else aff), aff
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def algorithmic_bytes(cnt):
@@ -49,61 +83,101 @@ def algorithmic_bytes(cnt):
     return 40 * cnt["rays"] + 32 * cnt["cells"] + 8 * cnt["levels"] + 76 * cnt["tests"]
 
 
-def reference_drop_in(cores):
-    """The shipped reference binary (oracle/_ref/rad, built from the
-    reference's sources by oracle/ref.mk) against the engine's drop-in `rad`
-    on the reference's own CPU-runnable case (the seeded C1 room): the
-    "Radiosity Method took" span of each (InitRadiosity .. final .fin) and
-    whether the .fin files are byte-identical."""
+def _load_json(path):
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh)
+    return None
+
+
+def _run_timed(exe, args, timeout=600, cwd=None):
+    """One `rad`-style run: its "Radiosity Method took" span, iterations and
+    the .fin md5."""
     import hashlib
     import re
     import subprocess
+    t0 = time.perf_counter()
+    try:
+        r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=timeout, cwd=cwd)
+    except (OSError, subprocess.TimeoutExpired) as e:
+        return {"error": str(e)}
+    wall = time.perf_counter() - t0
+    err = re.sub(r"\x1b\[[0-9;]*m", "", r.stderr)
+    m = re.search(r"Radiosity Method took \S*?([0-9.]+)", err)
+    it = re.search(r"#Iterations (\d+)", err) or re.search(r"iterations (\d+)", err)
+    prefix = args[args.index("-o") + 1]
+    fin = prefix + ".fin"
+    return {"rc": r.returncode, "wall_s": wall, "method_s": float(m.group(1)) if m else None,
+            "iterations": int(it.group(1)) if it else None,
+            "fin_md5": hashlib.md5(open(fin, "rb").read()).hexdigest() if os.path.exists(fin) else None}
+
+
+def reference_beside(threads):
+    """The reference binary built from its sources (oracle/_ref/rad), the
+    oracle's CLI (oracle/_bin/refsem, with the reference's exact O(N^2)
+    neighbour pass) and the engine's drop-in `rad` on the reference's own
+    models at -triangle 40000 (its published workloads, doc/README.md:404-408)
+    and on the seeded C1 room: the "Radiosity Method took" span of each
+    (InitRadiosity .. final .fin), iterations, and whether the .fin files are
+    byte-identical.  Gives the oracle/reference speed ratio that anchors the
+    CPU baseline, and the drop-in's speed-up over the reference."""
+    import gzip
+    import shutil
     from radgpu import scenes
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "rad")
+    sem_bin = os.path.join(ROOT, "oracle", "_bin", "refsem")
     gpu_bin = os.path.join(ROOT, "hw-radiosity_amd", "_bin", "rad")
     if not os.path.exists(ref_bin):
         return {"skipped": "oracle/_ref/rad not built"}
     out = {}
     with tempfile.TemporaryDirectory() as tmp:
-        tri = os.path.join(tmp, "c1.tri")
-        scenes.write_tri(tri, scenes.room_scene(1))
-        for name, opts in (("c1 (-converge 1200 -delta_ff 5e-6 -adapt_area 5 -triangle 20000)",
-                            ["-converge", "1200", "-delta_ff", "5e-6", "-adapt_area", "5", "-triangle", "20000"]),
-                           ("c1 at -converge 100", ["-converge", "100", "-triangle", "20000"])):
+        cases = []
+        for m in ("room", "hall", "church"):
+            tri = os.path.join(tmp, m + ".tri")
+            with gzip.open(os.path.join(MODELS, m + ".tri.gz"), "rb") as fi, open(tri, "wb") as fo:
+                shutil.copyfileobj(fi, fo)
+            cases.append(("%s.tri -triangle 40000" % m, tri, ["-triangle", "40000"]))
+        c1 = os.path.join(tmp, "c1.tri")
+        scenes.write_tri(c1, scenes.room_scene(1))
+        cases.append(("c1 (-converge 1200 -delta_ff 5e-6 -adapt_area 5 -triangle 20000)", c1,
+                      ["-converge", "1200", "-delta_ff", "5e-6", "-adapt_area", "5", "-triangle", "20000"]))
+        cases.append(("c1 at -converge 100", c1, ["-converge", "100", "-triangle", "20000"]))
+        for name, tri, opts in cases:
             row = {}
-            for who, exe in (("reference", ref_bin), ("engine", gpu_bin)):
-                prefix = os.path.join(tmp, who)
-                t0 = time.perf_counter()
-                try:
-                    r = subprocess.run([exe] + opts + ["-jobs", str(cores), tri, "-o", prefix], capture_output=True,
-                                       text=True, timeout=300)
-                except (OSError, subprocess.TimeoutExpired) as e:
-                    row[who] = {"error": str(e)}
+            for who, exe, extra in (("reference", ref_bin, []), ("oracle", sem_bin, ["-neighbors", "exact"]),
+                                    ("engine", gpu_bin, [])):
+                if who == "oracle" and name.startswith("c1"):
                     continue
-                wall = time.perf_counter() - t0
-                m = re.search(r"Radiosity Method took \S*?([0-9.]+)", re.sub(r"\x1b\[[0-9;]*m", "", r.stderr))
-                it = re.search(r"#Iterations (\d+)", r.stderr)
-                fin = prefix + ".fin"
-                row[who] = {"rc": r.returncode, "wall_s": wall, "method_s": float(m.group(1)) if m else None,
-                            "iterations": int(it.group(1)) if it else None,
-                            "fin_md5": hashlib.md5(open(fin, "rb").read()).hexdigest() if os.path.exists(fin) else None}
-            if "method_s" in row.get("reference", {}) and "method_s" in row.get("engine", {}):
-                a, b = row["reference"]["method_s"], row["engine"]["method_s"]
-                row["speedup_method_span"] = a / b if a and b else None
-                row["fin_identical"] = row["reference"]["fin_md5"] == row["engine"]["fin_md5"]
+                row[who] = _run_timed(exe, opts + extra + ["-write_cycle", "1000000", "-jobs", str(threads), tri,
+                                                           "-o", os.path.join(tmp, who)])
+            ref, eng = row["reference"], row["engine"]
+            if ref.get("method_s") and eng.get("method_s"):
+                row["speedup_method_span"] = ref["method_s"] / eng["method_s"]
+                row["fin_identical"] = ref["fin_md5"] == eng["fin_md5"]
+            if "oracle" in row and ref.get("method_s") and row["oracle"].get("method_s"):
+                row["oracle_over_reference_time"] = row["oracle"]["method_s"] / ref["method_s"]
+                row["oracle_fin_identical"] = row["oracle"]["fin_md5"] == ref["fin_md5"]
             out[name] = row
-            log("drop-in %s: %s" % (name, row))
+            log("beside the reference, %s: %s" % (name, row))
     return out
 
 
-def cpu_baseline(scene, seconds, cores):
+def _oracle_for(scene, opts, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
-    o = ol.Oracle(*(OPTS + ["-quiet", "1", "-jobs", str(cores)]))
+    o = ol.Oracle(*(opts + ["-quiet", "1", "-jobs", str(threads)]))
     o.load(scene)
     o.build_tree()
-    # InitRadiosity's O(N^2) neighbour pass as the reference runs it: a
-    # sample of rows, extrapolated to all of them
+    return o, ol
+
+
+def cpu_baseline(scene, seconds, threads, engine_ref_rays, engine_shots):
+    """The oracle on the headline solve: InitRadiosity's O(N^2) neighbour
+    pass sampled on 400 rows, then as many shots as fit in `seconds`; its
+    RayHitted calls per second, the time-to-converge that rate gives for the
+    whole solve's reference-equivalent rays, and its records after those
+    shots (for the comparison with the engine's)."""
+    o, ol = _oracle_for(scene, OPTS, threads)
     rows = 400
     t0 = time.perf_counter()
     o.neighbor_rows(rows)
@@ -111,15 +185,82 @@ def cpu_baseline(scene, seconds, cores):
     o.init_radiosity()
     o.prepartition()
     t0 = time.perf_counter()
-    shots = 0
+    shots, conv = 0, False
+    step = 4
     while time.perf_counter() - t0 < seconds:
-        d, conv = o.iterate(1)
+        d, conv = o.iterate(step)
         shots += d
         if conv:
             break
     dt = time.perf_counter() - t0
     c = o.counters()
-    return {"rays": int(c[ol.CNT_RAYS]), "seconds": dt, "shots": shots, "init_est_s": init_est, "rows": rows}
+    rays = int(c[ol.CNT_RAYS])
+    rec, _ = o.dump()
+    return {"rays": rays, "seconds": dt, "shots": shots, "init_est_s": init_est, "rows": rows,
+            "converged": bool(conv), "records": rec,
+            "time_to_converge_est_s": dt * engine_ref_rays / max(rays, 1) if shots < engine_shots else dt}
+
+
+def direct_light_leg(scene, threads, steps):
+    """The direct-light solve (-converge 1200: the 64 emitter shots) on the
+    device and, to the end, on the oracle: both times measured, records
+    compared."""
+    import radgpu
+    eng = radgpu.Engine(*(DIRECT_OPTS + ["-quiet", "1"]))
+    eng.set_scene(scene).prepare_host()
+    eng.upload()
+    eng.shoot()
+    tot, shots = 0.0, 0
+    for _ in range(steps):
+        eng.upload()
+        t0 = time.perf_counter()
+        shots, conv = eng.shoot()
+        tot += time.perf_counter() - t0
+        assert conv
+    cnt = eng.counters()
+    rec, _ = eng.records()
+    eng.close()
+    dev_s = tot / steps
+    out = {"options": " ".join(DIRECT_OPTS), "shots": shots, "device_time_to_converge_s": dev_s,
+           "rays_per_s": cnt["rays"] / dev_s}
+    o, ol = _oracle_for(scene, DIRECT_OPTS, threads)
+    o.init_radiosity()
+    o.prepartition()
+    t0 = time.perf_counter()
+    oshots, oconv = o.iterate(1 << 30)
+    odt = time.perf_counter() - t0
+    orec, _ = o.dump()
+    out.update({"oracle_time_to_converge_s": odt, "oracle_shots": oshots, "oracle_threads": threads,
+                "speedup_device_vs_oracle": odt / dev_s,
+                "output_identical": bool(oconv and oshots == shots and rec.tobytes() == orec.tobytes())})
+    log("direct light: %s" % out)
+    return out
+
+
+def c1_iteration_latency():
+    """Per-iteration device time of the small room (C1, 2k triangles) at
+    -converge 100 (1000 shots), where launch latency, not work, bounds the
+    loop."""
+    import radgpu
+    from radgpu import scenes
+    out = {}
+    for conv in ("100", "1200"):
+        with radgpu.Engine("-converge", conv, "-triangle", "20000", "-quiet", "1") as e:
+            e.set_scene(scenes.room_scene(1)).prepare_host()
+            best = None
+            for _ in range(3):
+                e.upload()
+                t0 = time.perf_counter()
+                n, conv_ = e.shoot()
+                wall = time.perf_counter() - t0
+                t = e.timings()
+                row = {"shots": n, "wall_ms": 1e3 * wall, "device_ms": t["shoot_ms"],
+                       "device_us_per_iteration": 1e3 * t["shoot_ms"] / max(t["iterations_launched"], 1)}
+                if best is None or row["wall_ms"] < best["wall_ms"]:
+                    best = row
+        out["converge_" + conv] = best
+    log("c1 iteration latency: %s" % out)
+    return out
 
 
 def dense_reference(F, rho, E, tmp):
@@ -304,17 +445,49 @@ def linear_leg(n=32768, n_adj=16384):
     return out
 
 
+def k_trace_roofline(kernel_ms, launches, bytes_per_launch):
+    """The ray-tracing kernel's roofline.  Its working set (octree, lists,
+    triangle records: ~14 MB) is served by L2 and the MALL, so the bound it
+    can reach is VALU issue: lane-operations per launch (SQ_THREAD_CYCLES_VALU
+    per dispatch from the committed rocprofv3 summary of this workload) over
+    the launch's duration (HIP events around every launch, an untimed solve),
+    against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz.  The HBM view is kept
+    beside it: the algorithmic bytes (which exceed HBM peak because they are
+    cache hits) and the measured HBM traffic."""
+    valu = _load_json(VALU_SUMMARY) or {}
+    pmc = _load_json(PMC_SUMMARY) or {}
+    traffic = pmc.get("traffic_bytes_per_dispatch")
+    lanes = valu.get("thread_valu_per_dispatch")
+    sec = kernel_ms * 1e-3
+    achieved = lanes / sec / 1e9 if (lanes and sec > 0) else None
+    alg_gbs = bytes_per_launch / sec / 1e9 if sec > 0 else None
+    return {"kernel": "k_trace (octree-DDA visibility of one source's rays)",
+            "bound": "valu", "achieved": achieved, "peak": VALU_PEAK_GLANES, "unit": "G lane-ops/s",
+            "frac": achieved / VALU_PEAK_GLANES if achieved else None,
+            "traffic": traffic, "avg_kernel_ms": kernel_ms, "launches": launches,
+            "valu_issue_frac": valu.get("valu_issue_frac"), "valu_lane_use": valu.get("valu_lane_use"),
+            "hbm": {"algorithmic_bytes_per_launch": bytes_per_launch, "algorithmic_GBps": alg_gbs,
+                    "algorithmic_frac_of_peak": alg_gbs / HBM_PEAK_GBS if alg_gbs else None,
+                    "measured_traffic_GBps": traffic / sec / 1e9 if (traffic and sec > 0) else None,
+                    "measured_traffic_frac_of_peak": traffic / sec / 1e9 / HBM_PEAK_GBS if (traffic and sec > 0) else None,
+                    "peak": HBM_PEAK_GBS, "fetch_scale": (pmc.get("FETCH_SIZE") or {}).get("scale"),
+                    "write_scale": (pmc.get("WRITE_SIZE") or {}).get("scale")},
+            "profiles": [os.path.relpath(VALU_SUMMARY, ROOT), os.path.relpath(PMC_SUMMARY, ROOT)],
+            "note": "frac = VALU lane-ops per launch / duration / VALU issue peak; the algorithmic bytes "
+                    "(40 + 32 C + 8 D + 76 T per ray) are cache hits, so their rate is not an HBM roofline"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs")
     ap.add_argument("--boxes", type=int, default=400)
     ap.add_argument("--no-linear", action="store_true", help="skip the explicit-matrix leg")
     ap.add_argument("--no-e2e", action="store_true",
-                    help="skip the end-to-end run (so a kernel trace holds the timed steps' launches only)")
+                    help="skip every run but the timed steps (so a kernel trace holds their launches only)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -333,7 +506,7 @@ def main():
     import ctypes
     ctypes.CDLL("libamdhip64.so").hipFree(ctypes.c_void_p(0))
     # end-to-end once: InitRadiosity through the final .fin, as the reference times it
-    e2e_s, e2e_t = None, {"setup_ms": None, "upload_ms": None}
+    e2e_s, e2e_t, iters = None, {"setup_ms": None, "upload_ms": None}, None
     if not args.no_e2e:
         with tempfile.TemporaryDirectory() as tmp:
             # the other once-per-process device costs (loading the device code,
@@ -350,14 +523,15 @@ def main():
                 e2e_t = e2e.timings()
         log("end-to-end solve (init .. .fin): %.3f s, %d iterations" % (e2e_s, iters))
 
-    eng = radgpu.Engine(*(OPTS + ["-quiet", "1", "-kernel_timing", "1"]))
+    # the timed steps: no per-kernel events, no host wait inside a batch
+    eng = radgpu.Engine(*(OPTS + ["-quiet", "1"]))
     eng.set_scene(scene).prepare_host()
     rec, _ = eng.records()
     slots = len(rec)
     for w in range(args.warmup):
         eng.upload()
         eng.shoot()
-    total_s, rays, ref_rays, shots_all, kms, kn = 0.0, 0, 0, 0, 0.0, 0
+    total_s, shots_all = 0.0, 0
     cnt_sum = None
     for k in range(args.steps):
         eng.upload()
@@ -366,22 +540,28 @@ def main():
         dt = time.perf_counter() - ts
         assert conv, "did not converge"
         c = eng.counters()
-        t = eng.timings()
         total_s += dt
         shots_all += shots
-        kms += t["kernel_ms"]
-        kn += t["kernels_timed"]
         cnt_sum = c if cnt_sum is None else {x: cnt_sum[x] + c[x] for x in c}
         log("step %d: %d shots, %.3f s, %d rays" % (k, shots, dt, c["rays"]))
+    final_records, _ = eng.records()
+    eng.close()
     rays, ref_rays = cnt_sum["rays"], cnt_sum["ref_rays"]
-    kernel_avg_ms = kms / max(kn, 1)
-    bytes_per_launch = algorithmic_bytes(cnt_sum) / max(kn, 1)
-    achieved = bytes_per_launch / (kernel_avg_ms * 1e-3) / 1e9 if kernel_avg_ms > 0 else 0.0
     value = rays / total_s
-    traffic = None
-    if os.path.exists(PMC_SUMMARY):
-        with open(PMC_SUMMARY) as fh:
-            traffic = json.load(fh).get("traffic_bytes_per_dispatch")
+    solve_ref_rays = ref_rays // args.steps
+    solve_shots = shots_all // args.steps
+
+    # the ray-tracing kernel's own duration: HIP events around every launch
+    # on the engine's stream, in one more (untimed) solve of the same workload
+    roofline = None
+    if not args.no_e2e:
+        with radgpu.Engine(*(OPTS + ["-quiet", "1", "-kernel_timing", "1"])) as ek:
+            ek.set_scene(scene).prepare_host().upload()
+            ek.shoot()
+            t = ek.timings()
+            ck = ek.counters()
+        kn = max(t["kernels_timed"], 1)
+        roofline = k_trace_roofline(t["kernel_ms"] / kn, t["kernels_timed"], algorithmic_bytes(ck) / kn)
     out = {
         "metric": METRIC,
         "value": value,
@@ -395,46 +575,55 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded hall scene, see hw-radiosity_amd/radgpu/scenes.py)",
-        "config": {"workload": "hall-200k: plain progressive shooting to convergence (config 2)",
-                   "triangles": scene.n, "slots_after_prepartition": slots, "shots_per_solve": shots_all // args.steps,
+        "config": {"workload": "hall-200k: plain progressive shooting to convergence with interreflection "
+                               "(config 2, -converge %s)" % HEADLINE_CONVERGE,
+                   "triangles": scene.n, "slots_after_prepartition": slots, "shots_per_solve": solve_shots,
                    "options": " ".join(OPTS), "parallelism": "single GPU"},
         "ref_equivalent_rays_per_s": ref_rays / total_s,
         "time_to_converge_s": {"device_shooting": total_s / args.steps, "end_to_end_init_to_fin": e2e_s,
                                "host_setup_ms": e2e_t["setup_ms"], "upload_ms": e2e_t["upload_ms"]},
-        "roofline": {"kernel": "k_trace (octree-DDA visibility of the iteration's rays)",
-                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "avg_kernel_ms": kernel_avg_ms, "launches": kn,
-                     "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "note": "algorithmic bytes count every octree and triangle byte a ray reads; most "
-                             "are L2/MALL hits (traffic = measured HBM bytes per launch), so frac above "
-                             "1 means the kernel is latency/divergence-bound, not HBM-bound; rocprofv3 "
-                             "SQ counters of the same workload put its VALU lane use at 0.44 "
-                             "(profiles/k_trace_hall200k_sq.md)"},
+        "roofline": roofline,
         "counters": cnt_sum,
     }
-    eng.close()
     if not args.no_linear:
         out["linear"] = linear_leg()
-    if not args.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
-        cb = cpu_baseline(scene, args.cpu_seconds, cores)
+    if not args.no_cpu and not args.no_e2e:
+        threads, aff = host_threads()
+        cb = cpu_baseline(scene, args.cpu_seconds, threads, solve_ref_rays, solve_shots)
         cpu_rate = cb["rays"] / cb["seconds"]
-        per_shot = cb["seconds"] / max(cb["shots"], 1)
-        out["cpu_baseline"] = {"value": cpu_rate, "unit": "rays/s", "cores": cores, "kind": "port",
-                               "sample": "oracle (refsem, -O2 OpenMP) on the same hall scene: first %d shots of the "
-                                         "solve, %.1f s; faithful to the reference, so it also casts AdaptCalFF's "
-                                         "discarded full-source rays" % (cb["shots"], cb["seconds"]),
-                               "time_to_converge_est_s": per_shot * shots_all / args.steps}
-        out["speedup_time_to_converge_device"] = out["cpu_baseline"]["time_to_converge_est_s"] / (total_s / args.steps)
+        # the engine after the same number of shots, for the record comparison
+        with radgpu.Engine(*(OPTS + ["-quiet", "1"])) as ep:
+            ep.set_scene(scene).prepare_host().upload()
+            done, _ = ep.shoot(cb["shots"])
+            erec, _ = ep.records()
+        same = done == cb["shots"] and erec.tobytes() == cb["records"].tobytes()
+        if cb["converged"]:
+            same = same and erec.tobytes() == final_records.tobytes()
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "rays/s", "cores": threads, "kind": "port",
+                               "cpu_model": cpu_model(), "host_cpus_available": aff,
+                               "sample": "oracle (refsem, -O2 OpenMP, %d threads) on the same hall solve: its first "
+                                         "%d shots (%d from receivers), %.1f s; faithful to the reference, so it also "
+                                         "casts AdaptCalFF's discarded full-source rays (value counts them)" % (
+                                             threads, cb["shots"], max(cb["shots"] - 64, 0), cb["seconds"]),
+                               "output_identical_after_sample_shots": bool(same),
+                               "time_to_converge_est_s": cb["time_to_converge_est_s"],
+                               "time_to_converge_est_method": "sample seconds x (the solve's reference-equivalent "
+                                                              "rays / the rays of the sample)"}
+        out["speedup_time_to_converge_device"] = cb["time_to_converge_est_s"] / (total_s / args.steps)
         out["cpu_baseline"]["init_neighbour_pass_est_s"] = cb["init_est_s"]
         out["cpu_baseline"]["init_sample"] = ("the reference's exact O(N^2) neighbour pass (rad.cpp:295-334, OpenMP "
                                               "over j) on the first %d rows, extrapolated to all %d" %
                                               (cb["rows"], scene.n))
-        est = cb["init_est_s"] + out["cpu_baseline"]["time_to_converge_est_s"]
+        est = cb["init_est_s"] + cb["time_to_converge_est_s"]
         out["cpu_baseline"]["end_to_end_est_s"] = est
         out["speedup_end_to_end_vs_cpu_est"] = est / e2e_s if e2e_s else None
-        out["reference_drop_in_c1"] = reference_drop_in(cores)
+        out["direct_light"] = direct_light_leg(scene, threads, max(1, min(args.steps, 5)))
+        beside = reference_beside(threads)
+        out["reference_beside"] = beside
+        ratios = [r["oracle_over_reference_time"] for r in beside.values()
+                  if isinstance(r, dict) and "oracle_over_reference_time" in r]
+        out["cpu_baseline"]["oracle_over_reference_time"] = ratios
+        out["c1_iteration_latency"] = c1_iteration_latency()
     print(json.dumps(out), flush=True)
 
 

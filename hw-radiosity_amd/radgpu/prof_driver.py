@@ -1,4 +1,5 @@
-"""Profiling driver: one hall-200k solve (or, with --linear, the explicit
+"""Profiling driver: one hall-200k solve at -converge 175 (with --direct at
+-converge 1200; or, with --linear, the explicit
 matrix workload: the 32768-patch room's form factors, FP16 gathers with 3
 and 192 right-hand sides, 2000 plain shots; with --gemm, the same matrix and
 five FP16 sweeps with 192 right-hand sides at the tile RT = 4, split-K 2;
@@ -102,8 +103,11 @@ def main():
     elif "--adjoint" in sys.argv:
         adjoint()
     else:
+        # the benchmark's hall-200k solve: -converge 175 (emitters, then the
+        # lit receivers), or with --direct -converge 1200 (emitters only)
         s = scenes.hall_scene(2)
-        e = radgpu.Engine("-converge", "1200", "-triangle", "1000000", "-quiet", "1")
+        conv = "1200" if "--direct" in sys.argv else "175"
+        e = radgpu.Engine("-converge", conv, "-triangle", "1000000", "-quiet", "1")
         e.set_scene(s).prepare_host().upload()
         n, conv = e.shoot()
         print("shots", n, conv, e.counters(), flush=True)

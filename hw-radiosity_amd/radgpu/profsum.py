@@ -67,8 +67,18 @@ def sq(db_path, kernel):
             tot[counter] = tot.get(counter, 0.0) + v
             disp.add(d)
     res = {"kernel_substring": kernel, "dispatches": len(disp), "counters": tot}
+    nd = max(len(disp), 1)
     if tot.get("SQ_ACTIVE_INST_VALU"):
         res["valu_lane_use"] = tot.get("SQ_THREAD_CYCLES_VALU", 0.0) / (64.0 * tot["SQ_ACTIVE_INST_VALU"])
+    if tot.get("SQ_THREAD_CYCLES_VALU"):
+        # lanes active over the VALU instructions issued: the kernel's VALU work
+        res["thread_valu_per_dispatch"] = tot["SQ_THREAD_CYCLES_VALU"] / nd
+    if tot.get("SQ_ACTIVE_INST_VALU") and tot.get("GRBM_GUI_ACTIVE"):
+        # GRBM_GUI_ACTIVE is summed over the 8 XCDs; a wave64 VALU
+        # instruction holds its SIMD (16 lanes) for 4 clocks; 1024 SIMDs
+        cycles = tot["GRBM_GUI_ACTIVE"] / 8.0
+        res["gui_cycles_per_dispatch"] = cycles / nd
+        res["valu_issue_frac"] = 4.0 * tot["SQ_ACTIVE_INST_VALU"] / (cycles * 1024.0)
     if tot.get("SQ_INSTS_VALU") and tot.get("SQ_WAVES"):
         res["valu_insts_per_wave"] = tot["SQ_INSTS_VALU"] / tot["SQ_WAVES"]
     return res
