@@ -5,7 +5,7 @@
 //       {g0.xyz, first item}, {g1.xyz, count | depth<<24}
 //   leaf lists, leaf-major with the triangle inline (the list order is the
 //       reference's ascending original index): lnc[2*k] = {n.xyz, c.x},
-//       lnc[2*k+1] = {c.y, c.z, original id, -}, lse[3*k+s] = side plane s.
+//       lnc[2*k+1] = {c.y, c.z, original id, cull radius}, lse[3*k+s] = side plane s.
 //       The contents are the originals' slots after pre-partition, which is
 //       what RayHitted tests (raycast.cpp:329).
 //   receivers (every pool slot), SoA by vertex: vx/vy/vz[3*r+v],
@@ -34,10 +34,12 @@
 // Every float operation is IEEE single or double in source order
 // (-ffp-contract=off, correctly rounded division and square root); with
 // F=true an explicit fma sits at each of the reference compiler's
-// contraction sites, so results are bit-identical to the reference.  Three
+// contraction sites, so results are bit-identical to the reference.  Four
 // shortcuts are exact by construction: a triangle test is rejected from
 // a2 * rcp(a1) only when the exact quotient is certainly on the rejected
-// side (a 2^-19 relative guard band around a <= 2^-21 error); TreeNodeNum's
+// side (a 2^-19 relative guard band around a <= 2^-21 error), or when the
+// point that estimate gives is farther from the triangle than any point its
+// side-plane tests can accept (test_entry); TreeNodeNum's
 // double divide is a multiply by 1/MinGridLen unless the product is within
 // 1e-9 of a cell edge; and the double compares against 1e-4 and 1e-6 are
 // float compares against the nearest floats on the right side.
@@ -83,6 +85,7 @@ struct DScene {
 	float thr_t;     // smallest float f with (double)f >= 1e-4
 	float thr_t_lo;  // thr_t * (1 - 2^-19)
 	float thr_v;     // largest float f with (double)f <= 1e-6
+	float cull_cl;   // bound on |c|_1 over the list records with a finite cull radius
 };
 
 struct DRecv {
@@ -190,6 +193,56 @@ __device__ __forceinline__ int cell_coord(float q, float g0, const DScene &S)
 }
 
 // ---- RayHitted ------------------------------------------------------------
+// One list entry of TriListHitted (raycast.cpp:316-333): TriHitted
+// (raycast.cpp:283-313) of the ray against the record {a, b} = {n.xyz, c.x},
+// {c.y, c.z, id, cull radius} with side planes se[0..2], keeping the nearest
+// hit.  Exact shortcuts before the correctly rounded quotient t = a2 / a1:
+//  - t from the reciprocal estimate (relative error <= 2^-21) decides when it
+//    is outside [1e-4, t_min) by more than a 2^-19 guard band;
+//  - the estimate's point q_a = p + t v rejects when it lies farther from the
+//    centroid than the record's cull radius r plus a margin.  r (finite only
+//    for records whose centroid, normal and side planes are exactly those of
+//    their own vertices and whose smallest angle keeps 1 / sin(angle / 2) <=
+//    16) bounds every point of the plane that passes the three side tests
+//    exactly; the margin 2^-12 (2 |t| |v|_1 + |p|_1 + max |c|_1 + 2 r) covers
+//    the side tests' rounding (at most 2^-21 of those magnitudes, moved by up
+//    to 16x at a corner), the distance of the reference's rounded hit point
+//    from the plane and |q_a - q|, with a factor of 4 to spare.
+template <bool F>
+__device__ __forceinline__ void test_entry(const DScene &S, const float4 &a, const float4 &b, const float4 *se,
+                                           float px, float py, float pz, float vx, float vy, float vz, int ban,
+                                           float ck, float cb, float &tmin, float &tmin_hi, int &best, Cnt &c)
+{
+	const int id = __float_as_int(b.z);
+	if (id == ban) return;
+	c.tests++;
+	const float a1 = dot3<F>(vx, vy, vz, a.x, a.y, a.z);
+	const float a2 = dot3<F>(a.w - px, b.x - py, b.y - pz, a.x, a.y, a.z);
+	const float fa1 = fabsf(a1);
+	if (fa1 >= 0x1p-100f && fa1 <= 0x1p100f && fabsf(a2) <= 0x1p100f) {
+		const float ta = a2 * __builtin_amdgcn_rcpf(a1);
+		if (ta < S.thr_t_lo || ta > tmin_hi) return;
+		const float qx = __builtin_fmaf(ta, vx, px) - a.w, qy = __builtin_fmaf(ta, vy, py) - b.x,
+		            qz = __builtin_fmaf(ta, vz, pz) - b.y;
+		const float d2 = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx));
+		const float lim = __builtin_fmaf(b.w, 1.0f + 0x1p-11f, __builtin_fmaf(fabsf(ta), ck, cb));
+		if (d2 > lim * lim) return;
+	}
+	const float tt = a2 / a1;
+	// TriHitted: t < 0.0001 (in double) or t >= t_min rejects
+	if (tt < S.thr_t || tt >= tmin) return;
+	const float hx = axpy<F>(tt, vx, px), hy = axpy<F>(tt, vy, py), hz = axpy<F>(tt, vz, pz);
+	const float4 s0 = se[0];
+	if (dot3<F>(hx, hy, hz, s0.x, s0.y, s0.z) + s0.w < 0.0f) return;
+	const float4 s1 = se[1];
+	if (dot3<F>(hx, hy, hz, s1.x, s1.y, s1.z) + s1.w < 0.0f) return;
+	const float4 s2 = se[2];
+	if (dot3<F>(hx, hy, hz, s2.x, s2.y, s2.z) + s2.w < 0.0f) return;
+	tmin = tt;
+	tmin_hi = tt * kGuardUp;
+	best = id;
+}
+
 // The walk of raycast.cpp:335-360 over the tabulated octree: find the leaf
 // of q (TreeNodeNum), test its list (TriListHitted/TriHitted,
 // raycast.cpp:283-333) and return the nearest hit of the first leaf with any
@@ -206,6 +259,9 @@ __device__ int walk(const DScene &S, float px, float py, float pz, float vx, flo
 	// a cycle the reference never leaves.
 	int tort = -1, power = 1, lam = 0;
 	const bool use_x = fabsf(vx) > S.thr_v, use_y = fabsf(vy) > S.thr_v, use_z = fabsf(vz) > S.thr_v;
+	// cull margin terms of test_entry: |ta| |v|_1 2^-11 + (|p|_1 + max |c|_1) 2^-12
+	const float ck = (fabsf(vx) + fabsf(vy) + fabsf(vz)) * 0x1p-11f;
+	const float cb = (fabsf(px) + fabsf(py) + fabsf(pz) + S.cull_cl) * 0x1p-12f;
 	for (int steps = 0;;) {
 		if (qx < S.g0x || qx > S.g1x || qy < S.g0y || qy > S.g1y || qz < S.g0z || qz > S.g1z) return -1;
 		const int gx = cell_coord(qx, S.g0x, S), gy = cell_coord(qy, S.g0y, S), gz = cell_coord(qz, S.g0z, S);
@@ -232,51 +288,26 @@ __device__ int walk(const DScene &S, float px, float py, float pz, float vx, flo
 		float tmin = 1000000.0f, tmin_hi = 1000000.0f * kGuardUp;
 		int best = -1;
 		const float4 *nc = S.lnc + 2 * (size_t)begin;
-		float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+		const float4 *se = S.lse + 3 * (size_t)begin;
+		// two entries per trip, the next one loaded while the current one is
+		// tested (no register copies between trips)
+		float4 ea0 = make_float4(0.f, 0.f, 0.f, 0.f), eb0 = ea0, ea1 = ea0, eb1 = ea0;
 		if (cnt > 0) {
-			a = nc[0];
-			b = nc[1];
+			ea0 = nc[0];
+			eb0 = nc[1];
 		}
-		for (int e = 0; e < cnt; e++) {
-			float4 an = a, bn = b;
-			if (e + 1 < cnt) {  // prefetch the next list entry
-				an = nc[2 * e + 2];
-				bn = nc[2 * e + 3];
+		for (int e = 0; e < cnt; e += 2) {
+			if (e + 1 < cnt) {
+				ea1 = nc[2 * e + 2];
+				eb1 = nc[2 * e + 3];
 			}
-			const int id = __float_as_int(b.z);
-			if (id != ban) {
-				c.tests++;
-				const float a1 = dot3<F>(vx, vy, vz, a.x, a.y, a.z);
-				const float a2 = dot3<F>(a.w - px, b.x - py, b.y - pz, a.x, a.y, a.z);
-				bool undecided = true;
-				const float fa1 = fabsf(a1);
-				if (fa1 >= 0x1p-100f && fa1 <= 0x1p100f && fabsf(a2) <= 0x1p100f) {
-					const float ta = a2 * __builtin_amdgcn_rcpf(a1);
-					if (ta < S.thr_t_lo || ta > tmin_hi) undecided = false;
-				}
-				if (undecided) {
-					const float tt = a2 / a1;
-					// TriHitted: t < 0.0001 (in double) or t >= t_min rejects
-					if (!(tt < S.thr_t) && !(tt >= tmin)) {
-						const float hx = axpy<F>(tt, vx, px), hy = axpy<F>(tt, vy, py), hz = axpy<F>(tt, vz, pz);
-						const float4 *se = S.lse + 3 * ((size_t)begin + e);
-						const float4 s0 = se[0];
-						if (!(dot3<F>(hx, hy, hz, s0.x, s0.y, s0.z) + s0.w < 0.0f)) {
-							const float4 s1 = se[1];
-							if (!(dot3<F>(hx, hy, hz, s1.x, s1.y, s1.z) + s1.w < 0.0f)) {
-								const float4 s2 = se[2];
-								if (!(dot3<F>(hx, hy, hz, s2.x, s2.y, s2.z) + s2.w < 0.0f)) {
-									tmin = tt;
-									tmin_hi = tt * kGuardUp;
-									best = id;
-								}
-							}
-						}
-					}
-				}
+			test_entry<F>(S, ea0, eb0, se + 3 * e, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, c);
+			if (e + 1 >= cnt) break;
+			if (e + 2 < cnt) {
+				ea0 = nc[2 * e + 4];
+				eb0 = nc[2 * e + 5];
 			}
-			a = an;
-			b = bn;
+			test_entry<F>(S, ea1, eb1, se + 3 * e + 3, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, c);
 		}
 		if (best >= 0) return best;
 		if (leaf == tort) { c.stalled++; return -2; }
@@ -1057,14 +1088,15 @@ __global__ __launch_bounds__(kBlock) void k_ff_seg(DRecv R, int n, int ld, int n
 // fills the octree's list entries with the originals inline, k_pack_recv the
 // receivers' arrays.  Pure data movement.
 __global__ __launch_bounds__(kBlock) void k_pack_items(const Patch *__restrict__ pool, const int *__restrict__ items,
-                                                       int ni, float4 *lnc, float4 *lse)
+                                                       const float *__restrict__ radius, int ni, float4 *lnc,
+                                                       float4 *lse)
 {
 	const int k = blockIdx.x * kBlock + threadIdx.x;
 	if (k >= ni) return;
 	const int id = items[k];
 	const Patch &p = pool[id];
 	lnc[2 * (size_t)k] = make_float4(p.n[0], p.n[1], p.n[2], p.c[0]);
-	lnc[2 * (size_t)k + 1] = make_float4(p.c[1], p.c[2], __int_as_float(id), 0.f);
+	lnc[2 * (size_t)k + 1] = make_float4(p.c[1], p.c[2], __int_as_float(id), radius[id]);
 	for (int q = 0; q < 3; q++) lse[3 * (size_t)k + q] = make_float4(p.se[q][0], p.se[q][1], p.se[q][2], p.se[q][3]);
 }
 
@@ -1196,6 +1228,7 @@ struct DeviceEngine::Impl {
 	DevBuf<int> order;         // receiver processing order (DRecv::order)
 	DevBuf<Patch> pool;        // the uploaded slot records (packed by k_pack_items / k_pack_recv)
 	DevBuf<int> items;         // the octree's list entries (original slots)
+	DevBuf<float> radius;      // per original: cull radius of its list records (cull_radius)
 	DScene S;
 	DRecv R;
 	Loop *host_loop = nullptr;  // pinned mirror of the loop header
@@ -1323,10 +1356,15 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 		}
 	std::vector<int> order(n);
 	spatial_order(s.pool.data(), n, s.G0, s.G1, order.data());
+	// cull radii of the originals as the octree sees them, and the bound on
+	// |c|_1 over those with a finite radius
+	std::vector<float> rad((size_t)n0 + 1);
+	const float cull_cl = cull_radii(s, rad.data());
 	hipStream_t st = d.stream;
 	if (d.order.put(order.data(), order.size(), st, err)) return -1;
 	// the slot records as they are, then gathered on the device
 	if (d.pool.put(s.pool.data(), (size_t)n, st, err) || d.items.put(t.items.data(), ni, st, err) ||
+	    d.radius.put(rad.data(), rad.size(), st, err) ||
 	    d.lnc.alloc(ni * 2, err) || d.lse.alloc(ni * 3, err) ||
 	    d.cell_leaf.put(t.cell_leaf.data(), t.cell_leaf.size(), st, err) ||
 	    d.leaf_box.put((const float4 *)t.leaf_box.data(), t.leaf_box.size() / 4, st, err) ||
@@ -1336,7 +1374,7 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 		return -1;
 	if (ni > 0)
 		hipLaunchKernelGGL(k_pack_items, dim3((unsigned)((ni + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d.pool.p,
-		                   d.items.p, (int)ni, d.lnc.p, d.lse.p);
+		                   d.items.p, d.radius.p, (int)ni, d.lnc.p, d.lse.p);
 	if (n > 0)
 		hipLaunchKernelGGL(k_pack_recv, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, d.pool.p, n, d.vx.p,
 		                   d.vy.p, d.vz.p, d.nrm.p, d.cA.p, d.rho.p, d.acc.p, d.dacc.p, d.dB.p);
@@ -1361,7 +1399,7 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	const float thr_t = smallest_float_at_least(0.0001);
 	d.S = DScene{d.lnc.p, d.lse.p, d.cell_leaf.p, d.leaf_box.p, s.G0[0], s.G0[1], s.G0[2], s.G1[0], s.G1[1], s.G1[2],
 	             (double)t.min_grid, 1.0 / (double)t.min_grid, t.gt, thr_t, thr_t * kGuardDown,
-	             largest_float_at_most(1e-6)};
+	             largest_float_at_most(1e-6), cull_cl};
 	d.R = DRecv{d.vx.p, d.vy.p, d.vz.p, d.nrm.p, d.cA.p, d.rho.p, d.acc.p, d.dacc.p, d.dB.p, n, d.order.p};
 	if (d.rkey.alloc((size_t)n * 3 + 1, err)) return -1;
 	if (n > 0)

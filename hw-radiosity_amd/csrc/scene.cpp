@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <set>
 #include <utility>
 
@@ -55,6 +56,54 @@ template <bool F> void derive_fields(Patch &t)
 }
 
 }  // namespace
+
+float cull_radius(const Patch &t, bool fma, float max_amp)
+{
+	constexpr float kInf = std::numeric_limits<float>::infinity();
+	Patch d;
+	std::memcpy(d.p, t.p, sizeof d.p);
+	if (fma) derive_fields<true>(d);
+	else derive_fields<false>(d);
+	// the record's planes must be exactly those of its own vertices
+	if (std::memcmp(d.c, t.c, sizeof d.c) || std::memcmp(d.n, t.n, sizeof d.n) || std::memcmp(d.se, t.se, sizeof d.se))
+		return kInf;
+	double e[3][3], len[3];
+	for (int i = 0; i < 3; i++) {
+		for (int k = 0; k < 3; k++) e[i][k] = (double)t.p[(i + 1) % 3][k] - (double)t.p[i][k];
+		len[i] = std::sqrt(e[i][0] * e[i][0] + e[i][1] * e[i][1] + e[i][2] * e[i][2]);
+		if (!(len[i] > 0.0) || !std::isfinite(len[i])) return kInf;
+	}
+	// smallest angle: at vertex i between -e[i-1] and e[i]
+	double min_angle = 4.0;
+	for (int i = 0; i < 3; i++) {
+		const double *a = e[(i + 2) % 3], *b = e[i];
+		double c = -(a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / (len[(i + 2) % 3] * len[i]);
+		c = std::max(-1.0, std::min(1.0, c));
+		min_angle = std::min(min_angle, std::acos(c));
+	}
+	if (!(min_angle > 0.0) || 1.0 / std::sin(min_angle / 2.0) > (double)max_amp) return kInf;
+	double r = 0.0;
+	for (int v = 0; v < 3; v++) {
+		double q = 0.0;
+		for (int k = 0; k < 3; k++) q += ((double)t.p[v][k] - t.c[k]) * ((double)t.p[v][k] - t.c[k]);
+		r = std::max(r, std::sqrt(q));
+	}
+	const float rf = (float)r;
+	return std::isfinite(rf) ? std::nextafter(rf, kInf) : kInf;
+}
+
+float cull_radii(const HostScene &s, float *radius)
+{
+	double cl = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : cl)
+	for (int i = 0; i < s.n_orig; i++) {
+		const Patch &p = s.pool[i];
+		radius[i] = cull_radius(p, s.o.fma != 0, 16.0f);
+		if (std::isfinite(radius[i]))
+			cl = std::max(cl, std::fabs((double)p.c[0]) + std::fabs((double)p.c[1]) + std::fabs((double)p.c[2]));
+	}
+	return std::nextafter((float)cl, std::numeric_limits<float>::infinity());
+}
 
 int HostScene::set_triangles(int n, const float *p9, const int *f3, const int *b3, std::string &err)
 {

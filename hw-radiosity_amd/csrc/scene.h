@@ -57,4 +57,16 @@ struct HostScene {
 	void log(const char *fmt, ...) const;
 };
 
+// Radius of the sphere about t's centroid that holds every point RayHitted
+// can accept on t, for the ray-tracing kernel's exact early rejection: the
+// largest vertex distance from the centroid when t's centroid, normal and
+// side planes are exactly those its own vertices give (ReadTriangle's
+// arithmetic) and no angle is so small that rounding in the side-plane tests
+// moves its corner by more than max_amp times their fuzz
+// (1 / sin(angle / 2) <= max_amp); +infinity otherwise (never rejected).
+float cull_radius(const Patch &t, bool fma, float max_amp);
+// cull_radius (max_amp 16) of every original of s into radius[0, n_orig);
+// returns a bound on |c|_1 over the originals with a finite radius.
+float cull_radii(const HostScene &s, float *radius);
+
 }  // namespace rgpu
