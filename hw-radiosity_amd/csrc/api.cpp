@@ -306,6 +306,19 @@ rad_status rad_cast(rad_ctx *c, int n, const void *rays, int *hit, int *cells, i
 	return RAD_OK;
 }
 
+rad_status rad_cast_static(rad_ctx *c, int n, const void *rays, int *hit, int *cells, int *tests)
+{
+	if (!c || n < 0 || (n > 0 && (!rays || !hit))) return RAD_EINVAL;
+	if (!c->uploaded) return fail(c, RAD_ESTATE, "rad_upload first");
+	if (n > RAD_CAST_STATIC_MAX) {
+		c->err = "rad_cast_static: " + std::to_string(n) + " rays exceed its limit of " +
+		         std::to_string(RAD_CAST_STATIC_MAX) + " per call";
+		return RAD_EINVAL;
+	}
+	if (c->dev->cast_static(n, (const RayIn *)rays, hit, cells, tests, c->err)) return RAD_EDEVICE;
+	return RAD_OK;
+}
+
 rad_status rad_ff_pairs(rad_ctx *c, int n, const int *pairs, float *ff3)
 {
 	if (!c || n < 0 || (n > 0 && (!pairs || !ff3))) return RAD_EINVAL;

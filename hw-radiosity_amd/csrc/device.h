@@ -52,6 +52,9 @@ public:
 	// Copies accB, deltaaccB and deltaB back into the host pool.
 	int download(HostScene &s, std::string &err);
 	int cast(int n, const RayIn *rays, int *hit, int *cells, int *tests, std::string &err);
+	// cast() through the device's static scratch instead of allocations
+	// (at most 1 << 18 rays; calls that overlap on one GPU race).
+	int cast_static(int n, const RayIn *rays, int *hit, int *cells, int *tests, std::string &err);
 	int ff_pairs(int n, const int *pairs, float *ff3, std::string &err);
 	void counters(long long *out);
 	// Explicit form factors of slots [0, n) into the device matrix dF

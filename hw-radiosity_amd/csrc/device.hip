@@ -48,9 +48,11 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "device.h"
+#include "../../include/radiosity_gpu.h"
 #include "pack.h"
 
 namespace rgpu {
@@ -1133,6 +1135,14 @@ __global__ __launch_bounds__(kBlock) void k_cast(DScene S, const RayIn *rays, in
 	tests[i] = c.tests;
 }
 
+// Scratch of rad_cast_static, the variant of rad_cast that allocates nothing:
+// the rays and their three results for up to kStaticRays rays per call.  One
+// array per GPU, held until the process ends; every call on a GPU shares it.
+constexpr int kStaticRays = RAD_CAST_STATIC_MAX;
+constexpr size_t kStaticRayBytes = (size_t)kStaticRays * sizeof(RayIn);
+constexpr size_t kStaticBytes = kStaticRayBytes + (size_t)3 * kStaticRays * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
 template <bool F>
 __global__ __launch_bounds__(kBlock) void k_ff_pairs(DScene S, DRecv R, const int *pairs, int n, float *ff3)
 {
@@ -1207,6 +1217,29 @@ float largest_float_at_most(double x)
 	return f;
 }
 
+// The device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy at its own address); host code uses
+// this address, never the array's name.
+int scratch_address(void **out, std::string &err)
+{
+	static std::mutex mu;
+	static void *addr[64] = {};
+	int dev = -1;
+	HIP_OK(hipGetDevice(&dev), "hipGetDevice");
+	if (dev < 0 || dev >= 64) {
+		err = "device ordinal out of range for the static scratch table";
+		return -1;
+	}
+	std::lock_guard<std::mutex> lock(mu);
+	if (!addr[dev]) {
+		void *p = nullptr;
+		HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+		addr[dev] = p;
+	}
+	*out = addr[dev];
+	return 0;
+}
+
 }  // namespace
 
 struct DeviceEngine::Impl {
@@ -1237,6 +1270,7 @@ struct DeviceEngine::Impl {
 	int parallel_src = 1, batch = 32;
 	float converge = 1200.f;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	void *scratch = nullptr;  // this device's address of the static scratch (rad_cast_static)
 	static constexpr int kEv = 16;
 	hipEvent_t kev[2 * kEv] = {};
 };
@@ -1279,6 +1313,10 @@ DeviceEngine *DeviceEngine::create(std::string &err)
 			delete e;
 			return nullptr;
 		}
+	if (scratch_address(&e->d_->scratch, err)) {
+		delete e;
+		return nullptr;
+	}
 	return e;
 }
 
@@ -1559,6 +1597,42 @@ int DeviceEngine::cast(int n, const RayIn *rays, int *hit, int *cells, int *test
 	HIP_OK(hipMemcpyAsync(hit, dh.p, n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
 	if (cells) HIP_OK(hipMemcpyAsync(cells, dc.p, n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
 	if (tests) HIP_OK(hipMemcpyAsync(tests, dt.p, n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
+	HIP_OK(hipStreamSynchronize(st), "cast");
+	float ms = 0;
+	if (hipEventElapsedTime(&ms, d.ev0, d.ev1) == hipSuccess) cast_ms_ = ms;
+	return 0;
+}
+
+int DeviceEngine::cast_static(int n, const RayIn *rays, int *hit, int *cells, int *tests, std::string &err)
+{
+	Impl &d = *d_;
+	if (n > kStaticRays) {
+		err = "rad_cast_static: " + std::to_string(n) + " rays exceed its limit of " + std::to_string(kStaticRays) +
+		      " per call";
+		return -1;
+	}
+	if (n <= 0) return 0;
+	if (!d.scratch) {
+		err = "rad_cast_static: no static scratch on this device";
+		return -1;
+	}
+	// staging in this device's scratch: rays, then hit, cells and tests;
+	// k_cast writes every result, so nothing needs clearing
+	unsigned char *base = static_cast<unsigned char *>(d.scratch);
+	RayIn *dr = reinterpret_cast<RayIn *>(base);
+	int *dh = reinterpret_cast<int *>(base + kStaticRayBytes);
+	int *dc = dh + kStaticRays, *dt = dc + kStaticRays;
+	hipStream_t st = d.stream;
+	HIP_OK(hipMemcpyAsync(dr, rays, (size_t)n * sizeof(RayIn), hipMemcpyHostToDevice, st), "upload");
+	const int blocks = (n + kBlock - 1) / kBlock;
+	HIP_OK(hipEventRecord(d.ev0, st), "event");
+	if (d.fma) hipLaunchKernelGGL(k_cast<true>, dim3(blocks), dim3(kBlock), 0, st, d.S, dr, n, dh, dc, dt);
+	else hipLaunchKernelGGL(k_cast<false>, dim3(blocks), dim3(kBlock), 0, st, d.S, dr, n, dh, dc, dt);
+	HIP_OK(hipGetLastError(), "cast launch");
+	HIP_OK(hipEventRecord(d.ev1, st), "event");
+	HIP_OK(hipMemcpyAsync(hit, dh, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
+	if (cells) HIP_OK(hipMemcpyAsync(cells, dc, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
+	if (tests) HIP_OK(hipMemcpyAsync(tests, dt, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
 	HIP_OK(hipStreamSynchronize(st), "cast");
 	float ms = 0;
 	if (hipEventElapsedTime(&ms, d.ev0, d.ev1) == hipSuccess) cast_ms_ = ms;

@@ -36,7 +36,7 @@ RAY_DTYPE = np.dtype([("p", "<f4", 3), ("v", "<f4", 3), ("ban", "<i4"), ("target
 
 EXPORTS = ("rad_create", "rad_destroy", "rad_last_error", "rad_load_tri", "rad_set_triangles",
            "rad_prepare_host", "rad_upload", "rad_shoot", "rad_run", "rad_write_fin", "rad_fin_to_json", "rad_records",
-           "rad_tree_info", "rad_cast", "rad_ff_pairs", "rad_counters", "rad_timings",
+           "rad_tree_info", "rad_cast", "rad_cast_static", "rad_ff_pairs", "rad_counters", "rad_timings",
            "rad_lin_create", "rad_lin_destroy", "rad_lin_last_error", "rad_lin_from_scene",
            "rad_lin_set_matrix", "rad_lin_get_matrix", "rad_lin_set_areas", "rad_lin_autotune", "rad_lin_set_tile",
            "rad_lin_gemm_ms", "rad_lin_multiply", "rad_lin_gather", "rad_lin_shoot", "rad_lin_adjoint",
@@ -44,6 +44,7 @@ EXPORTS = ("rad_create", "rad_destroy", "rad_last_error", "rad_load_tri", "rad_s
            "rad_batch_set_matrix", "rad_batch_autotune", "rad_batch_set_tile", "rad_batch_solve")
 
 PREC_F32, PREC_F16, PREC_BF16 = 0, 1, 2
+CAST_STATIC_MAX = 1 << 18
 VIS_SEGMENT, VIS_REFERENCE = 0, 1
 PRECISIONS = {"f32": PREC_F32, "f16": PREC_F16, "bf16": PREC_BF16}
 INFO_NAMES = ("outer", "sweeps", "rel_res", "ms", "ms_target", "err", "power_E", "power_B")
@@ -80,6 +81,7 @@ def library() -> ctypes.CDLL:
             "rad_records": [vp, ctypes.POINTER(ip), vp, vp],
             "rad_tree_info": [vp, vp, ctypes.POINTER(ctypes.c_float)],
             "rad_cast": [vp, ip, vp, vp, vp, vp],
+            "rad_cast_static": [vp, ip, vp, vp, vp, vp],
             "rad_ff_pairs": [vp, ip, vp, vp],
             "rad_counters": [vp, vp],
             "rad_timings": [vp, vp],
@@ -210,6 +212,15 @@ class Engine:
         n = len(rays)
         hit, cells, tests = (np.zeros(n, np.int32) for _ in range(3))
         self._ok(library().rad_cast(self.h, n, _p(rays), _p(hit), _p(cells), _p(tests)))
+        return hit, cells, tests
+
+    def cast_static(self, rays):
+        """cast() through rad_cast_static (no device allocation; at most
+        CAST_STATIC_MAX rays)."""
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        n = len(rays)
+        hit, cells, tests = (np.zeros(n, np.int32) for _ in range(3))
+        self._ok(library().rad_cast_static(self.h, n, _p(rays), _p(hit), _p(cells), _p(tests)))
         return hit, cells, tests
 
     def ff_pairs(self, pairs):

@@ -110,6 +110,16 @@ rad_status rad_tree_info(rad_ctx *ctx, int *out3, float *min_grid_len);
  * cells/tests (may be NULL) count leaves visited and triangle tests. */
 rad_status rad_cast(rad_ctx *ctx, int n, const void *rays, int *hit, int *cells, int *tests);
 
+/* rad_cast without allocating: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The rays and results are
+ * staged through a static device array that each GPU holds until the
+ * process ends, so at most RAD_CAST_STATIC_MAX rays per call (RAD_EINVAL
+ * beyond), and calls that overlap on one GPU (contexts on other threads)
+ * race: serialise them.  Results equal rad_cast's. */
+#define RAD_CAST_STATIC_MAX (1 << 18)
+rad_status rad_cast_static(rad_ctx *ctx, int n, const void *rays, int *hit, int *cells, int *tests);
+
 /* The per-vertex form factors Shade computes for receiver slot pairs[2i+1]
  * from source slot pairs[2i] (AdaptCalFF(CalFF(...)), shade.cpp:442-450). */
 rad_status rad_ff_pairs(rad_ctx *ctx, int n, const int *pairs, float *ff3);
