@@ -69,6 +69,36 @@ rad_status lfail(rad_lin *l, rad_status s, const std::string &msg)
 	return s;
 }
 
+// The device's start-up (HIP runtime, the context's stream, loading the
+// device code) on a thread of its own, on the calling thread's current
+// device, while the host works; a no-op when the context has a device or a
+// start-up under way, or when there is no device.
+void start_device(rad_ctx *c)
+{
+	int device = -1;
+	if (c->dev || c->starter.joinable() || (device = DeviceEngine::current_device()) < 0) return;
+	c->start_err.clear();
+	c->starter = std::thread([c, device] {
+		c->started = DeviceEngine::create_on(device, c->start_err);
+		if (c->started && c->started->warm(c->start_err)) {
+			delete c->started;
+			c->started = nullptr;
+		}
+	});
+}
+
+// Waits for a start-up under way; its failure is reported by the first call
+// that needs the device.
+void join_device(rad_ctx *c)
+{
+	if (!c->starter.joinable()) return;
+	c->starter.join();
+	if (c->started) {
+		c->dev.reset(c->started);
+		c->started = nullptr;
+	}
+}
+
 rad_status after_load(rad_ctx *c)
 {
 	const auto t0 = Clock::now();
@@ -147,21 +177,36 @@ const char *rad_last_error(const rad_ctx *c) { return c ? c->err.c_str() : "null
 rad_status rad_load_tri(rad_ctx *c, const char *path)
 {
 	if (!c || !path) return RAD_EINVAL;
+	// the device starts while the file is read and the octree built: its
+	// start-up belongs with the reference's "Init" (InitModelFromFile,
+	// rad.cpp:244-252), not with the solve
+	start_device(c);
 	const auto t0 = Clock::now();
 	c->ms_setup = 0;
-	if (c->scene.load_tri(path, c->err)) return RAD_EIO;
-	c->ms_setup += ms_since(t0);
-	return after_load(c);
+	rad_status st = RAD_OK;
+	if (c->scene.load_tri(path, c->err)) st = RAD_EIO;
+	else {
+		c->ms_setup += ms_since(t0);
+		st = after_load(c);
+	}
+	join_device(c);
+	return st;
 }
 
 rad_status rad_set_triangles(rad_ctx *c, int n, const float *p9, const int *frgb3, const int *brgb3)
 {
 	if (!c || n < 0 || (n > 0 && (!p9 || !frgb3 || !brgb3))) return RAD_EINVAL;
+	start_device(c);  // as rad_load_tri
 	const auto t0 = Clock::now();
 	c->ms_setup = 0;
-	if (c->scene.set_triangles(n, p9, frgb3, brgb3, c->err)) return RAD_EINVAL;
-	c->ms_setup += ms_since(t0);
-	return after_load(c);
+	rad_status st = RAD_OK;
+	if (c->scene.set_triangles(n, p9, frgb3, brgb3, c->err)) st = RAD_EINVAL;
+	else {
+		c->ms_setup += ms_since(t0);
+		st = after_load(c);
+	}
+	join_device(c);
+	return st;
 }
 
 rad_status rad_prepare_host(rad_ctx *c)
@@ -170,8 +215,18 @@ rad_status rad_prepare_host(rad_ctx *c)
 	if (!c->loaded) return fail(c, RAD_ESTATE, "no scene loaded");
 	if (c->prepared) return RAD_OK;
 	if (c->scene.o.write_cycle <= 0) return fail(c, RAD_EINVAL, "-write_cycle must be positive");
+	const bool on_device = c->scene.o.neighbors_host == 0;
+	if (on_device && !c->dev) {  // the neighbour search needs the device
+		join_device(c);
+		if (!c->dev) {
+			DeviceEngine *d = DeviceEngine::create(c->err);
+			if (!d) return RAD_EDEVICE;
+			c->dev.reset(d);
+		}
+	}
 	const auto t0 = Clock::now();
-	c->scene.init_radiosity();
+	c->scene.init_radiosity(!on_device);
+	if (on_device && c->dev->link_neighbors(c->scene, c->err)) return RAD_EDEVICE;
 	c->scene.prepartition();
 	c->ms_setup += ms_since(t0);
 	c->prepared = true;
@@ -183,10 +238,8 @@ rad_status rad_upload(rad_ctx *c)
 	if (!c) return RAD_EINVAL;
 	if (!c->prepared) return fail(c, RAD_ESTATE, "rad_prepare_host first");
 	if (!c->dev && c->starter.joinable()) {  // started during the host set-up
-		c->starter.join();
-		if (!c->started) return fail(c, RAD_EDEVICE, c->start_err);
-		c->dev.reset(c->started);
-		c->started = nullptr;
+		join_device(c);
+		if (!c->dev) return fail(c, RAD_EDEVICE, c->start_err);
 	}
 	if (!c->dev) {  // HIP start-up and the context's stream: not part of the upload time
 		DeviceEngine *d = DeviceEngine::create(c->err);
@@ -215,16 +268,7 @@ rad_status rad_run(rad_ctx *c, const char *prefix, int *iterations)
 {
 	if (!c || !prefix) return RAD_EINVAL;
 	rad_status st;
-	int device = -1;
-	if (!c->dev && !c->uploaded && !c->prepared && !c->starter.joinable() &&
-	    (device = DeviceEngine::current_device()) >= 0)
-		c->starter = std::thread([c, device] {
-			c->started = DeviceEngine::create_on(device, c->start_err);
-			if (c->started && c->started->warm(c->start_err)) {
-				delete c->started;
-				c->started = nullptr;
-			}
-		});
+	if (!c->uploaded && !c->prepared) start_device(c);  // overlaps InitRadiosity and pre-partition
 	if ((st = rad_prepare_host(c)) != RAD_OK) return st;
 	if (!c->uploaded && (st = rad_upload(c)) != RAD_OK) return st;
 	const HostScene &s = c->scene;

@@ -51,6 +51,9 @@ public:
 	int shoot(int max_shots, int *shots, int *converged, std::string &err);
 	// Copies accB, deltaaccB and deltaB back into the host pool.
 	int download(HostScene &s, std::string &err);
+	// InitRadiosity's neighbour search (rad.cpp:295-334) for slots [0,
+	// s.trinum) on the device: writes s.pool[i].neighbor.
+	int link_neighbors(HostScene &s, std::string &err);
 	int cast(int n, const RayIn *rays, int *hit, int *cells, int *tests, std::string &err);
 	// cast() through the device's static scratch instead of allocations
 	// (at most 1 << 18 rays; calls that overlap on one GPU race).

@@ -67,6 +67,7 @@ constexpr int kTraceBlock = 64;
 constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
 constexpr int kGroups = 2;        // receiver groups per block
 constexpr int kMinQueue = 8;      // iterations queued per host check at least (up to -batch)
+constexpr int kGraphIters = 8;    // shooting iterations per captured graph
 constexpr int kRecvPerBlock = kGroups * kWavesPerBlock * kRecvPerWave;
 constexpr int kMaxSrc = 256;
 constexpr int kRayBuckets = 16;   // ray keys k_compact groups by
@@ -1167,6 +1168,94 @@ __global__ __launch_bounds__(kBlock) void k_ff_pairs(DScene S, DRecv R, const in
 	}
 }
 
+// ---- InitRadiosity's neighbour search (rad.cpp:295-334) ----------------------
+// The reference compares every corner of triangle i with every corner of
+// every j (O(n^2)); here corners are hashed by their exact coordinates into
+// chained buckets and only corners in one bucket are compared.  For each
+// ordered pair (i, v), (j, c) with i != j and p_i[v] == p_j[c]: when
+// p_i[v+1] == p_j[c+1] ("same direction") or p_i[v+1] == p_j[c+2]
+// ("opposite") and CosTheta(n_i, n_j) > 0.85, the reference writes
+// i.neighbor[v] = j and j.neighbor[c or c+2] = i.  Each write carries its
+// position in the reference's loop order, (i, j, 3v + c) packed into 64
+// bits, and a slot edge keeps the greatest: the last write wins, as in the
+// reference's sequential order.
+struct NbPoint {
+	float p[3][3];
+	float n[3];
+};
+
+__device__ __forceinline__ unsigned nb_hash(float x, float y, float z, unsigned mask)
+{
+	// -0 folded onto +0: equal coordinates have equal bits
+	unsigned long long h = __float_as_uint(x + 0.0f) * 0x9E3779B97F4A7C15ull;
+	h ^= (h >> 29) + __float_as_uint(y + 0.0f) * 0xC2B2AE3D27D4EB4Full;
+	h ^= (h >> 31) + __float_as_uint(z + 0.0f) * 0x165667B19E3779F9ull;
+	h ^= h >> 32;
+	return (unsigned)h & mask;
+}
+
+__device__ __forceinline__ bool nb_same(const float *a, const float *b)
+{
+	return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
+}
+
+__global__ __launch_bounds__(kBlock) void k_nb_insert(const NbPoint *__restrict__ T, int n, unsigned mask, int *heads,
+                                                      int *next)
+{
+	const int r = blockIdx.x * kBlock + threadIdx.x;
+	if (r >= 3 * n) return;
+	const float *p = T[r / 3].p[r % 3];
+	if (isnan(p[0]) || isnan(p[1]) || isnan(p[2])) {  // never equal to anything
+		next[r] = -2;
+		return;
+	}
+	next[r] = atomicExch(&heads[nb_hash(p[0], p[1], p[2], mask)], r);
+}
+
+template <bool F>
+__global__ __launch_bounds__(kBlock) void k_nb_pairs(const NbPoint *__restrict__ T, int n, unsigned mask,
+                                                     const int *__restrict__ heads, const int *__restrict__ next,
+                                                     unsigned long long *best)
+{
+	const int u = blockIdx.x * kBlock + threadIdx.x;
+	if (u >= 3 * n || next[u] == -2) return;
+	const int i = u / 3, v = u % 3;
+	const NbPoint &tp = T[i];
+	const float *pv = tp.p[v], *pv1 = tp.p[(v + 1) % 3];
+	for (int w = heads[nb_hash(pv[0], pv[1], pv[2], mask)]; w >= 0; w = next[w]) {
+		const int j = w / 3, c = w % 3;
+		if (j == i) continue;
+		const NbPoint &tn = T[j];
+		if (!nb_same(pv, tn.p[c])) continue;  // another point in the same bucket
+		int other;
+		if (nb_same(pv1, tn.p[(c + 1) % 3])) other = c;
+		else if (nb_same(pv1, tn.p[(c + 2) % 3])) other = (c + 2) % 3;
+		else continue;
+		const float li = __builtin_sqrtf(dot3<F>(tp.n[0], tp.n[1], tp.n[2], tp.n[0], tp.n[1], tp.n[2]));
+		const float lj = __builtin_sqrtf(dot3<F>(tn.n[0], tn.n[1], tn.n[2], tn.n[0], tn.n[1], tn.n[2]));
+		const float cosine = dot3<F>(tp.n[0], tp.n[1], tp.n[2], tn.n[0], tn.n[1], tn.n[2]) / li / lj;
+		if (!((double)cosine > 0.85)) continue;
+		const unsigned long long order =
+		    ((unsigned long long)i << 34) | ((unsigned long long)j << 4) | (unsigned long long)(3 * v + c);
+		atomicMax(&best[3 * (size_t)i + v], order);      // i.neighbor[v] = j
+		atomicMax(&best[3 * (size_t)j + other], order);  // j.neighbor[c or c+2] = i
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_nb_final(const unsigned long long *__restrict__ best, int n, int *nb)
+{
+	const int k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= 3 * n) return;
+	const unsigned long long order = best[k];
+	const int s = k / 3, i = (int)(order >> 34), j = (int)((order >> 4) & 0x3fffffffu);
+	nb[k] = order == 0 ? -1 : (s == i ? j : i);
+}
+
+template <typename T> __global__ void k_fill_n(T *p, size_t n, T v)
+{
+	for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = v;
+}
+
 // ---- host side --------------------------------------------------------------
 #define HIP_OK(expr, what)                                                        \
 	do {                                                                          \
@@ -1271,6 +1360,16 @@ struct DeviceEngine::Impl {
 	float converge = 1200.f;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
 	void *scratch = nullptr;  // this device's address of the static scratch (rad_cast_static)
+	// kGraphIters shooting iterations captured once per upload and replayed
+	hipGraphExec_t graph_exec = nullptr;
+	hipGraph_t graph = nullptr;
+	void drop_graph()
+	{
+		if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+		if (graph) (void)hipGraphDestroy(graph);
+		graph_exec = nullptr;
+		graph = nullptr;
+	}
 	static constexpr int kEv = 16;
 	hipEvent_t kev[2 * kEv] = {};
 };
@@ -1279,6 +1378,7 @@ DeviceEngine::~DeviceEngine()
 {
 	if (!d_) return;
 	if (d_->stream) (void)hipStreamSynchronize(d_->stream);
+	d_->drop_graph();
 	if (d_->host_loop) (void)hipHostFree(d_->host_loop);
 	if (d_->ev0) (void)hipEventDestroy(d_->ev0);
 	if (d_->ev1) (void)hipEventDestroy(d_->ev1);
@@ -1342,10 +1442,16 @@ __global__ void k_warm(int *p)
 
 int DeviceEngine::warm(std::string &err)
 {
+	// the first launch loads the device code; the first copies each way set
+	// up the runtime's staging for pageable host memory (milliseconds each,
+	// otherwise paid by the first upload and the first download)
 	int *p = nullptr;
 	HIP_OK(hipMalloc(&p, 4096), "warm");
+	std::vector<int> h(1024, 0);
 	hipLaunchKernelGGL(k_warm, dim3(1), dim3(64), 0, d_->stream, p);
-	const hipError_t e = hipStreamSynchronize(d_->stream);
+	hipError_t e = hipMemcpyAsync(p, h.data(), 4096, hipMemcpyHostToDevice, d_->stream);
+	if (e == hipSuccess) e = hipMemcpyAsync(h.data(), p, 4096, hipMemcpyDeviceToHost, d_->stream);
+	if (e == hipSuccess) e = hipStreamSynchronize(d_->stream);
 	(void)hipFree(p);
 	HIP_OK(e, "warm");
 	return 0;
@@ -1354,6 +1460,7 @@ int DeviceEngine::warm(std::string &err)
 int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &err)
 {
 	d_->have_seg = false;
+	d_->drop_graph();  // its launches name the buffers of the previous upload
 	Impl &d = *d_;
 	const int n0 = s.n_orig, n = s.trinum;
 	d.n_orig = n0;
@@ -1483,6 +1590,54 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 	}
 	const bool multi = !d.lowest && d.parallel_src > 1;
 	const int cb = (d.R.nslots + kBlock - 1) / kBlock;
+	// one shooting iteration's launches; with events, the ray-tracing kernel
+	// of the first source is bracketed by event pair ev
+	auto iteration = [&](int ev) {
+		if (multi) {
+			hipLaunchKernelGGL(k_collect, dim3(cb > 0 ? cb : 1), dim3(kBlock), 0, st, d.R, d.loop.p, d.cand.p, d.converge);
+			if (d.fma) hipLaunchKernelGGL(k_topk<true>, dim3(1), dim3(1024), 0, st, d.R, d.loop.p, d.cand.p, d.parallel_src);
+			else hipLaunchKernelGGL(k_topk<false>, dim3(1), dim3(1024), 0, st, d.R, d.loop.p, d.cand.p, d.parallel_src);
+		} else if (d.fma) {
+			if (d.lowest) hipLaunchKernelGGL((k_prep<true, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+			else hipLaunchKernelGGL((k_prep<true, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+		} else {
+			if (d.lowest) hipLaunchKernelGGL((k_prep<false, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+			else hipLaunchKernelGGL((k_prep<false, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+		}
+		const int nk = multi ? d.parallel_src : 1;
+		for (int k = 0; grid > 0 && k < nk; k++) {
+			const dim3 g(grid), blk(kBlock), tg(tgrid);
+			if (d.fma) hipLaunchKernelGGL(k_gen<true>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
+			else hipLaunchKernelGGL(k_gen<false>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
+			hipLaunchKernelGGL(k_compact, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, d.order.p, 6 * d.R.nslots, d.raycnt.p,
+			                   d.nblocks, d.rkey.p, d.rays.p);
+			if (ev >= 0 && k == 0) (void)hipEventRecord(d.kev[2 * ev], st);
+			if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
+			else hipLaunchKernelGGL(k_trace<false>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
+			if (ev >= 0 && k == 0) (void)hipEventRecord(d.kev[2 * ev + 1], st);
+#define UPD(F, FS, LO) hipLaunchKernelGGL((k_update<F, FS, LO>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.converge, d.bcnt.p, d.nblocks)
+			if (multi) { if (d.fma) UPD(true, false, false); else UPD(false, false, false); }
+			else if (d.lowest) { if (d.fma) UPD(true, true, true); else UPD(false, true, true); }
+			else { if (d.fma) UPD(true, true, false); else UPD(false, true, false); }
+#undef UPD
+		}
+	};
+	// Without kernel timing, runs of kGraphIters iterations replay one
+	// captured graph (the kernels exit at once after convergence, and none
+	// of the launches is a memory set), so the loop is not bound by the
+	// host's launch rate on small scenes.
+	if (!timing && !d.graph_exec) {
+		HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
+		for (int i = 0; i < kGraphIters; i++) iteration(-1);
+		hipGraph_t gph = nullptr;
+		const hipError_t ce = hipStreamEndCapture(st, &gph);
+		if (ce != hipSuccess || !gph) {
+			err = std::string("graph capture: ") + hipGetErrorString(ce);
+			return -1;
+		}
+		d.graph = gph;
+		HIP_OK(hipGraphInstantiate(&d.graph_exec, d.graph, nullptr, nullptr, 0), "graph instantiate");
+	}
 	int queued = 0;
 	while (queued < max_shots) {
 		// A candidate's unshot energy only grows until it is shot, so the
@@ -1497,36 +1652,13 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 		                     : std::max(kMinQueue, (d.host_loop->ncand + per_iter - 1) / per_iter);
 		const int b = std::min(std::min(d.batch, left), max_shots - queued);
 		HIP_OK(hipEventRecord(d.ev0, st), "event");
-		for (int i = 0; i < b; i++) {
-			if (multi) {
-				hipLaunchKernelGGL(k_collect, dim3(cb > 0 ? cb : 1), dim3(kBlock), 0, st, d.R, d.loop.p, d.cand.p, d.converge);
-				if (d.fma) hipLaunchKernelGGL(k_topk<true>, dim3(1), dim3(1024), 0, st, d.R, d.loop.p, d.cand.p, d.parallel_src);
-				else hipLaunchKernelGGL(k_topk<false>, dim3(1), dim3(1024), 0, st, d.R, d.loop.p, d.cand.p, d.parallel_src);
-			} else if (d.fma) {
-				if (d.lowest) hipLaunchKernelGGL((k_prep<true, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
-				else hipLaunchKernelGGL((k_prep<true, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
-			} else {
-				if (d.lowest) hipLaunchKernelGGL((k_prep<false, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
-				else hipLaunchKernelGGL((k_prep<false, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
-			}
-			const int nk = multi ? d.parallel_src : 1;
-			for (int k = 0; grid > 0 && k < nk; k++) {
-				const dim3 g(grid), blk(kBlock), tg(tgrid);
-				if (d.fma) hipLaunchKernelGGL(k_gen<true>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
-				else hipLaunchKernelGGL(k_gen<false>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
-				hipLaunchKernelGGL(k_compact, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, d.order.p, 6 * d.R.nslots, d.raycnt.p,
-				                   d.nblocks, d.rkey.p, d.rays.p);
-				if (timing && k == 0) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv)], st), "event");
-				if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
-				else hipLaunchKernelGGL(k_trace<false>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
-				if (timing && k == 0) HIP_OK(hipEventRecord(d.kev[2 * (i % Impl::kEv) + 1], st), "event");
-#define UPD(F, FS, LO) hipLaunchKernelGGL((k_update<F, FS, LO>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.converge, d.bcnt.p, d.nblocks)
-				if (multi) { if (d.fma) UPD(true, false, false); else UPD(false, false, false); }
-				else if (d.lowest) { if (d.fma) UPD(true, true, true); else UPD(false, true, true); }
-				else { if (d.fma) UPD(true, true, false); else UPD(false, true, false); }
-#undef UPD
-			}
-			if (timing && (i % Impl::kEv == Impl::kEv - 1 || i == b - 1)) {
+		if (!timing) {
+			for (int i = 0; i + kGraphIters <= b; i += kGraphIters) HIP_OK(hipGraphLaunch(d.graph_exec, st), "graph launch");
+			for (int i = 0; i < b % kGraphIters; i++) iteration(-1);
+		}
+		for (int i = 0; timing && i < b; i++) {
+			iteration(i % Impl::kEv);
+			if (i % Impl::kEv == Impl::kEv - 1 || i == b - 1) {
 				// collect this group of kernel timings before the events are reused
 				HIP_OK(hipEventSynchronize(d.kev[2 * (i % Impl::kEv) + 1]), "event");
 				for (int j = 0; j <= i % Impl::kEv; j++) {
@@ -1600,6 +1732,44 @@ int DeviceEngine::cast(int n, const RayIn *rays, int *hit, int *cells, int *test
 	HIP_OK(hipStreamSynchronize(st), "cast");
 	float ms = 0;
 	if (hipEventElapsedTime(&ms, d.ev0, d.ev1) == hipSuccess) cast_ms_ = ms;
+	return 0;
+}
+
+int DeviceEngine::link_neighbors(HostScene &s, std::string &err)
+{
+	Impl &d = *d_;
+	const int n = s.trinum;
+	if (n <= 0) return 0;
+	if ((long long)n >= (1ll << 30)) {
+		err = "neighbour search: too many triangles for 30-bit slot indices";
+		return -1;
+	}
+	static_assert(sizeof(NbPoint) == 12 * sizeof(float), "NbPoint is 12 floats");
+	std::vector<NbPoint> pts(n);
+	pack_points(s, reinterpret_cast<float *>(pts.data()));
+	const size_t refs = (size_t)3 * n;
+	size_t cap = 1;
+	while (cap < 2 * refs) cap <<= 1;
+	const unsigned mask = (unsigned)(cap - 1);
+	DevBuf<NbPoint> dT;
+	DevBuf<int> heads, next, nb;
+	DevBuf<unsigned long long> best;
+	hipStream_t st = d.stream;
+	if (dT.put(pts.data(), n, st, err) || heads.alloc(cap, err) || next.alloc(refs, err) || nb.alloc(refs, err) ||
+	    best.alloc(refs, err))
+		return -1;
+	hipLaunchKernelGGL(k_fill_n<int>, dim3(1024), dim3(kBlock), 0, st, heads.p, cap, -1);
+	hipLaunchKernelGGL(k_fill_n<unsigned long long>, dim3(1024), dim3(kBlock), 0, st, best.p, refs, 0ull);
+	const unsigned g = (unsigned)((refs + kBlock - 1) / kBlock);
+	hipLaunchKernelGGL(k_nb_insert, dim3(g), dim3(kBlock), 0, st, dT.p, n, mask, heads.p, next.p);
+	if (s.o.fma) hipLaunchKernelGGL(k_nb_pairs<true>, dim3(g), dim3(kBlock), 0, st, dT.p, n, mask, heads.p, next.p, best.p);
+	else hipLaunchKernelGGL(k_nb_pairs<false>, dim3(g), dim3(kBlock), 0, st, dT.p, n, mask, heads.p, next.p, best.p);
+	hipLaunchKernelGGL(k_nb_final, dim3(g), dim3(kBlock), 0, st, best.p, n, nb.p);
+	HIP_OK(hipGetLastError(), "neighbour search launch");
+	std::vector<int> h(refs);
+	HIP_OK(hipMemcpyAsync(h.data(), nb.p, refs * sizeof(int), hipMemcpyDeviceToHost, st), "neighbour download");
+	HIP_OK(hipStreamSynchronize(st), "neighbour search");
+	unpack_neighbors(s, h.data());
 	return 0;
 }
 

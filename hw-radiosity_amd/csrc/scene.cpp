@@ -92,6 +92,21 @@ float cull_radius(const Patch &t, bool fma, float max_amp)
 	return std::isfinite(rf) ? std::nextafter(rf, kInf) : kInf;
 }
 
+void pack_points(const HostScene &s, float *out12)
+{
+#pragma omp parallel for schedule(static)
+	for (int i = 0; i < s.trinum; i++) {
+		std::memcpy(out12 + 12 * (size_t)i, s.pool[i].p, 9 * sizeof(float));
+		std::memcpy(out12 + 12 * (size_t)i + 9, s.pool[i].n, 3 * sizeof(float));
+	}
+}
+
+void unpack_neighbors(HostScene &s, const int *nb3)
+{
+#pragma omp parallel for schedule(static)
+	for (int i = 0; i < s.trinum; i++) std::memcpy(s.pool[i].neighbor, nb3 + 3 * (size_t)i, 3 * sizeof(int));
+}
+
 float cull_radii(const HostScene &s, float *radius)
 {
 	double cl = 0.0;
@@ -276,7 +291,7 @@ template <bool F> void link_neighbors(PatchPool &T, int n)
 
 }  // namespace
 
-void HostScene::init_radiosity()
+void HostScene::init_radiosity(bool link_on_host)
 {
 	float amin = 100000000.0f, amax = 0.0f;
 	for (int i = 0; i < trinum; i++) {
@@ -296,8 +311,10 @@ void HostScene::init_radiosity()
 	}
 	min_area = amin;
 	max_area = amax;
-	if (o.fma) link_neighbors<true>(pool, trinum);
-	else link_neighbors<false>(pool, trinum);
+	if (link_on_host) {
+		if (o.fma) link_neighbors<true>(pool, trinum);
+		else link_neighbors<false>(pool, trinum);
+	}
 	if (o.sample_area < 0.0f) o.sample_area = (float)((double)amax / 100.0);
 	if (!o.quiet) {
 		std::printf("[INFO] Max Patch Area = %f\n", amax);

@@ -49,8 +49,9 @@ struct HostScene {
 	// ReadTriangle (rad.cpp:47-111): derived fields and the grid corners.
 	int set_triangles(int n, const float *p9, const int *frgb3, const int *brgb3, std::string &err);
 	int load_tri(const char *path, std::string &err);
-	// InitRadiosity (rad.cpp:259-356).
-	void init_radiosity();
+	// InitRadiosity (rad.cpp:259-356); the neighbour search (rad.cpp:295-334)
+	// here when link_on_host, else the caller runs it on the device.
+	void init_radiosity(bool link_on_host = true);
 	// PrePartitionTriangles (shade.cpp:556-598).
 	void prepartition();
 
@@ -68,5 +69,9 @@ float cull_radius(const Patch &t, bool fma, float max_amp);
 // cull_radius (max_amp 16) of every original of s into radius[0, n_orig);
 // returns a bound on |c|_1 over the originals with a finite radius.
 float cull_radii(const HostScene &s, float *radius);
+// Slots [0, s.trinum) as 12 floats each (p[3][3], n[3]), and the inverse
+// for the neighbour links (3 ints each), for the device neighbour search.
+void pack_points(const HostScene &s, float *out12);
+void unpack_neighbors(HostScene &s, const int *nb3);
 
 }  // namespace rgpu

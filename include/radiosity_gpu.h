@@ -53,6 +53,7 @@ enum {
  *   -batch N          shooting iterations queued between host checks (default 32)
  *   -kernel_timing 0|1  time every shoot kernel with HIP events (rad_timings)
  *   -quiet 0|1        no log lines
+ *   -neighbors device|host  where InitRadiosity's neighbour search runs (default device)
  * Replaces ProcessOption (rad.cpp:525-631) for everything but the input name. */
 rad_status rad_create(rad_ctx **out, int argc, const char *const *argv);
 void rad_destroy(rad_ctx *ctx);
@@ -64,8 +65,10 @@ const char *rad_last_error(const rad_ctx *ctx);
 rad_status rad_load_tri(rad_ctx *ctx, const char *path);
 rad_status rad_set_triangles(rad_ctx *ctx, int n, const float *p9, const int *frgb3, const int *brgb3);
 
-/* InitRadiosity + PrePartitionTriangles on the host (rad.cpp:259-356,
- * shade.cpp:556-598).  Needs no GPU. */
+/* InitRadiosity + PrePartitionTriangles (rad.cpp:259-356,
+ * shade.cpp:556-598).  InitRadiosity's neighbour search (rad.cpp:295-334)
+ * runs on the device (needs a GPU: RAD_EDEVICE without one) unless the
+ * context was created with -neighbors host; the rest runs on the host. */
 rad_status rad_prepare_host(rad_ctx *ctx);
 /* Copies the prepared scene and octree to the device and selects the first
  * source.  Needs a GPU. */

@@ -30,7 +30,8 @@ def case_keys():
 
 
 def engine_for(entry, *extra):
-    return radgpu.Engine(*(entry["opts"] + ["-fma", "1" if entry["variant"] == "fma" else "0", "-quiet", "1"]
+    return radgpu.Engine(*(entry["opts"] + ["-fma", "1" if entry["variant"] == "fma" else "0", "-quiet", "1",
+                           "-neighbors", "host"]
                            + list(extra)))
 
 
@@ -77,7 +78,7 @@ def test_fin_text_matches_oracle(inputs, tmp_path):
         o.init_radiosity()
         o.prepartition()
         o.write_fin(str(tmp_path / "o.fin"))
-        with radgpu.Engine(*(opts + ["-quiet", "1"])) as eng:
+        with radgpu.Engine(*(opts + ["-quiet", "1", "-neighbors", "host"])) as eng:
             eng.load_tri(inputs[m]).prepare_host()
             eng.write_fin(str(tmp_path / "e.fin"))
         assert open(tmp_path / "o.fin", "rb").read() == open(tmp_path / "e.fin", "rb").read(), m
@@ -149,7 +150,7 @@ def test_hall_host_stages_match_oracle():
     from radgpu import scenes
     s = scenes.hall_scene(2)
     opts = ["-converge", "1200", "-triangle", "1000000", "-quiet", "1"]
-    with radgpu.Engine(*opts) as eng:
+    with radgpu.Engine(*(opts + ["-neighbors", "host"])) as eng:
         rec, g = eng.set_scene(s).prepare_host().records()
     o = ol.Oracle(*(opts + ["-jobs", "8"]))
     o.load(s)
@@ -248,7 +249,7 @@ def test_edge_scene_host_stages_match_oracle(name):
     from edge_scenes import edge_scenes
     s = edge_scenes()[name]
     opts = ["-converge", "1200", "-triangle", "20000", "-quiet", "1"]
-    with radgpu.Engine(*opts) as eng:
+    with radgpu.Engine(*(opts + ["-neighbors", "host"])) as eng:
         a = eng.set_scene(s).tree_info()
         rec, g = eng.prepare_host().records()
     o = ol.Oracle(*opts)

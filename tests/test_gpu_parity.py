@@ -351,3 +351,64 @@ def test_parallel_src_zero_shoots_nothing(inputs, tmp_path):
         assert rr.returncode == 0
         assert "Complete in #Iterations 1\n" in re.sub(r"\x1b\[[0-9;]*m", "", rr.stderr)
         assert fin == open(tmp_path / "ref.fin", "rb").read()
+
+
+@pytest.mark.parametrize("scene", ["c1", "hall_subset"])
+def test_graph_replays_equal_eager_launches(scene):
+    """The shooting loop replays captured graphs of 8 iterations between host
+    checks; with -kernel_timing 1 it launches every kernel eagerly instead.
+    Solves in pieces (graph replays, eager tails and host synchronisations
+    interleaved) give the same shots and records as the eager solve."""
+    if scene == "c1":
+        s, opts = scenes.room_scene(1), ["-converge", "100", "-triangle", "20000", "-quiet", "1"]
+    else:
+        s, opts = scenes.hall_scene(2, n_boxes=40), ["-converge", "175", "-triangle", "1000000", "-quiet", "1"]
+    print("graph vs eager", scene, flush=True)
+    with radgpu.Engine(*(opts + ["-kernel_timing", "1"])) as a, radgpu.Engine(*opts) as b:
+        a.set_scene(s).prepare_host().upload()
+        b.set_scene(s).prepare_host().upload()
+        na, ca = a.shoot()
+        nb = 0
+        for chunk in (5, 8, 17, 1 << 30):
+            k, cb = b.shoot(chunk)
+            nb += k
+            if cb:
+                break
+        assert ca and cb and na == nb and na > 24
+        assert a.records()[0].tobytes() == b.records()[0].tobytes()
+        assert a.counters() == b.counters()
+
+
+@pytest.mark.parametrize("key", case_keys())
+def test_device_neighbour_search_matches_reference(key, goldens, inputs):
+    """InitRadiosity's neighbour search on the device (the default): the
+    records after pre-partition (which reads and rewrites the neighbour
+    links) hash to the reference's, as the host search's do."""
+    e = goldens["cases"][key]
+    with radgpu.Engine(*opts_of(e)) as eng:
+        eng.load_tri(inputs[e["input"]]).prepare_host()
+        rec, g = eng.records()
+    assert len(rec) == e["prepart"]["trinum"]
+    assert sha(rec, g) == e["prepart"]["sha256"]
+
+
+def test_device_neighbour_search_on_the_hall():
+    """The 200k-triangle hall: device and host neighbour searches give the
+    same records after InitRadiosity and pre-partition, equal to the
+    oracle's; and with the device search the set-up takes less host time."""
+    s = scenes.hall_scene(2)
+    opts = ["-converge", "1200", "-triangle", "1000000", "-quiet", "1"]
+    print("hall neighbours", s.n, flush=True)
+    with radgpu.Engine(*opts) as dev, radgpu.Engine(*(opts + ["-neighbors", "host"])) as host:
+        rd, gd = dev.set_scene(s).prepare_host().records()
+        rh, gh = host.set_scene(s).prepare_host().records()
+        td, th = dev.timings()["setup_ms"], host.timings()["setup_ms"]
+    print("setup ms: device search %.1f, host search %.1f" % (td, th), flush=True)
+    assert (rd["neighbor"] >= 0).mean() > 0.9
+    assert rd.tobytes() == rh.tobytes() and np.array_equal(gd, gh)
+    o = ol.Oracle(*(opts + ["-jobs", "16"]))
+    o.load(s)
+    o.init_radiosity()
+    o.prepartition()
+    orec, _ = o.dump()
+    assert rd.tobytes() == orec.tobytes()
