@@ -99,11 +99,28 @@ void join_device(rad_ctx *c)
 	}
 }
 
+// A device for the context: the start-up under way, or a new one.
+rad_status need_device(rad_ctx *c)
+{
+	if (c->dev) return RAD_OK;
+	join_device(c);
+	if (c->dev) return RAD_OK;
+	DeviceEngine *d = DeviceEngine::create(c->err);
+	if (!d) return RAD_EDEVICE;
+	c->dev.reset(d);
+	return RAD_OK;
+}
+
 rad_status after_load(rad_ctx *c)
 {
-	const auto t0 = Clock::now();
 	HostScene &s = c->scene;
-	c->tree = build_octree(s.pool.data(), s.n_orig, s.G0, s.G1, s.o.fma != 0, s.o.jobs > 0 ? std::max(s.o.jobs, 8) : 0);
+	c->loaded = c->prepared = c->uploaded = c->device_newer = false;
+	if (!s.o.octree_host && need_device(c) != RAD_OK) return RAD_EDEVICE;
+	const auto t0 = Clock::now();
+	if (s.o.octree_host)
+		c->tree = build_octree(s.pool.data(), s.n_orig, s.G0, s.G1, s.o.fma != 0, s.o.jobs > 0 ? std::max(s.o.jobs, 8) : 0);
+	else if (c->dev->build_octree(s, c->tree, c->err))
+		return RAD_EDEVICE;
 	s.log("[INFO] 3D-DDA #TreeNode = %d, #Trilist = %d\n", c->tree.n_nodes, c->tree.n_list);
 	c->ms_setup += ms_since(t0);
 	c->loaded = true;
@@ -183,12 +200,12 @@ rad_status rad_load_tri(rad_ctx *c, const char *path)
 	start_device(c);
 	const auto t0 = Clock::now();
 	c->ms_setup = 0;
-	rad_status st = RAD_OK;
-	if (c->scene.load_tri(path, c->err)) st = RAD_EIO;
-	else {
-		c->ms_setup += ms_since(t0);
-		st = after_load(c);
+	if (c->scene.load_tri(path, c->err)) {
+		join_device(c);
+		return RAD_EIO;
 	}
+	c->ms_setup += ms_since(t0);
+	const rad_status st = after_load(c);  // joins the start-up for the device octree
 	join_device(c);
 	return st;
 }
@@ -199,12 +216,12 @@ rad_status rad_set_triangles(rad_ctx *c, int n, const float *p9, const int *frgb
 	start_device(c);  // as rad_load_tri
 	const auto t0 = Clock::now();
 	c->ms_setup = 0;
-	rad_status st = RAD_OK;
-	if (c->scene.set_triangles(n, p9, frgb3, brgb3, c->err)) st = RAD_EINVAL;
-	else {
-		c->ms_setup += ms_since(t0);
-		st = after_load(c);
+	if (c->scene.set_triangles(n, p9, frgb3, brgb3, c->err)) {
+		join_device(c);
+		return RAD_EINVAL;
 	}
+	c->ms_setup += ms_since(t0);
+	const rad_status st = after_load(c);
 	join_device(c);
 	return st;
 }
@@ -216,14 +233,7 @@ rad_status rad_prepare_host(rad_ctx *c)
 	if (c->prepared) return RAD_OK;
 	if (c->scene.o.write_cycle <= 0) return fail(c, RAD_EINVAL, "-write_cycle must be positive");
 	const bool on_device = c->scene.o.neighbors_host == 0;
-	if (on_device && !c->dev) {  // the neighbour search needs the device
-		join_device(c);
-		if (!c->dev) {
-			DeviceEngine *d = DeviceEngine::create(c->err);
-			if (!d) return RAD_EDEVICE;
-			c->dev.reset(d);
-		}
-	}
+	if (on_device && need_device(c) != RAD_OK) return RAD_EDEVICE;  // the neighbour search needs it
 	const auto t0 = Clock::now();
 	c->scene.init_radiosity(!on_device);
 	if (on_device && c->dev->link_neighbors(c->scene, c->err)) return RAD_EDEVICE;
@@ -339,6 +349,15 @@ rad_status rad_tree_info(rad_ctx *c, int *out3, float *mgl)
 	out3[1] = c->tree.n_list;
 	out3[2] = c->tree.grid_num;
 	*mgl = c->tree.min_grid;
+	return RAD_OK;
+}
+
+rad_status rad_tree_arrays(rad_ctx *c, void *nodes, int *lists)
+{
+	if (!c) return RAD_EINVAL;
+	if (!c->loaded) return fail(c, RAD_ESTATE, "no scene loaded");
+	if (nodes) std::memcpy(nodes, c->tree.ref_nodes.data(), c->tree.ref_nodes.size() * sizeof(RefNode));
+	if (lists) std::memcpy(lists, c->tree.ref_lists.data(), c->tree.ref_lists.size() * sizeof(int));
 	return RAD_OK;
 }
 

@@ -51,6 +51,8 @@ public:
 	int shoot(int max_shots, int *shots, int *converged, std::string &err);
 	// Copies accB, deltaaccB and deltaB back into the host pool.
 	int download(HostScene &s, std::string &err);
+	// BuildTree (raycast.cpp:168-233) over the originals of s on the device.
+	int build_octree(const HostScene &s, FlatOctree &t, std::string &err);
 	// InitRadiosity's neighbour search (rad.cpp:295-334) for slots [0,
 	// s.trinum) on the device: writes s.pool[i].neighbor.
 	int link_neighbors(HostScene &s, std::string &err);

@@ -53,6 +53,7 @@
 
 #include "device.h"
 #include "../../include/radiosity_gpu.h"
+#include "octree_gpu.h"
 #include "pack.h"
 
 namespace rgpu {
@@ -1733,6 +1734,11 @@ int DeviceEngine::cast(int n, const RayIn *rays, int *hit, int *cells, int *test
 	float ms = 0;
 	if (hipEventElapsedTime(&ms, d.ev0, d.ev1) == hipSuccess) cast_ms_ = ms;
 	return 0;
+}
+
+int DeviceEngine::build_octree(const HostScene &s, FlatOctree &t, std::string &err)
+{
+	return build_octree_gpu(d_->stream, s.pool.data(), s.n_orig, s.G0, s.G1, s.o.fma != 0, t, err);
 }
 
 int DeviceEngine::link_neighbors(HostScene &s, std::string &err)

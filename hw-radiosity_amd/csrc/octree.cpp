@@ -236,6 +236,24 @@ FlatOctree build_octree(const Patch *pool, int n, const float *G0, const float *
 	T.n_list = (int)st.items.size() + T.n_leaves;
 	T.min_grid = st.min_grid;
 	T.grid_num = (int)((G1[0] - G0[0]) / st.min_grid);
+	// the nodes are already in depth-first preorder (children stitched in
+	// order after their parent), so the reference's arrays follow directly
+	T.ref_nodes.resize(st.nodes.size());
+	T.ref_lists.reserve((size_t)T.n_list);
+	for (size_t i = 0; i < st.nodes.size(); i++) {
+		const BNode &b = st.nodes[i];
+		RefNode &r = T.ref_nodes[i];
+		std::memcpy(r.g0, b.g0, 12);
+		std::memcpy(r.g1, b.g1, 12);
+		std::memcpy(r.sub, b.sub, sizeof r.sub);
+		r.list = -1;
+		if (b.leaf >= 0) {
+			r.list = (int)T.ref_lists.size();
+			const int beg = st.item_begin[b.leaf], cnt = st.item_count[b.leaf];
+			T.ref_lists.insert(T.ref_lists.end(), st.items.begin() + beg, st.items.begin() + beg + cnt);
+			T.ref_lists.push_back(-1);
+		}
+	}
 	T.items = std::move(st.items);
 	T.leaf_box.assign((size_t)T.n_leaves * 8, 0.0f);
 	for (const BNode &b : st.nodes) {

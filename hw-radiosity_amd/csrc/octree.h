@@ -9,7 +9,20 @@
 
 namespace rgpu {
 
+// TreeNode (rad.h:41-45): 60 bytes, as TreeNodeStore holds it.
+struct RefNode {
+	float g0[3], g1[3];
+	int sub[8];
+	int list;
+};
+static_assert(sizeof(RefNode) == 60, "TreeNode layout");
+
 struct FlatOctree {
+	// the reference's own arrays (raycast.cpp:9-13) as BuildTree leaves them:
+	// nodes in depth-first preorder, each leaf's list followed by -1 in
+	// TriListStore in the same order
+	std::vector<RefNode> ref_nodes;
+	std::vector<int> ref_lists;
 	// reference-equivalent facts
 	int n_nodes = 0;      // TreeNodeStorePtr
 	int n_list = 0;       // TriListStorePtr (items plus one -1 per leaf)

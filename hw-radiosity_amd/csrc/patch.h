@@ -62,6 +62,7 @@ struct Options {
 	int batch = 32;         // shooting iterations queued per host check
 	int kernel_timing = 0;  // time every shoot kernel with HIP events
 	int neighbors_host = 0; // InitRadiosity's neighbour search on the host instead of the device
+	int octree_host = 0;    // BuildTree on the host instead of the device
 };
 
 // Arithmetic at the reference compiler's contraction sites.  With F the

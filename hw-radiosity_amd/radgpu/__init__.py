@@ -36,7 +36,7 @@ RAY_DTYPE = np.dtype([("p", "<f4", 3), ("v", "<f4", 3), ("ban", "<i4"), ("target
 
 EXPORTS = ("rad_create", "rad_destroy", "rad_last_error", "rad_load_tri", "rad_set_triangles",
            "rad_prepare_host", "rad_upload", "rad_shoot", "rad_run", "rad_write_fin", "rad_fin_to_json", "rad_records",
-           "rad_tree_info", "rad_cast", "rad_cast_static", "rad_ff_pairs", "rad_counters", "rad_timings",
+           "rad_tree_info", "rad_tree_arrays", "rad_cast", "rad_cast_static", "rad_ff_pairs", "rad_counters", "rad_timings",
            "rad_lin_create", "rad_lin_destroy", "rad_lin_last_error", "rad_lin_from_scene",
            "rad_lin_set_matrix", "rad_lin_get_matrix", "rad_lin_set_areas", "rad_lin_autotune", "rad_lin_set_tile",
            "rad_lin_gemm_ms", "rad_lin_multiply", "rad_lin_gather", "rad_lin_shoot", "rad_lin_adjoint",
@@ -45,6 +45,8 @@ EXPORTS = ("rad_create", "rad_destroy", "rad_last_error", "rad_load_tri", "rad_s
 
 PREC_F32, PREC_F16, PREC_BF16 = 0, 1, 2
 CAST_STATIC_MAX = 1 << 18
+# TreeNode (rad.h:41-45)
+NODE_DTYPE = np.dtype([("g0", "<f4", 3), ("g1", "<f4", 3), ("sub", "<i4", 8), ("list", "<i4")])
 VIS_SEGMENT, VIS_REFERENCE = 0, 1
 PRECISIONS = {"f32": PREC_F32, "f16": PREC_F16, "bf16": PREC_BF16}
 INFO_NAMES = ("outer", "sweeps", "rel_res", "ms", "ms_target", "err", "power_E", "power_B")
@@ -80,6 +82,7 @@ def library() -> ctypes.CDLL:
             "rad_fin_to_json": [vp, cp, cp, ctypes.POINTER(ip)],
             "rad_records": [vp, ctypes.POINTER(ip), vp, vp],
             "rad_tree_info": [vp, vp, ctypes.POINTER(ctypes.c_float)],
+            "rad_tree_arrays": [vp, vp, vp],
             "rad_cast": [vp, ip, vp, vp, vp, vp],
             "rad_cast_static": [vp, ip, vp, vp, vp, vp],
             "rad_ff_pairs": [vp, ip, vp, vp],
@@ -206,6 +209,15 @@ class Engine:
         mgl = ctypes.c_float(0)
         self._ok(library().rad_tree_info(self.h, _p(out), ctypes.byref(mgl)))
         return int(out[0]), int(out[1]), int(out[2]), mgl.value
+
+    def tree_arrays(self):
+        """(nodes, lists): the octree as the reference's TreeNodeStore
+        (NODE_DTYPE records, depth-first preorder) and TriListStore."""
+        nn, nl, _, _ = self.tree_info()
+        nodes = np.zeros(nn, NODE_DTYPE)
+        lists = np.zeros(nl, np.int32)
+        self._ok(library().rad_tree_arrays(self.h, _p(nodes), _p(lists)))
+        return nodes, lists
 
     def cast(self, rays):
         rays = np.ascontiguousarray(rays, RAY_DTYPE)

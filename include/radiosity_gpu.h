@@ -54,6 +54,7 @@ enum {
  *   -kernel_timing 0|1  time every shoot kernel with HIP events (rad_timings)
  *   -quiet 0|1        no log lines
  *   -neighbors device|host  where InitRadiosity's neighbour search runs (default device)
+ *   -octree device|host     where BuildTree runs (default device)
  * Replaces ProcessOption (rad.cpp:525-631) for everything but the input name. */
 rad_status rad_create(rad_ctx **out, int argc, const char *const *argv);
 void rad_destroy(rad_ctx *ctx);
@@ -61,7 +62,10 @@ const char *rad_last_error(const rad_ctx *ctx);
 
 /* Input.  ReadTriangle + BuildTree (InitModelFromFile, rad.cpp:244-252,
  * rad.cpp:47-111, raycast.cpp:216-233).  rad_set_triangles takes n
- * triangles as p9 (x,y,z of 3 vertices), Frgb and Brgb triples. */
+ * triangles as p9 (x,y,z of 3 vertices), Frgb and Brgb triples.  The
+ * octree is built on the device (needs a GPU: RAD_EDEVICE without one)
+ * unless the context was created with -octree host; the device starts up
+ * while the input is read. */
 rad_status rad_load_tri(rad_ctx *ctx, const char *path);
 rad_status rad_set_triangles(rad_ctx *ctx, int n, const float *p9, const int *frgb3, const int *brgb3);
 
@@ -106,6 +110,13 @@ rad_status rad_records(rad_ctx *ctx, int *trinum, void *records, float *g6);
 /* Octree facts: out3 = {#nodes, #list entries, GridNum}, *min_grid_len =
  * MinGridLen (raycast.cpp:175-232). */
 rad_status rad_tree_info(rad_ctx *ctx, int *out3, float *min_grid_len);
+
+/* The octree as the reference's TreeNodeStore / TriListStore hold it after
+ * BuildTree (raycast.cpp:9-13, 168-233): out3[0] (rad_tree_info) records of
+ * the 60-byte TreeNode layout (rad.h:41-45: g0, g1, sub[8], list) in
+ * depth-first preorder, and out3[1] list entries (each leaf's ascending
+ * originals followed by -1).  nodes or lists may be NULL. */
+rad_status rad_tree_arrays(rad_ctx *ctx, void *nodes, int *lists);
 
 /* RayHitted (raycast.cpp:335-360) for n rays, each {float p[3], float v[3],
  * int ban, int target} (32 bytes), against the uploaded scene.  hit is the

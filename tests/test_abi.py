@@ -50,17 +50,20 @@ def test_cli_usage_and_open_error(tmp_path):
 
 
 def test_run_without_a_device_is_a_device_error(tmp_path):
-    """The product path has no CPU fallback: with no HIP device the run stops
-    with RAD_EDEVICE (the device start-up thread does not start) and writes
-    no .fin."""
+    """The product path has no CPU fallback: with no HIP device the octree
+    build, and then the run, stop with RAD_EDEVICE (the device start-up
+    thread does not start) and write no .fin."""
     import pytest
     from radgpu import scenes
     probe = subprocess.run(["bash", "-c", "ls /dev/kfd"], capture_output=True)
     if probe.returncode == 0:
         pytest.skip("a GPU device node is present")
-    with radgpu.Engine("-quiet", "1") as e:
+    with pytest.raises(radgpu.RadError) as ei:   # the octree is built on the device
+        radgpu.Engine("-quiet", "1").set_scene(scenes.room_scene(1))
+    assert ei.value.status == radgpu.RAD_EDEVICE and "GPU only" in str(ei.value)
+    with radgpu.Engine("-quiet", "1", "-octree", "host") as e:
         e.set_scene(scenes.room_scene(1))
-        with pytest.raises(radgpu.RadError) as ei:
+        with pytest.raises(radgpu.RadError) as ei:   # so is the neighbour search, and the rest
             e.run(str(tmp_path / "out"))
         assert ei.value.status == radgpu.RAD_EDEVICE and "GPU only" in str(ei.value)
     assert not os.path.exists(str(tmp_path / "out.fin"))

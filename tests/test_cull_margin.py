@@ -33,7 +33,7 @@ def smallest_float_at_least(x):
 def test_cull_never_rejects_an_accepted_hit():
     import radgpu
     s = scenes.hall_scene(2, n_boxes=60)
-    with radgpu.Engine("-quiet", "1", "-triangle", "1000000") as e:
+    with radgpu.Engine("-quiet", "1", "-triangle", "1000000", "-octree", "host") as e:
         e.set_scene(s)                      # ReadTriangle's records: every one consistent with its vertices
         rec, _ = e.records()
     rng = np.random.default_rng(7)

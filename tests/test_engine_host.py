@@ -31,7 +31,7 @@ def case_keys():
 
 def engine_for(entry, *extra):
     return radgpu.Engine(*(entry["opts"] + ["-fma", "1" if entry["variant"] == "fma" else "0", "-quiet", "1",
-                           "-neighbors", "host"]
+                           "-neighbors", "host", "-octree", "host"]
                            + list(extra)))
 
 
@@ -52,7 +52,7 @@ def test_host_stages_match_reference(key, goldens, inputs):
 def test_set_triangles_equals_file(inputs):
     from radgpu import scenes
     s = scenes.read_tri(inputs["c1_seed1.tri"])
-    with radgpu.Engine("-quiet", "1") as a, radgpu.Engine("-quiet", "1") as b:
+    with radgpu.Engine("-quiet", "1", "-octree", "host") as a, radgpu.Engine("-quiet", "1", "-octree", "host") as b:
         a.load_tri(inputs["c1_seed1.tri"])
         b.set_scene(s)
         assert a.records()[0].tobytes() == b.records()[0].tobytes()
@@ -63,7 +63,7 @@ def test_tree_matches_oracle(inputs):
         o = ol.Oracle("-quiet", "1")
         o.load_file(inputs[m])
         o.build_tree()
-        with radgpu.Engine("-quiet", "1") as eng:
+        with radgpu.Engine("-quiet", "1", "-octree", "host") as eng:
             eng.load_tri(inputs[m])
             a, b = eng.tree_info(), o.tree_info()
             assert a[:3] == b[:3] and struct.pack("f", a[3]) == struct.pack("f", b[3]), m
@@ -78,7 +78,7 @@ def test_fin_text_matches_oracle(inputs, tmp_path):
         o.init_radiosity()
         o.prepartition()
         o.write_fin(str(tmp_path / "o.fin"))
-        with radgpu.Engine(*(opts + ["-quiet", "1", "-neighbors", "host"])) as eng:
+        with radgpu.Engine(*(opts + ["-quiet", "1", "-neighbors", "host", "-octree", "host"])) as eng:
             eng.load_tri(inputs[m]).prepare_host()
             eng.write_fin(str(tmp_path / "e.fin"))
         assert open(tmp_path / "o.fin", "rb").read() == open(tmp_path / "e.fin", "rb").read(), m
@@ -126,7 +126,7 @@ def test_number_format_known_answers(tmp_path):
     s = scenes.Scene(p, np.full((len(p), 3), 100, np.int32), np.zeros((len(p), 3), np.int32))
     path = str(tmp_path / "v.tri")
     scenes.write_tri(path, s)
-    with radgpu.Engine("-quiet", "1") as eng:
+    with radgpu.Engine("-quiet", "1", "-octree", "host") as eng:
         eng.load_tri(path)
         eng.write_fin(str(tmp_path / "v.fin"))
     lines = open(tmp_path / "v.fin").read().split("\n")
@@ -150,7 +150,7 @@ def test_hall_host_stages_match_oracle():
     from radgpu import scenes
     s = scenes.hall_scene(2)
     opts = ["-converge", "1200", "-triangle", "1000000", "-quiet", "1"]
-    with radgpu.Engine(*(opts + ["-neighbors", "host"])) as eng:
+    with radgpu.Engine(*(opts + ["-neighbors", "host", "-octree", "host"])) as eng:
         rec, g = eng.set_scene(s).prepare_host().records()
     o = ol.Oracle(*(opts + ["-jobs", "8"]))
     o.load(s)
@@ -171,7 +171,7 @@ def test_hall_tree_matches_oracle():
     o = ol.Oracle("-quiet", "1", "-triangle", "1000000", "-jobs", "8")
     o.load(s)
     o.build_tree()
-    with radgpu.Engine("-quiet", "1", "-triangle", "1000000") as eng:
+    with radgpu.Engine("-quiet", "1", "-triangle", "1000000", "-octree", "host") as eng:
         eng.set_scene(s)
         a, b = eng.tree_info(), o.tree_info()
     assert a[:3] == b[:3] and struct.pack("f", a[3]) == struct.pack("f", b[3])
@@ -197,7 +197,7 @@ def test_fin_unconvertible_values_across_chunks(tmp_path):
     o = ol.Oracle(*opts)
     o.load(s)
     o.write_fin(str(tmp_path / "o.fin"))
-    with radgpu.Engine(*opts) as eng:
+    with radgpu.Engine(*(opts + ["-octree", "host"])) as eng:
         eng.set_scene(s)
         eng.write_fin(str(tmp_path / "e.fin"))
     a, b = open(tmp_path / "o.fin", "rb").read(), open(tmp_path / "e.fin", "rb").read()
@@ -230,7 +230,7 @@ def test_fin_wholly_unconvertible_record_at_chunk_end(tmp_path):
     o = ol.Oracle(*opts)
     o.load(s)
     o.write_fin(str(tmp_path / "o.fin"))
-    with radgpu.Engine(*opts) as eng:
+    with radgpu.Engine(*(opts + ["-octree", "host"])) as eng:
         eng.set_scene(s)
         eng.write_fin(str(tmp_path / "e.fin"))
     a, b = open(tmp_path / "o.fin", "rb").read(), open(tmp_path / "e.fin", "rb").read()
@@ -249,7 +249,7 @@ def test_edge_scene_host_stages_match_oracle(name):
     from edge_scenes import edge_scenes
     s = edge_scenes()[name]
     opts = ["-converge", "1200", "-triangle", "20000", "-quiet", "1"]
-    with radgpu.Engine(*(opts + ["-neighbors", "host"])) as eng:
+    with radgpu.Engine(*(opts + ["-neighbors", "host", "-octree", "host"])) as eng:
         a = eng.set_scene(s).tree_info()
         rec, g = eng.prepare_host().records()
     o = ol.Oracle(*opts)
@@ -287,7 +287,7 @@ def test_viewer_json_matches_reference_converter(case, goldens, inputs, tmp_path
         o.iterate(1 << 30)
         o.write_fin(fin)
     assert hashlib.md5(open(fin, "rb").read()).hexdigest() == g["fin_md5"]
-    with radgpu.Engine("-quiet", "1") as eng:
+    with radgpu.Engine("-quiet", "1", "-octree", "host") as eng:
         n = eng.fin_to_json(fin, str(tmp_path / "e.json"))
     js = open(tmp_path / "e.json", "rb").read()
     assert n == int(g["stderr"].split()[1])
@@ -313,7 +313,7 @@ def test_tri2json_cli_matches_reference_messages(goldens, inputs, tmp_path):
     r = subprocess.run([exe, "-i", fin, "-o", str(tmp_path / "a.json"), "--format", "COLOR"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0 and r.stderr.endswith("Model-Converter Success\n")
-    with radgpu.Engine("-quiet", "1") as eng:
+    with radgpu.Engine("-quiet", "1", "-octree", "host") as eng:
         eng.fin_to_json(fin, str(tmp_path / "b.json"))
     assert open(tmp_path / "a.json", "rb").read() == open(tmp_path / "b.json", "rb").read()
     assert subprocess.run([exe, "--help"], capture_output=True, timeout=60).returncode == 0
@@ -324,6 +324,28 @@ def test_tri2json_cli_matches_reference_messages(goldens, inputs, tmp_path):
     bad.write_bytes(b"Triangle \n1 2 3 0 0 0\n0 0 0 0 0 1\n1 0 0 0 0 1\n")  # cut short
     assert subprocess.run([exe, "-i", str(bad), "-o", str(tmp_path / "d.json")], capture_output=True,
                           timeout=60).returncode == 1
-    with radgpu.Engine("-quiet", "1") as eng:
+    with radgpu.Engine("-quiet", "1", "-octree", "host") as eng:
         with pytest.raises(radgpu.RadError):
             eng.fin_to_json(str(bad), str(tmp_path / "e.json"))
+
+
+def test_tree_arrays_match_oracle(inputs):
+    """The host octree as the reference's TreeNodeStore / TriListStore
+    (depth-first node order, every leaf list and its -1) equals the
+    oracle's arrays byte for byte."""
+    from radgpu import scenes
+    cases = [(m, None) for m in ("room.tri", "blocks.tri", "c1_seed1.tri")] + [("hall", scenes.hall_scene(2, n_boxes=80))]
+    for m, sc in cases:
+        o = ol.Oracle("-quiet", "1", "-triangle", "1000000")
+        with radgpu.Engine("-quiet", "1", "-triangle", "1000000", "-octree", "host") as eng:
+            if sc is None:
+                o.load_file(inputs[m])
+                eng.load_tri(inputs[m])
+            else:
+                o.load(sc)
+                eng.set_scene(sc)
+            o.build_tree()
+            nodes, lists = eng.tree_arrays()
+        onodes, olists = o.tree_arrays()
+        assert nodes.tobytes() == onodes.tobytes(), m
+        assert np.array_equal(lists, olists), m

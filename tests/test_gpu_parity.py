@@ -412,3 +412,41 @@ def test_device_neighbour_search_on_the_hall():
     o.prepartition()
     orec, _ = o.dump()
     assert rd.tobytes() == orec.tobytes()
+
+
+def test_device_octree_matches_oracle(inputs):
+    """BuildTree on the device (the default): the reference's TreeNodeStore
+    and TriListStore (node boxes and child links in depth-first order, every
+    leaf's list) byte-identical to the oracle's and to the host builder's, on
+    the reference's models and the 200k-triangle hall; the hall's build
+    times are printed."""
+    import time
+    cases = [(m, None) for m in ("box.tri", "room.tri", "blocks.tri", "c1_seed1.tri")] + [("hall-200k", scenes.hall_scene(2))]
+    for m, sc in cases:
+        print("octree", m, flush=True)
+        opts = ["-quiet", "1", "-triangle", "1000000"]
+        o = ol.Oracle(*(opts + ["-jobs", "16"]))
+        ms = {}
+        trees = {}
+        for where in ("device", "host"):
+            with radgpu.Engine(*(opts + ["-octree", where])) as eng:
+                t0 = time.perf_counter()
+                if sc is None:
+                    eng.load_tri(inputs[m])
+                else:
+                    eng.set_scene(sc)
+                ms[where] = 1e3 * (time.perf_counter() - t0)
+                trees[where] = (eng.tree_info(), eng.tree_arrays())
+        if sc is None:
+            o.load_file(inputs[m])
+        else:
+            o.load(sc)
+        o.build_tree()
+        onodes, olists = o.tree_arrays()
+        info, (nodes, lists) = trees["device"]
+        print("%s: %d nodes, %d list entries; set_scene %.1f ms (device octree) %.1f ms (host)" % (
+            m, info[0], info[1], ms["device"], ms["host"]), flush=True)
+        assert info == trees["host"][0] and info[:3] == o.tree_info()[:3]
+        assert nodes.tobytes() == onodes.tobytes(), m
+        assert np.array_equal(lists, olists), m
+        assert nodes.tobytes() == trees["host"][1][0].tobytes()
