@@ -25,8 +25,14 @@ NOFMA_FL:= $(BASE) -ffp-contract=off
 SRCS    := $(REF)/src/shade.cpp $(REF)/src/raycast.cpp $(REF)/src/rad.cpp
 HDRS    := $(wildcard $(REF)/include/*.h)
 
+# rad_v3 is the reference's own variant with Delta-ff adaptive splitting live:
+# its config switches (include/config.h) set on the command line, with
+# PARALLEL (centroid 3-way splits, PartitionTriangleWithCenterAndStore) and
+# without DONT_PARTITION; _CONFIG_H keeps config.h itself from defining them.
+V3_FL   := -D_CONFIG_H -DADAPT -DROLLBACK -DPRE_PARTITION -DPARALLEL
+
 .PHONY: all clean
-all: $(OUT)/rad $(OUT)/rad_nofma $(OUT)/rad_oldfmt $(OUT)/refdump $(OUT)/refdump_nofma $(OUT)/tri2json
+all: $(OUT)/rad $(OUT)/rad_nofma $(OUT)/rad_oldfmt $(OUT)/rad_v3 $(OUT)/refdump $(OUT)/refdump_nofma $(OUT)/tri2json
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -39,6 +45,9 @@ $(OUT)/rad_nofma: $(SRCS) $(HDRS) | $(OUT)
 
 $(OUT)/rad_oldfmt: $(SRCS) $(HDRS) | $(OUT)
 	$(CXX) $(FMA_FL) -DOLD_FORMAT $(SRCS) -o $@ -lm
+
+$(OUT)/rad_v3: $(SRCS) $(HDRS) | $(OUT)
+	$(CXX) $(FMA_FL) $(V3_FL) $(SRCS) -o $@ -lm
 
 $(OUT)/%.fma.o: $(REF)/src/%.cpp $(HDRS) | $(OUT)
 	$(CXX) $(FMA_FL) -Dmain=radref_main -c $< -o $@

@@ -349,3 +349,23 @@ def test_tree_arrays_match_oracle(inputs):
         onodes, olists = o.tree_arrays()
         assert nodes.tobytes() == onodes.tobytes(), m
         assert np.array_equal(lists, olists), m
+
+
+def test_prepartition_at_scale_matches_oracle():
+    """Pre-partition keeps its queue as a binary heap (a std::set of (area,
+    slot) in the reference and the oracle); on the hall scaled 3x, where it
+    grows 206k originals to 444k slots through stale entries and neighbour
+    splits, every record equals the oracle's."""
+    from radgpu import scenes
+    s = scenes.hall_scene(2)
+    s3 = scenes.Scene(p=(s.p * 3).astype(np.float32), frgb=s.frgb, brgb=s.brgb)
+    opts = ["-triangle", "4000000", "-max_tri", "4000000", "-quiet", "1"]
+    with radgpu.Engine(*(opts + ["-neighbors", "host", "-octree", "host"])) as eng:
+        rec, g = eng.set_scene(s3).prepare_host().records()
+    o = ol.Oracle(*(opts + ["-jobs", "8"]))
+    o.load(s3)
+    o.init_radiosity()
+    o.prepartition()
+    orec, og = o.dump()
+    assert len(rec) == len(orec) == 443890
+    assert rec.tobytes() == orec.tobytes()
