@@ -171,6 +171,26 @@ def room_scene(seed: int = 1) -> Scene:
     return bld.scene()
 
 
+def v3_room(seed: int = 4, h: float = 1.25) -> Scene:
+    """A 10 x 8 x 6 room gridded coarsely (cells of about h, so most
+    triangles have every edge >= 1 and area < 1: pre-partition leaves them
+    whole, and the Delta-ff splitting of the reference's V3 variant, which
+    needs squared edges >= 1, can split them), two boxes gridded alike as
+    occluders, and a coloured ceiling light (255, 200, 150) whose vertices,
+    not saturated, are receivers too."""
+    rng = SplitMix64(seed)
+    bld = _Builder()
+    nx, ny = _cells(10, h), _cells(8, h)
+    i0, j0 = nx // 2 - 1, ny // 2 - 1
+    _room_walls(bld, rng, 10.0, 8.0, 6.0, h, (0.2, 0.8),
+                light_rect=[(i0, i0 + 2, j0, j0 + 2)], light_brgb=[(255, 200, 150)])
+    for _ in range(2):
+        a, b, c = rng.uniform(1.5, 3), rng.uniform(1.5, 3), rng.uniform(1.5, 3)
+        cx, cy = rng.uniform(1 + a / 2, 9 - a / 2), rng.uniform(1 + b / 2, 7 - b / 2)
+        _box(bld, rng, cx, cy, a, b, c, 1.0, 0.0, h, (0.2, 0.8))
+    return bld.scene()
+
+
 def _room_cells(h: float, boxes) -> int:
     """Grid cells of the room walls and box faces at cell size h."""
     nx, ny, nz = _cells(10.0, h), _cells(8.0, h), _cells(6.0, h)

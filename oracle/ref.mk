@@ -32,7 +32,8 @@ HDRS    := $(wildcard $(REF)/include/*.h)
 V3_FL   := -D_CONFIG_H -DADAPT -DROLLBACK -DPRE_PARTITION -DPARALLEL
 
 .PHONY: all clean
-all: $(OUT)/rad $(OUT)/rad_nofma $(OUT)/rad_oldfmt $(OUT)/rad_v3 $(OUT)/refdump $(OUT)/refdump_nofma $(OUT)/tri2json
+all: $(OUT)/rad $(OUT)/rad_nofma $(OUT)/rad_oldfmt $(OUT)/rad_v3 $(OUT)/refdump $(OUT)/refdump_nofma $(OUT)/refdump_v3 \
+     $(OUT)/tri2json
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -48,6 +49,12 @@ $(OUT)/rad_oldfmt: $(SRCS) $(HDRS) | $(OUT)
 
 $(OUT)/rad_v3: $(SRCS) $(HDRS) | $(OUT)
 	$(CXX) $(FMA_FL) $(V3_FL) $(SRCS) -o $@ -lm
+
+$(OUT)/%.v3.o: $(REF)/src/%.cpp $(HDRS) | $(OUT)
+	$(CXX) $(FMA_FL) $(V3_FL) -Dmain=radref_main -c $< -o $@
+
+$(OUT)/refdump_v3: $(HERE)/refdump.cpp $(OUT)/shade.v3.o $(OUT)/raycast.v3.o $(OUT)/rad.v3.o
+	$(CXX) $(FMA_FL) $(V3_FL) $^ -o $@ -lm
 
 $(OUT)/%.fma.o: $(REF)/src/%.cpp $(HDRS) | $(OUT)
 	$(CXX) $(FMA_FL) -Dmain=radref_main -c $< -o $@

@@ -160,6 +160,42 @@ struct Options {
 	std::string out = "./test";
 	int interactive = 0, jobs = 2, parallel_src = 1;
 	int fma = 1, lowest = 0, hash = 1, skip_discarded = 0, max_tri = 1000000, old_format = 0, quiet = 0;
+	int v3 = 0;  // the reference's V3 build: Delta-ff splitting live (config.h without DONT_PARTITION)
+};
+
+// glibc's rand() with its default seed (srand is never called): the TYPE_3
+// additive feedback generator of degree 31 and separation 3.  srandom(1)
+// fills r[0..30] with r[0] = 1, r[i] = 16807 r[i-1] mod (2^31 - 1), then
+// r[i] = r[i-31] + r[i-3] (mod 2^32) and output k is r[k + 344] >> 1
+// (the first 310 values are discarded).  Used only by V3's sampling
+// (GetPointInTriangle, shade.cpp:49-68).
+struct GlibcRand {
+	uint32_t r[34];
+	int k = 0;  // r[k % 34] is the oldest of the last 34 values
+	GlibcRand()
+	{
+		int32_t w = 1;
+		uint32_t s[344];
+		s[0] = 1;
+		for (int i = 1; i < 31; i++) {
+			const int32_t hi = w / 127773, lo = w % 127773;
+			w = 16807 * lo - 2836 * hi;
+			if (w < 0) w += 2147483647;
+			s[i] = (uint32_t)w;
+		}
+		for (int i = 31; i < 34; i++) s[i] = s[i - 31];
+		for (int i = 34; i < 344; i++) s[i] = s[i - 31] + s[i - 3];
+		for (int i = 0; i < 34; i++) r[i] = s[310 + i];
+		k = 0;  // r holds s[310 .. 343]; the next value is s[344]
+	}
+	int next()
+	{
+		// s[n] = s[n - 31] + s[n - 3] with n = 344 + k: r holds s[n-34 .. n-1]
+		const uint32_t v = r[(k + 3) % 34] + r[(k + 31) % 34];
+		r[k % 34] = v;
+		k = (k + 1) % 34;
+		return (int)(v >> 1);
+	}
 };
 
 struct Counters {
@@ -198,6 +234,7 @@ struct rs_ctx {
 	int loop = 0;
 	bool warned_full = false;
 	char fmt_buf[64] = {0};  // the reference's static buffer (rad.cpp:179)
+	GlibcRand rng;           // rand()'s state, from the process start
 	Counters cnt;
 
 	void log(const char *fmt, ...)
@@ -669,6 +706,161 @@ struct rs_ctx {
 			ff[v] = f;
 		}
 	}
+	// AdaptCalFF(CalFF(..)) at one point p of the receiver with normal dn
+	// (shade.cpp:141-168, 445-446)
+	template <bool F> float point_ff(const Source &s, const float *dn, int logdst, const float *p, long long *c) const
+	{
+		float f = 0.0f;
+		if (!s.split || !o.skip_discarded) f = cal_ff<F>(s.whole, s.log, dn, logdst, p, c);
+		if (s.split) {
+			float f1 = cal_ff<F>(s.half[0], s.log, dn, logdst, p, c);
+			float f2 = cal_ff<F>(s.half[1], s.log, dn, logdst, p, c);
+			f = f1 + f2;
+		}
+		return f;
+	}
+	static float ffabs(float x) { return (float)fabs((double)x); }  // rad.h:16
+	// GetPointInTriangle (shade.cpp:49-68): a point from three rand() weights
+	// and the form factor interpolated there; GCC contracts both sums as dot
+	// products (fma(w3, x3, fma(w1, x1, w2 * x2))).
+	template <bool F> float random_point(const Rec &t, float *p, const float ff[3])
+	{
+		float r1 = (float)rng.next(), r2 = (float)rng.next(), r3 = (float)rng.next();
+		const float tot = r1 + r2 + r3;
+		r1 /= tot;
+		r2 /= tot;
+		r3 /= tot;
+		const float w[3] = {r1, r2, r3};
+		for (int v = 0; v < 3; v++) {
+			const float x[3] = {t.p[0][v], t.p[1][v], t.p[2][v]};
+			p[v] = dot3<F>(w, x);
+		}
+		return dot3<F>(w, ff);
+	}
+	// CalArea (vector.h:58-62): n = (p1 - p0) x (p2 - p0), not normalized;
+	// area = |n| / 2
+	template <bool F> static void cal_area(Rec &t)
+	{
+		float v0[3], v1[3];
+		for (int k = 0; k < 3; k++) {
+			v0[k] = t.p[1][k] - t.p[0][k];
+			v1[k] = t.p[2][k] - t.p[0][k];
+		}
+		cross<F>(v0, v1, t.n);
+		t.area = norm<F>(t.n) / 2;
+	}
+	// PartitionTriangleWithCenterAndStore (shade.cpp:261-306) with its 3-way
+	// PartitionDestination (shade.cpp:213-241): -1 when slot i cannot split,
+	// 0 when it split and, being its own parent, became logical, 1 when its
+	// slot now holds the third part (the last slot is given back).
+	template <bool F> int split_center(int i)
+	{
+		if (full_pool(4) || (double)T[i].area < 1e-8) return -1;
+		float maxl = 0.0f, minl = 1e+30f;
+		for (int v = 0; v < 3; v++) {
+			float l[3];
+			for (int k = 0; k < 3; k++) l[k] = T[i].p[(v + 1) % 3][k] - T[i].p[v][k];
+			const float len = norm2<F>(l);
+			maxl = (maxl < len) ? len : maxl;  // std::max
+			minl = (len < minl) ? len : minl;  // std::min
+		}
+		if ((double)(minl / maxl) < 1e-1 || minl < 1) return -1;
+		const int id[3] = {alloc_slot(), alloc_slot(), alloc_slot()};
+		const Rec src = T[i];
+		for (int k = 0; k < 3; k++) {
+			Rec &t = T[id[k]];
+			t = src;
+			memcpy(t.p[k], src.c, 12);
+			memcpy(t.accB[k], src.accB[k], 12);
+			center(t.p, t.c);
+			cal_area<F>(t);
+			t.parent = src.parent;
+		}
+		T[id[0]].neighbor[0] = id[2], T[id[0]].neighbor[2] = id[1];
+		T[id[1]].neighbor[1] = id[0], T[id[1]].neighbor[0] = id[2];
+		T[id[2]].neighbor[2] = id[1], T[id[2]].neighbor[1] = id[0];
+		if (src.parent == i) {
+			T[i].parent = -1;
+			return 0;
+		}
+		T[i] = T[id[2]];
+		trinum--;  // FreeTriangle
+		return 1;
+	}
+	// Shade in the V3 build (shade.cpp:435-525, PARALLEL without
+	// DONT_PARTITION): returns the split flag
+	template <bool F> int shade_v3(const Source &s, int i, int logdst, long long *c)
+	{
+		Rec &d = T[i];
+		float ff[3];
+		for (int v = 0; v < 3; v++) ff[v] = point_ff<F>(s, d.n, logdst, d.p[v], c);
+		if (d.area < o.adapt_area) {
+			receive(s.deltaB, d, ff);
+			return 0;
+		}
+		if (full_pool(4) || (double)d.area < 1e-8) {
+			receive(s.deltaB, d, ff);
+			return 0;
+		}
+		float groud = (float)((double)(ff[0] + ff[1] + ff[2]) / 3.0);
+		float ffs = point_ff<F>(s, d.n, logdst, d.c, c);
+		float dff = ffabs(groud - ffs);
+		if (dff < o.delta_ff) {
+			const int samples = (int)(d.area / o.sample_area);
+			for (int k = 0; k < samples; k++) {
+				float q[3];
+				groud = random_point<F>(d, q, ff);
+				ffs = point_ff<F>(s, d.n, logdst, q, c);
+				dff = ffabs(groud - ffs);
+				if (dff > o.delta_ff) break;
+			}
+			if (dff < o.delta_ff) {
+				receive(s.deltaB, d, ff);
+				return 0;
+			}
+		}
+		const int r = split_center<F>(i);
+		if (r == 1) {  // slot i holds the third part: its centroid corner shaded anew
+			Rec &t3 = T[i];
+			const float ff2[3] = {ff[0], ff[1], point_ff<F>(s, t3.n, logdst, t3.p[2], c)};
+			receive(s.deltaB, t3, ff2);
+		}
+		if (r < 0) receive(s.deltaB, d, ff);
+		return r;
+	}
+	// parallelRadiosity in the V3 build, run in receiver order (its -jobs 1
+	// order; more threads race on the slot pool): new slots made by splits
+	// are shaded in further passes over the slots they occupy
+	// (rad.cpp:406-451).
+	template <bool F> int shoot_once_v3()
+	{
+		std::vector<int> C = candidates();
+		if (C.empty()) return 0;
+		std::vector<Source> src;
+		for (int s : C) src.push_back(make_source<F>(s));
+		std::set<int> chosen(C.begin(), C.end());
+		long long c[RS_CNT_N] = {0};
+		int prev = 0;
+		for (;;) {
+			const int n = trinum;
+			for (int i = prev; i < n; i++) {
+				if (T[i].parent < 0 || chosen.count(i) || saturated(T[i])) continue;
+				const int logdst = T[i].parent;
+				for (const Source &s : src) {
+					if (s.phys == i) continue;
+					shade_v3<F>(s, i, logdst, c);
+					c[RS_CNT_SHADES]++;
+				}
+			}
+			prev = n;
+			if (n == trinum) break;
+		}
+		for (int k = 0; k < RS_CNT_N; k++) cnt.v[k] += c[k];
+		for (int s : C) T[s].deltaB[0] = T[s].deltaB[1] = T[s].deltaB[2] = 0.0f;
+		cnt.v[RS_CNT_ITERS]++;
+		return 1;
+	}
+
 	// CalRadiosity (shade.cpp:244-259)
 	static void receive(const float *src_dB, Rec &d, const float ff[3])
 	{
@@ -1009,7 +1201,11 @@ struct rs_ctx {
 		cnt.v[RS_CNT_ITERS]++;
 		return 1;
 	}
-	int shoot() { return o.fma ? shoot_once<true>() : shoot_once<false>(); }
+	int shoot()
+	{
+		if (o.v3) return o.fma ? shoot_once_v3<true>() : shoot_once_v3<false>();
+		return o.fma ? shoot_once<true>() : shoot_once<false>();
+	}
 
 	// ---------------- output (rad.cpp:117-242) ----------------
 	const char *fmt(float x)
@@ -1111,6 +1307,7 @@ bool parse_opts(Options &o, int argc, const char **argv, std::string &err)
 		OPT_I("-quiet", quiet)
 		else if (a == "-select") { if (!(v = need("-select"))) return false; o.lowest = std::string(v) == "low"; }
 		else if (a == "-neighbors") { if (!(v = need("-neighbors"))) return false; o.hash = std::string(v) != "exact"; }
+		else if (a == "-variant") { if (!(v = need("-variant"))) return false; o.v3 = std::string(v) == "v3"; }
 		else { err = "unknown option " + a; return false; }
 #undef OPT_F
 #undef OPT_I
@@ -1166,6 +1363,12 @@ int rs_tree_info(rs_ctx *c, int *out3, float *mgl)
 	out3[1] = (int)c->lists.size();
 	out3[2] = c->grid_num;
 	*mgl = c->min_grid;
+	return 0;
+}
+int rs_rand(int n, int *out)
+{
+	GlibcRand r;
+	for (int i = 0; i < n; i++) out[i] = r.next();
 	return 0;
 }
 int rs_descend(rs_ctx *c, int gx, int gy, int gz)

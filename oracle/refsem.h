@@ -67,6 +67,7 @@ int rs_dump(rs_ctx *, void *records, float *g);
 int rs_tree_info(rs_ctx *, int *out3, float *min_grid_len);
 /* Leaf lookup for the engine's flat table check: leaf node index for
  * integer cell coordinates (gx, gy, gz), by the reference descent. */
+int rs_rand(int n, int *out);  /* glibc rand()'s first n outputs (default seed) */
 int rs_descend(rs_ctx *, int gx, int gy, int gz);
 /* Octree arrays: nodes as {g0[3], g1[3], sub[8], list} (15 x 4 bytes). */
 int rs_tree_arrays(rs_ctx *, void *nodes, int *lists);

@@ -14,7 +14,7 @@ int main(int argc, char **argv)
 {
 	static const char *takes_value[] = {"-debug", "-adapt_area", "-sample_area", "-converge", "-delta_ff",
 		"-write_cycle", "-triangle", "-light", "-o", "-jobs", "-parallel_src", "-fma", "-select",
-		"-neighbors", "-skip_discarded", "-max_tri", "-old_format", "-quiet", nullptr};
+		"-neighbors", "-skip_discarded", "-max_tri", "-old_format", "-quiet", "-variant", nullptr};
 	static const char *flags[] = {"-zip", "-interactive", nullptr};
 	std::vector<const char *> opts;
 	const char *input = nullptr, *prefix = "./test";

@@ -36,4 +36,7 @@ def inputs(tmp_path_factory):
     c1 = str(d / "c1_seed1.tri")
     scenes.write_tri(c1, scenes.room_scene(1))
     out["c1_seed1.tri"] = c1
+    v3 = str(d / "v3room.tri")
+    scenes.write_tri(v3, scenes.v3_room())
+    out["v3room.tri"] = v3
     return out

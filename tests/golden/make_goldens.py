@@ -19,6 +19,11 @@ the reference's OLD_FORMAT switch defined) to goldens.json.
 
 adds the md5s of the reference's WebGL model converter (oracle/_ref/tri2json)
 run on the reference's own .fin outputs and on an empty .fin.
+
+    python tests/golden/make_goldens.py --v3
+
+adds the final states of the reference's V3 build (oracle/_ref/refdump_v3:
+its config switches with Delta-ff splitting live) at -jobs 1.
 """
 from __future__ import annotations
 
@@ -242,6 +247,58 @@ def viewer_json():
     shutil.rmtree(tmp)
 
 
+# The reference's V3 build (Delta-ff splitting live, oracle/_ref/refdump_v3)
+# at -jobs 1, the thread count at which its splitting is deterministic.  On
+# box and room pre-partition fills the pool, so V3 cannot split there and
+# must equal V0; the coarse room (scenes.v3_room) leaves room to split.
+V3_CASES = [
+    ("v3room_a05", "v3room.tri", ["-adapt_area", "0.5", "-triangle", "50000"]),
+    ("v3room_a05_d1e3_s005", "v3room.tri", ["-adapt_area", "0.5", "-delta_ff", "1e-3", "-sample_area", "0.05",
+                                            "-triangle", "50000"]),
+    ("v3room_a05_l05", "v3room.tri", ["-adapt_area", "0.5", "-light", "0.5", "-triangle", "50000"]),
+    ("v3room_a05_d1e4_s01_c300", "v3room.tri", ["-adapt_area", "0.5", "-delta_ff", "1e-4", "-sample_area", "0.1",
+                                                "-converge", "300", "-triangle", "50000"]),
+    ("v3_box_t2000", "box.tri", ["-adapt_area", "0.5", "-triangle", "2000"]),
+    ("v3_room_t5000", "room.tri", ["-adapt_area", "0.5", "-triangle", "5000"]),
+]
+
+
+def v3():
+    """Final TriStore hashes, .fin md5s and iteration counts of the V3 build."""
+    path = os.path.join(HERE, "goldens.json")
+    with open(path) as fh:
+        out = json.load(fh)
+    tmp = tempfile.mkdtemp(prefix="goldens_")
+    v3room = os.path.join(tmp, "v3room.tri")
+    scenes.write_tri(v3room, scenes.v3_room())
+    out["generator"]["v3room.tri"] = hashlib.sha256(open(v3room, "rb").read()).hexdigest()
+    inputs = {"v3room.tri": v3room, "room.tri": os.path.join(REF_MODELS, "room.tri"),
+              "box.tri": os.path.join(REF_MODELS, "box.tri")}
+    exe = os.path.join(ROOT, "oracle", "_ref", "refdump_v3")
+    res = {}
+    for case, model, opts in V3_CASES:
+        prefix = os.path.join(tmp, case)
+        dump = os.path.join(tmp, "dump.bin")
+        args = opts + ["-jobs", "1", "-write_cycle", "1000000"]
+        r = subprocess.run([exe, "final", dump] + args + [inputs[model], "-o", prefix], capture_output=True,
+                           text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr[-2000:])
+        rec, g = ol.read_dump(dump)
+        fin = open(prefix + ".fin", "rb").read()
+        res[case] = {"input": model, "opts": args, "iterations": int(re.search(r"Complete in #Iterations (\d+)",
+                                                                             r.stderr).group(1)),
+                     "fin_md5": hashlib.md5(fin).hexdigest(), "fin_records": fin.count(b"Triangle"),
+                     "final": {"trinum": int(len(rec)), "sha256": sha(rec, g)}}
+        np.savez_compressed(os.path.join(HERE, "final_%s.npz" % case), parent=rec["parent"], accB=rec["accB"],
+                            deltaB=rec["deltaB"])
+        print("v3", case, res[case], flush=True)
+    out["v3"] = res
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+    shutil.rmtree(tmp)
+
+
 def model_path(inputs, model):
     return inputs[model]
 
@@ -251,5 +308,7 @@ if __name__ == "__main__":
         old_format()
     elif "--viewer-json" in sys.argv:
         viewer_json()
+    elif "--v3" in sys.argv:
+        v3()
     else:
         main()

@@ -179,6 +179,13 @@ class Oracle:
         return out
 
 
+def rand_outputs(n):
+    """glibc rand()'s first n outputs as the oracle restates them."""
+    out = np.zeros(n, np.int32)
+    lib().rs_rand(n, _ptr(out))
+    return out
+
+
 def read_dump(path):
     """Parse a state dump written by oracle/refdump (the reference harness)."""
     raw = np.fromfile(path, np.uint8)

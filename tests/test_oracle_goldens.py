@@ -126,3 +126,46 @@ def test_fma_sites_matter(inputs):
                 break
         its.append(n + 1)
     assert its == [62, 86]
+
+
+def v3_keys():
+    import json
+    with open(os.path.join(GOLDEN, "goldens.json")) as fh:
+        return sorted(json.load(fh)["v3"])
+
+
+@pytest.mark.parametrize("key", v3_keys())
+def test_v3_matches_reference(key, goldens, inputs, tmp_path):
+    """The reference's V3 build (Delta-ff splitting live: config switches
+    without DONT_PARTITION, oracle/ref.mk refdump_v3) at -jobs 1: iterations,
+    the whole final TriStore (split topology, parents, neighbours, accB) and
+    the .fin bytes.  On box and room pre-partition fills the pool, so V3
+    equals V0 there."""
+    e = goldens["v3"][key]
+    o = ol.Oracle(*(e["opts"] + ["-variant", "v3", "-quiet", "1"]))
+    o.load_file(inputs[e["input"]])
+    o.build_tree()
+    it = o.run(str(tmp_path / "out"))
+    assert it == e["iterations"]
+    rec, g = o.dump()
+    assert len(rec) == e["final"]["trinum"]
+    assert sha(rec, g) == e["final"]["sha256"]
+    fin = open(tmp_path / "out.fin", "rb").read()
+    assert hashlib.md5(fin).hexdigest() == e["fin_md5"]
+
+
+def test_v3_input_is_the_generated_scene(goldens, inputs):
+    import hashlib as h
+    assert h.sha256(open(inputs["v3room.tri"], "rb").read()).hexdigest() == goldens["generator"]["v3room.tri"]
+
+
+def test_glibc_rand_restatement():
+    """The oracle's rand() restatement (TYPE_3, default seed) against the C
+    library's own rand() in a fresh process, and the first published outputs."""
+    import subprocess
+    import sys
+    code = "import ctypes; l = ctypes.CDLL('libc.so.6'); print(' '.join(str(l.rand()) for _ in range(2000)))"
+    libc = [int(x) for x in subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                                           check=True).stdout.split()]
+    assert libc[:3] == [1804289383, 846930886, 1681692777]
+    assert ol.rand_outputs(2000).tolist() == libc
