@@ -127,6 +127,12 @@ struct Loop {
 	int nsrc;
 	int ncand;  // candidates after the last update: a lower bound on the shots left
 	int nrays;  // rays of the current source (length of the compacted ray list)
+	// V3 splitting: slots in use, an error raised by the commit, and glibc
+	// rand()'s state (34 words, the oldest at rng[rng_k])
+	int trinum;
+	int v3_error;
+	int rng_k;
+	unsigned rng[34];
 	SrcDesc src[kMaxSrc];
 };
 
@@ -1169,6 +1175,363 @@ __global__ __launch_bounds__(kBlock) void k_ff_pairs(DScene S, DRecv R, const in
 	}
 }
 
+// ---- V3: Delta-ff adaptive splitting (shade.cpp:463-525 with PARALLEL) -------
+// The reference's V3 build shades receivers in slot order, and a receiver of
+// area >= -adapt_area whose vertex form factors do not predict the one at its
+// centroid (|mean - centre| >= -delta_ff), or at random points drawn with
+// rand() (GetPointInTriangle, shade.cpp:49-68), is split at its centroid into
+// three (PartitionTriangleWithCenterAndStore, shade.cpp:261-306); the new
+// slots are shaded in further passes (rad.cpp:406-451).  On the device:
+// k_v3_gen computes every receiver's form factors (its vertices and centroid)
+// in parallel; k_v3_commit walks the receivers in order on one wave, keeping
+// the pool count and the rand() state, drawing each receiver's samples 64 at
+// a time across the lanes, and writing the splits; k_v3_apply then runs
+// CalRadiosity (shade.cpp:244-259) for the receivers that keep their form
+// factors.  A split only changes its own slot and new ones, so the form
+// factors of the later receivers of a pass are those the reference computes.
+enum { V3_NONE = 0, V3_APPLY = 1 };
+
+struct V3Dev {
+	Patch *pool;         // the slot records (static fields; the SoA arrays hold accB, deltaaccB, deltaB)
+	float *ff;           // 4 per slot: the vertices' form factors, then the centroid's
+	unsigned char *act;  // per slot: V3_APPLY when CalRadiosity follows
+	int cap;             // slots available (min(MaxTri, -triangle))
+	int n_orig;
+	float adapt_area, sample_area, delta_ff;
+};
+
+// AdaptCalFF(CalFF(..)) at a point of a receiver with normal n (shade.cpp:141-168)
+template <bool F>
+__device__ float adapt_ff(const DScene &S, const SrcDesc &s, float nx, float ny, float nz, float nlen, int logdst,
+                          float px, float py, float pz, Cnt &c)
+{
+	if (!s.split) return point_ff<F>(S, s.whole, s.log, nx, ny, nz, nlen, logdst, px, py, pz, c);
+	if (ff_prefix<F>(s.whole, nx, ny, nz, nlen, px, py, pz) != 0.0f) c.ref_rays++;  // computed and discarded
+	const float f1 = point_ff<F>(S, s.half[0], s.log, nx, ny, nz, nlen, logdst, px, py, pz, c);
+	const float f2 = point_ff<F>(S, s.half[1], s.log, nx, ny, nz, nlen, logdst, px, py, pz, c);
+	return f1 + f2;
+}
+
+__device__ __forceinline__ float ffabs(float x) { return (float)fabs((double)x); }
+
+// Slot r as the reference holds it: static fields from the record, the rest
+// from the SoA arrays
+__device__ void v3_load(const V3Dev &V, const DRecv &R, int r, Patch &t)
+{
+	t = V.pool[r];
+	for (int v = 0; v < 3; v++) {
+		const float4 a = R.acc[3 * r + v], d = R.dacc[3 * r + v];
+		t.accB[v][0] = a.x; t.accB[v][1] = a.y; t.accB[v][2] = a.z;
+		t.deltaaccB[v][0] = d.x; t.deltaaccB[v][1] = d.y; t.deltaaccB[v][2] = d.z;
+	}
+	const float4 b = R.dB[r];
+	t.deltaB[0] = b.x; t.deltaB[1] = b.y; t.deltaB[2] = b.z;
+}
+__device__ void v3_store(const V3Dev &V, const DRecv &R, int r, const Patch &t)
+{
+	V.pool[r] = t;
+	// the receiver arrays are read-only to the shooting kernels; the V3 commit
+	// writes only slots no kernel of its pass reads afterwards
+	for (int v = 0; v < 3; v++) {
+		const_cast<float *>(R.vx)[3 * r + v] = t.p[v][0];
+		const_cast<float *>(R.vy)[3 * r + v] = t.p[v][1];
+		const_cast<float *>(R.vz)[3 * r + v] = t.p[v][2];
+		R.acc[3 * r + v] = make_float4(t.accB[v][0], t.accB[v][1], t.accB[v][2], 0.f);
+		R.dacc[3 * r + v] = make_float4(t.deltaaccB[v][0], t.deltaaccB[v][1], t.deltaaccB[v][2], 0.f);
+	}
+	const_cast<float4 *>(R.nrm)[r] = make_float4(t.n[0], t.n[1], t.n[2], __int_as_float(t.parent));
+	const_cast<float4 *>(R.cA)[r] = make_float4(t.c[0], t.c[1], t.c[2], t.area);
+	const_cast<float4 *>(R.rho)[r] = make_float4((float)t.Frgb[0], (float)t.Frgb[1], (float)t.Frgb[2], 0.f);
+	R.dB[r] = make_float4(t.deltaB[0], t.deltaB[1], t.deltaB[2], 0.f);
+}
+
+// Form factors of receivers [r0, r1): 4 lanes per receiver, the vertices'
+// (shade.cpp:442-450) and, where area >= -adapt_area, the centroid's
+// (shade.cpp:469-470); act = V3_APPLY for each receiver the reference
+// shades (parent >= 0, not the source, not saturated: rad.cpp:419-428).
+template <bool F>
+__global__ __launch_bounds__(kBlock) void k_v3_gen(DScene S, DRecv R, V3Dev V, Loop *L, int r0, int r1, long long *bcnt)
+{
+	Cnt c = {0, 0, 0, 0, 0, 0, 0};
+	const int q = blockIdx.x * kBlock + threadIdx.x;
+	const int r = r0 + q / 4, w = q % 4;
+	if (!L->done && r < r1) {
+		const SrcDesc &s = L->src[0];
+		const float4 nr = R.nrm[r];
+		const int parent = __float_as_int(nr.w);
+		bool sat = true;
+		for (int v = 0; v < 3; v++) {
+			const float4 a = R.acc[3 * r + v];
+			sat = sat && !(a.x < 255.0f || a.y < 255.0f || a.z < 255.0f);
+		}
+		const bool live = parent >= 0 && r != s.phys && !sat;
+		if (w == 0) V.act[r] = live ? V3_APPLY : V3_NONE;
+		float f = 0.0f;
+		const float4 ca = R.cA[r];
+		if (live && (w < 3 || ca.w >= V.adapt_area)) {
+			const float nlen = __builtin_sqrtf(dot3<F>(nr.x, nr.y, nr.z, nr.x, nr.y, nr.z));
+			float px, py, pz;
+			if (w < 3) {
+				px = R.vx[3 * r + w];
+				py = R.vy[3 * r + w];
+				pz = R.vz[3 * r + w];
+			} else {
+				px = ca.x;
+				py = ca.y;
+				pz = ca.z;
+			}
+			f = adapt_ff<F>(S, s, nr.x, nr.y, nr.z, nlen, parent, px, py, pz, c);
+		}
+		V.ff[4 * (size_t)r + w] = f;
+	}
+	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
+	add_block_counters(vals, bcnt);
+}
+
+// glibc rand() (TYPE_3, degree 31, separation 3): the next output of the
+// state st[34] (the oldest word at st[k])
+__device__ __forceinline__ int rand_next(unsigned *st, int &k)
+{
+	const unsigned v = st[(k + 3) % 34] + st[(k + 31) % 34];
+	st[k] = v;
+	k = (k + 1) % 34;
+	return (int)(v >> 1);
+}
+
+template <bool F> __device__ void v3_center(float p[3][3], float *c)
+{
+	for (int k = 0; k < 3; k++) c[k] = (float)((double)(p[0][k] + p[1][k] + p[2][k]) / 3.0);
+}
+// CalArea (vector.h:58-62): n = (p1 - p0) x (p2 - p0), not normalised
+template <bool F> __device__ void v3_area(Patch &t)
+{
+	float a[3], b[3];
+	for (int k = 0; k < 3; k++) {
+		a[k] = t.p[1][k] - t.p[0][k];
+		b[k] = t.p[2][k] - t.p[0][k];
+	}
+	auto det2 = [](float x, float y, float z, float w) { return F ? __builtin_fmaf(x, y, -(z * w)) : x * y - z * w; };
+	t.n[0] = det2(a[1], b[2], b[1], a[2]);
+	t.n[1] = det2(a[2], b[0], b[2], a[0]);
+	t.n[2] = det2(a[0], b[1], b[0], a[1]);
+	t.area = __builtin_sqrtf(dot3<F>(t.n[0], t.n[1], t.n[2], t.n[0], t.n[1], t.n[2])) / 2;
+}
+
+// The receivers [r0, r1) in order, one wave (see above).
+template <bool F>
+__global__ __launch_bounds__(kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V, Loop *L, int r0, int r1, long long *bcnt)
+{
+	__shared__ unsigned st[34], snap[34];
+	__shared__ int outs[3 * kWave];
+	__shared__ int sk, snap_k;
+	const int lane = threadIdx.x;
+	Cnt c = {0, 0, 0, 0, 0, 0, 0};
+	if (L->done || L->v3_error) return;
+	if (lane < 34) st[lane] = L->rng[lane];
+	if (lane == 0) sk = L->rng_k;
+	__syncthreads();
+	const SrcDesc &s = L->src[0];
+	const int limit = V.cap;
+	int trinum = L->trinum;
+	for (int r = r0; r < r1; r++) {
+		if (V.act[r] != V3_APPLY) continue;
+		const float4 ca = R.cA[r];
+		const float area = ca.w;
+		if (area < V.adapt_area) continue;                          // shade.cpp:455-458
+		if (trinum + 4 >= limit || (double)area < 1e-8) continue;   // shade.cpp:463-466
+		const float f0 = V.ff[4 * (size_t)r], f1 = V.ff[4 * (size_t)r + 1], f2 = V.ff[4 * (size_t)r + 2];
+		float groud = (float)((double)(f0 + f1 + f2) / 3.0);
+		float dff = ffabs(groud - V.ff[4 * (size_t)r + 3]);
+		if (dff < V.delta_ff) {
+			// GetPointInTriangle samples until one departs by more than -delta_ff
+			const int samples = (int)(area / V.sample_area);
+			const float4 nr = R.nrm[r];
+			const float nlen = __builtin_sqrtf(dot3<F>(nr.x, nr.y, nr.z, nr.x, nr.y, nr.z));
+			const int logdst = __float_as_int(nr.w);
+			float P[3][3];
+			for (int v = 0; v < 3; v++) {
+				P[v][0] = R.vx[3 * r + v];
+				P[v][1] = R.vy[3 * r + v];
+				P[v][2] = R.vz[3 * r + v];
+			}
+			for (int base = 0; base < samples; base += kWave) {
+				const int m = min(kWave, samples - base);
+				if (lane < 34) snap[lane] = st[lane];
+				__syncthreads();
+				if (lane == 0) {
+					snap_k = sk;
+					int k = sk;
+					for (int j = 0; j < 3 * m; j++) outs[j] = rand_next(st, k);
+					sk = k;
+				}
+				__syncthreads();
+				bool brk = false;
+				float mydff = 0.0f;
+				if (lane < m) {
+					float w1 = (float)outs[3 * lane], w2 = (float)outs[3 * lane + 1], w3 = (float)outs[3 * lane + 2];
+					const float tot = w1 + w2 + w3;
+					w1 /= tot;
+					w2 /= tot;
+					w3 /= tot;
+					const float qx = dot3<F>(w1, w2, w3, P[0][0], P[1][0], P[2][0]);
+					const float qy = dot3<F>(w1, w2, w3, P[0][1], P[1][1], P[2][1]);
+					const float qz = dot3<F>(w1, w2, w3, P[0][2], P[1][2], P[2][2]);
+					const float g = dot3<F>(w1, w2, w3, f0, f1, f2);
+					const float fs = adapt_ff<F>(S, s, nr.x, nr.y, nr.z, nlen, logdst, qx, qy, qz, c);
+					mydff = ffabs(g - fs);
+					brk = mydff > V.delta_ff;
+				}
+				const unsigned long long bal = __ballot(brk);
+				if (bal) {
+					const int j = __ffsll((long long)bal) - 1;
+					dff = __shfl(mydff, j);
+					// the state after the 3 (j + 1) draws the reference made
+					__syncthreads();
+					if (lane < 34) st[lane] = snap[lane];
+					__syncthreads();
+					if (lane == 0) {
+						int k = snap_k;
+						for (int t = 0; t < 3 * (j + 1); t++) (void)rand_next(st, k);
+						sk = k;
+					}
+					__syncthreads();
+					break;
+				}
+				dff = __shfl(mydff, m - 1);
+				__syncthreads();
+			}
+			if (dff < V.delta_ff) continue;  // CalRadiosity with the vertices' form factors
+		}
+		// PartitionTriangleWithCenterAndStore (shade.cpp:261-306)
+		if (trinum + 4 >= limit || (double)area < 1e-8) continue;
+		Patch src;
+		v3_load(V, R, r, src);
+		float maxl = 0.0f, minl = 1e+30f;
+		for (int v = 0; v < 3; v++) {
+			const int w = (v + 1) % 3;
+			const float lx = src.p[w][0] - src.p[v][0], ly = src.p[w][1] - src.p[v][1], lz = src.p[w][2] - src.p[v][2];
+			const float len = dot3<F>(lx, ly, lz, lx, ly, lz);
+			maxl = (maxl < len) ? len : maxl;
+			minl = (len < minl) ? len : minl;
+		}
+		if ((double)(minl / maxl) < 1e-1 || minl < 1) continue;
+		if (src.parent != r && r < V.n_orig) {
+			// the split would rewrite an original the octree lists (a slot
+			// pre-partition overwrote): the rays of the later receivers
+			// would change mid-pass; not supported
+			if (lane == 0) L->v3_error = r + 1;
+			break;
+		}
+		const int id[3] = {trinum, trinum + 1, trinum + 2};
+		Patch t[3];
+		for (int k = 0; k < 3; k++) {
+			t[k] = src;
+			for (int x = 0; x < 3; x++) t[k].p[k][x] = src.c[x];
+			v3_center<F>(t[k].p, t[k].c);
+			v3_area<F>(t[k]);
+		}
+		t[0].neighbor[0] = id[2], t[0].neighbor[2] = id[1];
+		t[1].neighbor[1] = id[0], t[1].neighbor[0] = id[2];
+		t[2].neighbor[2] = id[1], t[2].neighbor[1] = id[0];
+		if (lane == 0) {
+			v3_store(V, R, id[0], t[0]);
+			v3_store(V, R, id[1], t[1]);
+		}
+		if (src.parent == r) {
+			// an original becomes logical; its three parts are shaded next pass
+			if (lane == 0) {
+				v3_store(V, R, id[2], t[2]);
+				src.parent = -1;
+				v3_store(V, R, r, src);
+			}
+			trinum += 3;
+			V.act[r] = V3_NONE;
+		} else {
+			// slot r holds the third part (the last slot is given back), its
+			// centroid corner shaded anew (shade.cpp:508-520)
+			if (lane == 0) v3_store(V, R, r, t[2]);
+			trinum += 2;
+			const Patch &t3 = t[2];
+			const float nlen = __builtin_sqrtf(dot3<F>(t3.n[0], t3.n[1], t3.n[2], t3.n[0], t3.n[1], t3.n[2]));
+			float f3 = 0.0f;
+			if (lane == 0)
+				f3 = adapt_ff<F>(S, s, t3.n[0], t3.n[1], t3.n[2], nlen, src.parent, t3.p[2][0], t3.p[2][1], t3.p[2][2], c);
+			if (lane == 0) V.ff[4 * (size_t)r + 2] = f3;
+		}
+		__syncthreads();
+	}
+	__syncthreads();
+	if (lane < 34) L->rng[lane] = st[lane];
+	if (lane == 0) {
+		L->rng_k = sk;
+		L->trinum = trinum;
+	}
+	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
+	add_block_counters<1>(vals, bcnt);
+}
+
+// CalRadiosity (shade.cpp:244-259) for the receivers [r0, r1) that keep
+// their form factors, one lane each, in the reference's order
+__global__ __launch_bounds__(kBlock) void k_v3_apply(DRecv R, V3Dev V, Loop *L, int r0, int r1)
+{
+	const int r = r0 + blockIdx.x * kBlock + threadIdx.x;
+	if (L->done || L->v3_error || r >= r1 || V.act[r] != V3_APPLY) return;
+	const SrcDesc &s = L->src[0];
+	const float4 rho = R.rho[r];
+	const float kr = (float)(0.5 / 255.0);
+	const float lo[3] = {rho.x * kr * s.dB[0] / 3, rho.y * kr * s.dB[1] / 3, rho.z * kr * s.dB[2] / 3};
+	float4 db = R.dB[r];
+	float dbv[3] = {db.x, db.y, db.z};
+	for (int v = 0; v < 3; v++) {
+		const float ff = V.ff[4 * (size_t)r + v];
+		float4 a = R.acc[3 * r + v];
+		float av[3] = {a.x, a.y, a.z}, dv[3];
+		for (int c = 0; c < 3; c++) {
+			const float d = lo[c] * ff;
+			dbv[c] += d;
+			av[c] += d;
+			dv[c] = d;
+		}
+		R.acc[3 * r + v] = make_float4(av[0], av[1], av[2], 0.f);
+		R.dacc[3 * r + v] = make_float4(dv[0], dv[1], dv[2], 0.f);
+	}
+	R.dB[r] = make_float4(dbv[0], dbv[1], dbv[2], 0.f);
+}
+
+// the best candidate over the slots in use (listCanditate with one source,
+// rad.cpp:387-404: the highest slot on ties)
+__global__ __launch_bounds__(kBlock) void k_v3_select(DRecv R, Loop *L, float converge)
+{
+	__shared__ unsigned long long wmax[kWavesPerBlock];
+	__shared__ int wn[kWavesPerBlock];
+	const int r = blockIdx.x * kBlock + threadIdx.x;
+	unsigned long long key = 0;
+	int n = 0;
+	if (r < L->trinum && __float_as_int(R.nrm[r].w) >= 0) {
+		const float4 db = R.dB[r];
+		const float sum = db.x + db.y + db.z;
+		if (sum > converge) {
+			key = cand_key<false>(sum, r);
+			n = 1;
+		}
+	}
+	key = wave_max(key);
+	n = wave_sum(n);
+	if ((threadIdx.x & 63) == 0) {
+		wmax[threadIdx.x >> 6] = key;
+		wn[threadIdx.x >> 6] = n;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int w = 1; w < kWavesPerBlock; w++) {
+			key = wmax[w] > key ? wmax[w] : key;
+			n += wn[w];
+		}
+		if (key) atomicMax(&L->key, key);
+		if (n) atomicAdd(&L->ncand, n);
+	}
+}
+
 // ---- InitRadiosity's neighbour search (rad.cpp:295-334) ----------------------
 // The reference compares every corner of triangle i with every corner of
 // every j (O(n^2)); here corners are hashed by their exact coordinates into
@@ -1330,6 +1693,25 @@ int scratch_address(void **out, std::string &err)
 	return 0;
 }
 
+// glibc rand()'s state at the start of a process (srandom(1): the TYPE_3
+// generator of degree 31 and separation 3, its first 310 outputs
+// discarded), as the 34 most recent words with the oldest first.
+void glibc_rand_start(unsigned st[34])
+{
+	unsigned w[344];
+	int x = 1;
+	w[0] = 1;
+	for (int i = 1; i < 31; i++) {
+		const int hi = x / 127773, lo = x % 127773;
+		x = 16807 * lo - 2836 * hi;
+		if (x < 0) x += 2147483647;
+		w[i] = (unsigned)x;
+	}
+	for (int i = 31; i < 34; i++) w[i] = w[i - 31];
+	for (int i = 34; i < 344; i++) w[i] = w[i - 31] + w[i - 3];
+	for (int i = 0; i < 34; i++) st[i] = w[310 + i];
+}
+
 }  // namespace
 
 struct DeviceEngine::Impl {
@@ -1359,6 +1741,11 @@ struct DeviceEngine::Impl {
 	bool fma = true, lowest = false;
 	int parallel_src = 1, batch = 32;
 	float converge = 1200.f;
+	// V3 splitting
+	bool v3 = false;
+	V3Dev V = {};
+	DevBuf<float> v3ff;
+	DevBuf<unsigned char> v3act;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
 	void *scratch = nullptr;  // this device's address of the static scratch (rad_cast_static)
 	// kGraphIters shooting iterations captured once per upload and replayed
@@ -1470,6 +1857,11 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	d.parallel_src = s.o.parallel_src;
 	d.batch = s.o.batch > 0 ? s.o.batch : 32;
 	d.converge = s.o.converge;
+	d.v3 = s.o.v3 != 0;
+	if (d.v3 && (d.lowest || d.parallel_src != 1)) {
+		err = "-variant v3 runs with one source per iteration (-parallel_src 1, -select high)";
+		return -1;
+	}
 	if (!d.lowest && d.parallel_src > kMaxSrc) {
 		err = "-parallel_src above " + std::to_string(kMaxSrc) + " is not supported";
 		return -1;
@@ -1500,6 +1892,9 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 			err = "octree cell table entry out of range";
 			return -1;
 		}
+	// slots the SoA arrays and the record pool hold: V3 splits add slots up
+	// to the pool limit min(MaxTri, -triangle) (shade.cpp:16-30)
+	const int cap = d.v3 ? std::max(n, std::min(s.o.max_tri, s.o.triangle)) : n;
 	std::vector<int> order(n);
 	spatial_order(s.pool.data(), n, s.G0, s.G1, order.data());
 	// cull radii of the originals as the octree sees them, and the bound on
@@ -1509,15 +1904,19 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	hipStream_t st = d.stream;
 	if (d.order.put(order.data(), order.size(), st, err)) return -1;
 	// the slot records as they are, then gathered on the device
-	if (d.pool.put(s.pool.data(), (size_t)n, st, err) || d.items.put(t.items.data(), ni, st, err) ||
+	if (d.pool.alloc((size_t)cap, err) ||
+	    (n && hipMemcpyAsync(d.pool.p, s.pool.data(), (size_t)n * sizeof(Patch), hipMemcpyHostToDevice, st) != hipSuccess) ||
+	    d.items.put(t.items.data(), ni, st, err) ||
 	    d.radius.put(rad.data(), rad.size(), st, err) ||
 	    d.lnc.alloc(ni * 2, err) || d.lse.alloc(ni * 3, err) ||
 	    d.cell_leaf.put(t.cell_leaf.data(), t.cell_leaf.size(), st, err) ||
 	    d.leaf_box.put((const float4 *)t.leaf_box.data(), t.leaf_box.size() / 4, st, err) ||
-	    d.vx.alloc((size_t)n * 3, err) || d.vy.alloc((size_t)n * 3, err) || d.vz.alloc((size_t)n * 3, err) ||
-	    d.nrm.alloc(n, err) || d.cA.alloc(n, err) || d.rho.alloc(n, err) || d.acc.alloc((size_t)n * 3, err) ||
-	    d.dacc.alloc((size_t)n * 3, err) || d.dB.alloc(n, err))
+	    d.vx.alloc((size_t)cap * 3, err) || d.vy.alloc((size_t)cap * 3, err) || d.vz.alloc((size_t)cap * 3, err) ||
+	    d.nrm.alloc(cap, err) || d.cA.alloc(cap, err) || d.rho.alloc(cap, err) || d.acc.alloc((size_t)cap * 3, err) ||
+	    d.dacc.alloc((size_t)cap * 3, err) || d.dB.alloc(cap, err)) {
+		if (err.empty()) err = "upload of the slot records failed";
 		return -1;
+	}
 	if (ni > 0)
 		hipLaunchKernelGGL(k_pack_items, dim3((unsigned)((ni + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d.pool.p,
 		                   d.items.p, d.radius.p, (int)ni, d.lnc.p, d.lse.p);
@@ -1531,13 +1930,21 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	d.trace_nblocks = (int)(((size_t)n * 6 + kTraceBlock - 1) / kTraceBlock);
 	const int grid = ((d.nblocks + 7) / 8) * 8;  // a whole number of blocks per XCD
 	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
-	std::vector<long long> zeros((size_t)std::max(grid, tgrid) * 8, 0);
+	const int v3grid = d.v3 ? (int)(((size_t)cap * 4 + kBlock - 1) / kBlock) : 0;
+	std::vector<long long> zeros((size_t)std::max(std::max(grid, tgrid), v3grid) * 8, 0);
 	if (d.bcnt.put(zeros.data(), zeros.size(), st, err)) return -1;
 	if (!d.lowest && d.parallel_src > 1 && d.cand.alloc((size_t)n + 1, err)) return -1;
 	if (d.live.alloc((size_t)n + 1, err) || d.ffbuf.alloc((size_t)n * 6 + 2, err) ||
 	    d.raycnt.alloc((size_t)d.nblocks + 1, err) || d.rays.alloc((size_t)n * 6 + 1, err))
 		return -1;
 	std::memset(d.host_loop, 0, sizeof(Loop));
+	d.host_loop->trinum = n;
+	glibc_rand_start(d.host_loop->rng);  // rand() is first called in the solve (shade.cpp:53)
+	if (d.v3) {
+		if (d.v3ff.alloc((size_t)cap * 4 + 4, err) || d.v3act.alloc((size_t)cap + 1, err)) return -1;
+		d.V = V3Dev{d.pool.p, d.v3ff.p, d.v3act.p, std::min(s.o.max_tri, s.o.triangle), n0, s.o.adapt_area,
+		            s.o.sample_area, s.o.delta_ff};
+	}
 	// -parallel_src 0: listCanditate returns no source (rad.cpp:400), so the
 	// first iteration finds nothing to shoot
 	if (!d.lowest && d.parallel_src <= 0) d.host_loop->done = 1;
@@ -1582,6 +1989,57 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 		return 0;
 	}
 	const int start = d.host_loop->shots;
+	if (d.v3) {
+		// one iteration at a time: the source, then passes over the receivers
+		// until a pass makes no new slot (rad.cpp:414-444), then the next source
+		float ms_total = 0.f;
+		for (int it = 0; it < max_shots; it++) {
+			HIP_OK(hipEventRecord(d.ev0, st), "event");
+			if (d.fma) hipLaunchKernelGGL((k_prep<true, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+			else hipLaunchKernelGGL((k_prep<false, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+			HIP_OK(hipMemcpyAsync(d.host_loop, d.loop.p, 64, hipMemcpyDeviceToHost, st), "loop read");
+			HIP_OK(hipStreamSynchronize(st), "v3 source");
+			if (d.host_loop->done) break;
+			int prev = 0, n = d.host_loop->trinum;
+			while (prev < n) {
+				const unsigned g = (unsigned)(((size_t)(n - prev) * 4 + kBlock - 1) / kBlock);
+				const unsigned ga = (unsigned)((n - prev + kBlock - 1) / kBlock);
+				if (d.fma) {
+					hipLaunchKernelGGL(k_v3_gen<true>, dim3(g), dim3(kBlock), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
+					hipLaunchKernelGGL(k_v3_commit<true>, dim3(1), dim3(kWave), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
+				} else {
+					hipLaunchKernelGGL(k_v3_gen<false>, dim3(g), dim3(kBlock), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
+					hipLaunchKernelGGL(k_v3_commit<false>, dim3(1), dim3(kWave), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
+				}
+				hipLaunchKernelGGL(k_v3_apply, dim3(ga), dim3(kBlock), 0, st, d.R, d.V, d.loop.p, prev, n);
+				HIP_OK(hipGetLastError(), "v3 launch");
+				HIP_OK(hipMemcpyAsync(d.host_loop, d.loop.p, 64, hipMemcpyDeviceToHost, st), "loop read");
+				HIP_OK(hipStreamSynchronize(st), "v3 pass");
+				if (d.host_loop->v3_error) {
+					err = "-variant v3: the split of slot " + std::to_string(d.host_loop->v3_error - 1) +
+					      " would rewrite an original the octree lists (pre-partition overwrote it); not supported";
+					return -1;
+				}
+				prev = n;
+				n = d.host_loop->trinum;
+			}
+			const unsigned gs = (unsigned)((n + kBlock - 1) / kBlock);
+			hipLaunchKernelGGL(k_v3_select, dim3(gs > 0 ? gs : 1), dim3(kBlock), 0, st, d.R, d.loop.p, d.converge);
+			HIP_OK(hipGetLastError(), "v3 select");
+			HIP_OK(hipEventRecord(d.ev1, st), "event");
+			HIP_OK(hipEventSynchronize(d.ev1), "v3 iteration");
+			float ms = 0;
+			(void)hipEventElapsedTime(&ms, d.ev0, d.ev1);
+			ms_total += ms;
+			kernel_launches_++;
+		}
+		HIP_OK(hipMemcpyAsync(d.host_loop, d.loop.p, 64, hipMemcpyDeviceToHost, st), "loop read");
+		HIP_OK(hipStreamSynchronize(st), "v3 shoot");
+		shoot_ms_ += ms_total;
+		*shots = d.host_loop->shots - start;
+		*converged = d.host_loop->done;
+		return 0;
+	}
 	const int grid = ((d.nblocks + 7) / 8) * 8;
 	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
 	if (d.bcnt.n < (size_t)std::max(grid, tgrid) * 8 || d.ffbuf.n < (size_t)d.R.nslots * 6 || d.live.n < (size_t)d.R.nslots ||
@@ -1690,6 +2148,19 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 int DeviceEngine::download(HostScene &s, std::string &err)
 {
 	Impl &d = *d_;
+	if (d.v3) {
+		// the slots V3 splits made or rewrote: the records, then the counts
+		HIP_OK(hipMemcpyAsync(d.host_loop, d.loop.p, 64, hipMemcpyDeviceToHost, d.stream), "loop read");
+		HIP_OK(hipStreamSynchronize(d.stream), "download");
+		const int tn = d.host_loop->trinum;
+		if (tn < 0 || (size_t)tn > s.pool.size()) {
+			err = "V3 slot count beyond the host pool";
+			return -1;
+		}
+		if (tn) HIP_OK(hipMemcpy(s.pool.data(), d.pool.p, (size_t)tn * sizeof(Patch), hipMemcpyDeviceToHost), "download");
+		s.trinum = tn;
+		d.R.nslots = tn;
+	}
 	const int n = d.R.nslots;
 	std::vector<float4> acc((size_t)n * 3), dacc((size_t)n * 3), dB(n);
 	hipStream_t st = d.stream;

@@ -56,6 +56,13 @@ bool parse_option(Options &o, int argc, const char *const *argv, int &i, std::st
 	if (a == "-quiet") return as_int(o.quiet);
 	if (a == "-batch") return as_int(o.batch);
 	if (a == "-kernel_timing") return as_int(o.kernel_timing);
+	if (a == "-variant") {
+		if (!value(v)) return false;
+		if (!std::strcmp(v, "v3")) o.v3 = 1;
+		else if (!std::strcmp(v, "v0")) o.v3 = 0;
+		else { err = std::string("bad value for -variant: ") + v; return false; }
+		return true;
+	}
 	if (a == "-octree") {
 		if (!value(v)) return false;
 		if (!std::strcmp(v, "host")) o.octree_host = 1;

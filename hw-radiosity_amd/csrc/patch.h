@@ -63,6 +63,7 @@ struct Options {
 	int kernel_timing = 0;  // time every shoot kernel with HIP events
 	int neighbors_host = 0; // InitRadiosity's neighbour search on the host instead of the device
 	int octree_host = 0;    // BuildTree on the host instead of the device
+	int v3 = 0;             // the reference's V3 build: Delta-ff splitting live (config.h without DONT_PARTITION)
 };
 
 // Arithmetic at the reference compiler's contraction sites.  With F the

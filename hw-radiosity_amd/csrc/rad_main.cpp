@@ -36,7 +36,7 @@ bool takes_value(const char *a)
 {
 	static const char *with[] = {"-debug", "-adapt_area", "-sample_area", "-converge", "-delta_ff", "-write_cycle",
 		"-triangle", "-light", "-o", "-jobs", "-parallel_src", "-fma", "-select", "-max_tri", "-old_format",
-		"-quiet", "-batch", "-kernel_timing", "-neighbors", "-octree", nullptr};
+		"-quiet", "-batch", "-kernel_timing", "-neighbors", "-octree", "-variant", nullptr};
 	for (int k = 0; with[k]; k++)
 		if (!std::strcmp(a, with[k])) return true;
 	return false;

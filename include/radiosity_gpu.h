@@ -55,6 +55,11 @@ enum {
  *   -quiet 0|1        no log lines
  *   -neighbors device|host  where InitRadiosity's neighbour search runs (default device)
  *   -octree device|host     where BuildTree runs (default device)
+ *   -variant v0|v3    v0: the shipped build (default); v3: the reference's
+ *                     V3 build, Delta-ff adaptive splitting live
+ *                     (shade.cpp:463-525 with PARALLEL, config.h), in the
+ *                     receiver order of its -jobs 1 run; one source per
+ *                     iteration (-parallel_src 1)
  * Replaces ProcessOption (rad.cpp:525-631) for everything but the input name. */
 rad_status rad_create(rad_ctx **out, int argc, const char *const *argv);
 void rad_destroy(rad_ctx *ctx);
