@@ -1,5 +1,6 @@
 // C ABI of the engine (include/radiosity_gpu.h).
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -115,12 +116,16 @@ rad_status after_load(rad_ctx *c)
 {
 	HostScene &s = c->scene;
 	c->loaded = c->prepared = c->uploaded = c->device_newer = false;
+	for (int k = 0; k < 3; k++)
+		if (!std::isfinite(s.G0[k]) || !std::isfinite(s.G1[k]))  // the reference's grid is undefined then
+			return fail(c, RAD_EINVAL, "scene coordinates are not finite: the octree grid is undefined");
 	if (!s.o.octree_host && need_device(c) != RAD_OK) return RAD_EDEVICE;
 	const auto t0 = Clock::now();
 	if (s.o.octree_host)
 		c->tree = build_octree(s.pool.data(), s.n_orig, s.G0, s.G1, s.o.fma != 0, s.o.jobs > 0 ? std::max(s.o.jobs, 8) : 0);
 	else if (c->dev->build_octree(s, c->tree, c->err))
 		return RAD_EDEVICE;
+	if (c->tree.gt <= 0) return fail(c, RAD_EINVAL, "octree grid too large to tabulate");
 	s.log("[INFO] 3D-DDA #TreeNode = %d, #Trilist = %d\n", c->tree.n_nodes, c->tree.n_list);
 	c->ms_setup += ms_since(t0);
 	c->loaded = true;

@@ -235,7 +235,10 @@ FlatOctree build_octree(const Patch *pool, int n, const float *G0, const float *
 	T.n_leaves = (int)st.item_begin.size();
 	T.n_list = (int)st.items.size() + T.n_leaves;
 	T.min_grid = st.min_grid;
-	T.grid_num = (int)((G1[0] - G0[0]) / st.min_grid);
+	// GridNum; bounds that are not finite leave it -1 (no float-to-int cast
+	// of a value out of range)
+	const float grid_q = (G1[0] - G0[0]) / st.min_grid;
+	T.grid_num = grid_q >= 0.0f && grid_q < 1023.0f ? (int)grid_q : -1;
 	// the nodes are already in depth-first preorder (children stitched in
 	// order after their parent), so the reference's arrays follow directly
 	T.ref_nodes.resize(st.nodes.size());
@@ -265,7 +268,12 @@ FlatOctree build_octree(const Patch *pool, int n, const float *G0, const float *
 		o[4] = b.g1[0]; o[5] = b.g1[1]; o[6] = b.g1[2];
 		std::memcpy(&o[7], &cd, 4);
 	}
-	// tabulate TreeNodeNum's descent (raycast.cpp:262-280)
+	// tabulate TreeNodeNum's descent (raycast.cpp:262-280); a grid the
+	// table cannot hold (bounds that are not finite) is left empty, gt = 0
+	if (T.grid_num < 0) {
+		T.gt = 0;
+		return T;
+	}
 	T.gt = T.grid_num + 2;
 	const int gt = T.gt;
 	T.cell_leaf.assign((size_t)gt * gt * gt, -1);

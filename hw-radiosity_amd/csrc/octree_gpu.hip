@@ -484,12 +484,13 @@ int build_octree_gpu(void *stream, const Patch *pool, int n, const float *G0, co
 	T.n_leaves = n_leaves;
 	T.n_list = n_items + n_leaves;
 	T.min_grid = min_grid;
-	T.grid_num = (int)((G1[0] - G0[0]) / min_grid);
-	T.gt = T.grid_num + 2;
-	if (T.gt <= 0 || (double)T.gt * T.gt * T.gt > (double)(1u << 30)) {
-		err = "octree cell table too large";
+	const float grid_q = (G1[0] - G0[0]) / min_grid;  // GridNum, range-checked before the cast
+	if (!(grid_q >= 0.0f && grid_q < 1023.0f)) {
+		err = "octree grid of " + std::to_string(grid_q) + " cells per axis: too large to tabulate";
 		return -1;
 	}
+	T.grid_num = (int)grid_q;
+	T.gt = T.grid_num + 2;
 	const size_t cells = (size_t)T.gt * T.gt * T.gt;
 	Dev<int> d_cells;
 	if (d_cells.alloc(cells, err)) return -1;

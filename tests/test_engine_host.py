@@ -369,3 +369,17 @@ def test_prepartition_at_scale_matches_oracle():
     orec, og = o.dump()
     assert len(rec) == len(orec) == 443890
     assert rec.tobytes() == orec.tobytes()
+
+
+@pytest.mark.parametrize("octree", ["host", "device"])
+def test_nonfinite_coordinates_refused(tmp_path, octree):
+    """A .tri whose vertices are not finite has no octree grid (GridNum is
+    (G1[0]-G0[0])/MinGrid, raycast.cpp): rad_load_tri refuses it with
+    RAD_EINVAL before any octree is built, on either builder."""
+    p = tmp_path / "nonfinite.tri"
+    p.write_text("Triangle\n1 2 3 4 5 6\nnan 0 0\n1 inf 0\n0 1 1e39\n"
+                 "Triangle\n1 1 1 0 0 0\n0 0 0\n1 0 0\n0 1 0\n")
+    with radgpu.Engine("-quiet", "1", "-octree", octree) as eng:
+        with pytest.raises(radgpu.RadError) as ei:
+            eng.load_tri(str(p))
+        assert ei.value.status == radgpu.RAD_EINVAL and "not finite" in str(ei.value)
