@@ -488,6 +488,9 @@ def main():
     ap.add_argument("--no-linear", action="store_true", help="skip the explicit-matrix leg")
     ap.add_argument("--no-e2e", action="store_true",
                     help="skip every run but the timed steps (so a kernel trace holds their launches only)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="the timed steps launch every kernel from the host (-graph 0) instead of replaying "
+                         "captured graphs: the same kernels, for profilers that do not follow graph launches")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -524,7 +527,7 @@ def main():
         log("end-to-end solve (init .. .fin): %.3f s, %d iterations" % (e2e_s, iters))
 
     # the timed steps: no per-kernel events, no host wait inside a batch
-    eng = radgpu.Engine(*(OPTS + ["-quiet", "1"]))
+    eng = radgpu.Engine(*(OPTS + ["-quiet", "1"] + (["-graph", "0"] if args.no_graph else [])))
     eng.set_scene(scene).prepare_host()
     rec, _ = eng.records()
     slots = len(rec)

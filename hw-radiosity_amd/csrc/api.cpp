@@ -263,6 +263,7 @@ rad_status rad_upload(rad_ctx *c)
 	}
 	const auto t0 = Clock::now();
 	c->dev->timing = c->scene.o.kernel_timing != 0;
+	c->dev->graphs = c->scene.o.graph != 0;
 	if (c->dev->upload(c->scene, c->tree, c->err)) return RAD_EDEVICE;
 	c->ms_upload = ms_since(t0);
 	c->uploaded = true;

@@ -2085,7 +2085,8 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 	// captured graph (the kernels exit at once after convergence, and none
 	// of the launches is a memory set), so the loop is not bound by the
 	// host's launch rate on small scenes.
-	if (!timing && !d.graph_exec) {
+	const bool replay = graphs && !timing;
+	if (replay && !d.graph_exec) {
 		HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
 		for (int i = 0; i < kGraphIters; i++) iteration(-1);
 		hipGraph_t gph = nullptr;
@@ -2111,9 +2112,11 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 		                     : std::max(kMinQueue, (d.host_loop->ncand + per_iter - 1) / per_iter);
 		const int b = std::min(std::min(d.batch, left), max_shots - queued);
 		HIP_OK(hipEventRecord(d.ev0, st), "event");
-		if (!timing) {
+		if (replay) {
 			for (int i = 0; i + kGraphIters <= b; i += kGraphIters) HIP_OK(hipGraphLaunch(d.graph_exec, st), "graph launch");
 			for (int i = 0; i < b % kGraphIters; i++) iteration(-1);
+		} else if (!timing) {
+			for (int i = 0; i < b; i++) iteration(-1);
 		}
 		for (int i = 0; timing && i < b; i++) {
 			iteration(i % Impl::kEv);

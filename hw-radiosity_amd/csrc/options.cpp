@@ -56,6 +56,7 @@ bool parse_option(Options &o, int argc, const char *const *argv, int &i, std::st
 	if (a == "-quiet") return as_int(o.quiet);
 	if (a == "-batch") return as_int(o.batch);
 	if (a == "-kernel_timing") return as_int(o.kernel_timing);
+	if (a == "-graph") return as_int(o.graph);
 	if (a == "-variant") {
 		if (!value(v)) return false;
 		if (!std::strcmp(v, "v3")) o.v3 = 1;

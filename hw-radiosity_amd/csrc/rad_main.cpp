@@ -30,13 +30,13 @@ const char *kUsage =
 	"       -o </<path>/file>       Output prefix (default ./test).\n"
 	"       -interactive            After each output file run ./WebGL_demo/run.sh on it.\n"
 	"   Engine Options\n"
-	"       -fma 0|1  -select high|low  -max_tri N  -old_format 0|1  -batch N  -quiet 0|1\n";
+	"       -fma 0|1  -select high|low  -max_tri N  -old_format 0|1  -batch N  -quiet 0|1  -graph 0|1\n";
 
 bool takes_value(const char *a)
 {
 	static const char *with[] = {"-debug", "-adapt_area", "-sample_area", "-converge", "-delta_ff", "-write_cycle",
 		"-triangle", "-light", "-o", "-jobs", "-parallel_src", "-fma", "-select", "-max_tri", "-old_format",
-		"-quiet", "-batch", "-kernel_timing", "-neighbors", "-octree", "-variant", nullptr};
+		"-quiet", "-batch", "-kernel_timing", "-graph", "-neighbors", "-octree", "-variant", nullptr};
 	for (int k = 0; with[k]; k++)
 		if (!std::strcmp(a, with[k])) return true;
 	return false;

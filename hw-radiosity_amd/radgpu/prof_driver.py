@@ -107,7 +107,9 @@ def main():
         # lit receivers), or with --direct -converge 1200 (emitters only)
         s = scenes.hall_scene(2)
         conv = "1200" if "--direct" in sys.argv else "175"
-        e = radgpu.Engine("-converge", conv, "-triangle", "1000000", "-quiet", "1")
+        # every kernel launched from the host (-graph 0): counter collection
+        # then sees each dispatch of the same kernels the graphs replay
+        e = radgpu.Engine("-converge", conv, "-triangle", "1000000", "-quiet", "1", "-graph", "0")
         e.set_scene(s).prepare_host().upload()
         n, conv = e.shoot()
         print("shots", n, conv, e.counters(), flush=True)

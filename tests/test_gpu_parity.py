@@ -356,9 +356,10 @@ def test_parallel_src_zero_shoots_nothing(inputs, tmp_path):
 @pytest.mark.parametrize("scene", ["c1", "hall_subset"])
 def test_graph_replays_equal_eager_launches(scene):
     """The shooting loop replays captured graphs of 8 iterations between host
-    checks; with -kernel_timing 1 it launches every kernel eagerly instead.
-    Solves in pieces (graph replays, eager tails and host synchronisations
-    interleaved) give the same shots and records as the eager solve."""
+    checks; with -kernel_timing 1 or -graph 0 it launches every kernel eagerly
+    instead.  Solves in pieces (graph replays, eager tails and host
+    synchronisations interleaved) give the same shots and records as the
+    eager solves."""
     if scene == "c1":
         s, opts = scenes.room_scene(1), ["-converge", "100", "-triangle", "20000", "-quiet", "1"]
     else:
@@ -377,6 +378,13 @@ def test_graph_replays_equal_eager_launches(scene):
         assert ca and cb and na == nb and na > 24
         assert a.records()[0].tobytes() == b.records()[0].tobytes()
         assert a.counters() == b.counters()
+        rec_b, cnt_b = b.records()[0].tobytes(), b.counters()
+    with radgpu.Engine(*(opts + ["-graph", "0"])) as c:
+        c.set_scene(s).prepare_host().upload()
+        nc, cc = c.shoot()
+        assert cc and nc == na
+        assert c.records()[0].tobytes() == rec_b
+        assert c.counters() == cnt_b
 
 
 @pytest.mark.parametrize("key", case_keys())
