@@ -223,21 +223,23 @@ __device__ __forceinline__ void test_entry(const DScene &S, const float4 &a, con
                                            float px, float py, float pz, float vx, float vy, float vz, int ban,
                                            float ck, float cb, float &tmin, float &tmin_hi, int &best, Cnt &c)
 {
+	// The shortcuts are evaluated for every lane without branches (most
+	// waves hold lanes on every path, so branches would only add mask
+	// updates); a lane leaves at one branch: banned or rejected.
 	const int id = __float_as_int(b.z);
-	if (id == ban) return;
-	c.tests++;
+	const bool own = id == ban;
+	c.tests += own ? 0 : 1;
 	const float a1 = dot3<F>(vx, vy, vz, a.x, a.y, a.z);
 	const float a2 = dot3<F>(a.w - px, b.x - py, b.y - pz, a.x, a.y, a.z);
 	const float fa1 = fabsf(a1);
-	if (fa1 >= 0x1p-100f && fa1 <= 0x1p100f && fabsf(a2) <= 0x1p100f) {
-		const float ta = a2 * __builtin_amdgcn_rcpf(a1);
-		if (ta < S.thr_t_lo || ta > tmin_hi) return;
-		const float qx = __builtin_fmaf(ta, vx, px) - a.w, qy = __builtin_fmaf(ta, vy, py) - b.x,
-		            qz = __builtin_fmaf(ta, vz, pz) - b.y;
-		const float d2 = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx));
-		const float lim = __builtin_fmaf(b.w, 1.0f + 0x1p-11f, __builtin_fmaf(fabsf(ta), ck, cb));
-		if (d2 > lim * lim) return;
-	}
+	const bool est = fa1 >= 0x1p-100f && fa1 <= 0x1p100f && fabsf(a2) <= 0x1p100f;
+	const float ta = a2 * __builtin_amdgcn_rcpf(a1);
+	const float qx = __builtin_fmaf(ta, vx, px) - a.w, qy = __builtin_fmaf(ta, vy, py) - b.x,
+	            qz = __builtin_fmaf(ta, vz, pz) - b.y;
+	const float d2 = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx));
+	const float lim = __builtin_fmaf(b.w, 1.0f + 0x1p-11f, __builtin_fmaf(fabsf(ta), ck, cb));
+	const bool rej = est && (ta < S.thr_t_lo || ta > tmin_hi || d2 > lim * lim);
+	if (own || rej) return;
 	const float tt = a2 / a1;
 	// TriHitted: t < 0.0001 (in double) or t >= t_min rejects
 	if (tt < S.thr_t || tt >= tmin) return;
@@ -300,24 +302,21 @@ __device__ int walk(const DScene &S, float px, float py, float pz, float vx, flo
 		const float4 *nc = S.lnc + 2 * (size_t)begin;
 		const float4 *se = S.lse + 3 * (size_t)begin;
 		// two entries per trip, the next one loaded while the current one is
-		// tested (no register copies between trips)
-		float4 ea0 = make_float4(0.f, 0.f, 0.f, 0.f), eb0 = ea0, ea1 = ea0, eb1 = ea0;
+		// tested (no register copies between trips); the loads are not
+		// conditional (the index is clamped to the list), so that the wait
+		// before a test covers only the loads that entry needs
 		if (cnt > 0) {
-			ea0 = nc[0];
-			eb0 = nc[1];
-		}
-		for (int e = 0; e < cnt; e += 2) {
-			if (e + 1 < cnt) {
-				ea1 = nc[2 * e + 2];
-				eb1 = nc[2 * e + 3];
+			float4 ea0 = nc[0], eb0 = nc[1];
+			for (int e = 0; e < cnt; e += 2) {
+				const int e1 = min(e + 1, cnt - 1);
+				const float4 ea1 = nc[2 * e1], eb1 = nc[2 * e1 + 1];
+				test_entry<F>(S, ea0, eb0, se + 3 * e, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, c);
+				if (e + 1 >= cnt) break;
+				const int e2 = min(e + 2, cnt - 1);
+				ea0 = nc[2 * e2];
+				eb0 = nc[2 * e2 + 1];
+				test_entry<F>(S, ea1, eb1, se + 3 * e + 3, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, c);
 			}
-			test_entry<F>(S, ea0, eb0, se + 3 * e, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, c);
-			if (e + 1 >= cnt) break;
-			if (e + 2 < cnt) {
-				ea0 = nc[2 * e + 4];
-				eb0 = nc[2 * e + 5];
-			}
-			test_entry<F>(S, ea1, eb1, se + 3 * e + 3, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, c);
 		}
 		if (best >= 0) return best;
 		if (leaf == tort) { c.stalled++; return -2; }
