@@ -117,10 +117,12 @@ def reference_beside(threads):
     oracle's CLI (oracle/_bin/refsem, with the reference's exact O(N^2)
     neighbour pass) and the engine's drop-in `rad` on the reference's own
     models at -triangle 40000 (its published workloads, doc/README.md:404-408)
-    and on the seeded C1 room: the "Radiosity Method took" span of each
-    (InitRadiosity .. final .fin), iterations, and whether the .fin files are
-    byte-identical.  Gives the oracle/reference speed ratio that anchors the
-    CPU baseline, and the drop-in's speed-up over the reference."""
+    and on the seeded C1 room; and the reference's V3 build (oracle/_ref/rad_v3,
+    Delta-ff splitting live, at -jobs 1) beside `rad -variant v3` on the
+    coarse room: the "Radiosity Method took" span of each (InitRadiosity ..
+    final .fin), iterations, and whether the .fin files are byte-identical.
+    Gives the oracle/reference speed ratio that anchors the CPU baseline, and
+    the drop-in's speed-up over the reference."""
     import gzip
     import shutil
     from radgpu import scenes
@@ -142,14 +144,27 @@ def reference_beside(threads):
         cases.append(("c1 (-converge 1200 -delta_ff 5e-6 -adapt_area 5 -triangle 20000)", c1,
                       ["-converge", "1200", "-delta_ff", "5e-6", "-adapt_area", "5", "-triangle", "20000"]))
         cases.append(("c1 at -converge 100", c1, ["-converge", "100", "-triangle", "20000"]))
+        # the V3 build (Delta-ff splitting live) at -jobs 1, its deterministic
+        # order, against -variant v3, on the coarse room where receivers split
+        v3room = os.path.join(tmp, "v3room.tri")
+        scenes.write_tri(v3room, scenes.v3_room())
+        v3_bin = os.path.join(ROOT, "oracle", "_ref", "rad_v3")
+        if os.path.exists(v3_bin):
+            cases.append(("v3 v3room -adapt_area 0.5 -triangle 50000", v3room,
+                          ["-adapt_area", "0.5", "-triangle", "50000"]))
+            cases.append(("v3 v3room -adapt_area 0.5 -delta_ff 1e-4 -sample_area 0.1 -converge 300 -triangle 50000",
+                          v3room, ["-adapt_area", "0.5", "-delta_ff", "1e-4", "-sample_area", "0.1",
+                                   "-converge", "300", "-triangle", "50000"]))
         for name, tri, opts in cases:
             row = {}
-            for who, exe, extra in (("reference", ref_bin, []), ("oracle", sem_bin, ["-neighbors", "exact"]),
-                                    ("engine", gpu_bin, [])):
+            v3 = name.startswith("v3 ")
+            runs = ((("reference", v3_bin, []), ("engine", gpu_bin, ["-variant", "v3"])) if v3 else
+                    (("reference", ref_bin, []), ("oracle", sem_bin, ["-neighbors", "exact"]), ("engine", gpu_bin, [])))
+            for who, exe, extra in runs:
                 if who == "oracle" and name.startswith("c1"):
                     continue
-                row[who] = _run_timed(exe, opts + extra + ["-write_cycle", "1000000", "-jobs", str(threads), tri,
-                                                           "-o", os.path.join(tmp, who)])
+                row[who] = _run_timed(exe, opts + extra + ["-write_cycle", "1000000", "-jobs", "1" if v3 else str(threads),
+                                                           tri, "-o", os.path.join(tmp, who)])
             ref, eng = row["reference"], row["engine"]
             if ref.get("method_s") and eng.get("method_s"):
                 row["speedup_method_span"] = ref["method_s"] / eng["method_s"]

@@ -1332,132 +1332,145 @@ __global__ __launch_bounds__(kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V,
 	const SrcDesc &s = L->src[0];
 	const int limit = V.cap;
 	int trinum = L->trinum;
-	for (int r = r0; r < r1; r++) {
-		if (V.act[r] != V3_APPLY) continue;
-		const float4 ca = R.cA[r];
-		const float area = ca.w;
-		if (area < V.adapt_area) continue;                          // shade.cpp:455-458
-		if (trinum + 4 >= limit || (double)area < 1e-8) continue;   // shade.cpp:463-466
-		const float f0 = V.ff[4 * (size_t)r], f1 = V.ff[4 * (size_t)r + 1], f2 = V.ff[4 * (size_t)r + 2];
-		float groud = (float)((double)(f0 + f1 + f2) / 3.0);
-		float dff = ffabs(groud - V.ff[4 * (size_t)r + 3]);
-		if (dff < V.delta_ff) {
-			// GetPointInTriangle samples until one departs by more than -delta_ff
-			const int samples = (int)(area / V.sample_area);
-			const float4 nr = R.nrm[r];
-			const float nlen = __builtin_sqrtf(dot3<F>(nr.x, nr.y, nr.z, nr.x, nr.y, nr.z));
-			const int logdst = __float_as_int(nr.w);
-			float P[3][3];
-			for (int v = 0; v < 3; v++) {
-				P[v][0] = R.vx[3 * r + v];
-				P[v][1] = R.vy[3 * r + v];
-				P[v][2] = R.vz[3 * r + v];
-			}
-			for (int base = 0; base < samples; base += kWave) {
-				const int m = min(kWave, samples - base);
-				if (lane < 34) snap[lane] = st[lane];
-				__syncthreads();
-				if (lane == 0) {
-					snap_k = sk;
-					int k = sk;
-					for (int j = 0; j < 3 * m; j++) outs[j] = rand_next(st, k);
-					sk = k;
+	bool stop = false;
+	for (int base = r0; base < r1 && !stop; base += kWave) {
+		// Once the pool is full every later receiver keeps its form factors
+		// (the first test below), so nothing is left to do in order.
+		if (trinum + 4 >= limit) break;
+		// This chunk's receivers that need the ordered steps, found in
+		// parallel: live, of area >= -adapt_area (shade.cpp:455-458); the
+		// others keep their vertices' form factors.
+		const int rl = base + lane;
+		const bool cand = rl < r1 && V.act[rl] == V3_APPLY && R.cA[rl].w >= V.adapt_area;
+		unsigned long long todo = __ballot(cand);
+		while (todo) {
+			const int r = base + (__ffsll((long long)todo) - 1);
+			todo &= todo - 1;
+			const float4 ca = R.cA[r];
+			const float area = ca.w;
+			if (trinum + 4 >= limit || (double)area < 1e-8) continue;   // shade.cpp:463-466
+			const float f0 = V.ff[4 * (size_t)r], f1 = V.ff[4 * (size_t)r + 1], f2 = V.ff[4 * (size_t)r + 2];
+			float groud = (float)((double)(f0 + f1 + f2) / 3.0);
+			float dff = ffabs(groud - V.ff[4 * (size_t)r + 3]);
+			if (dff < V.delta_ff) {
+				// GetPointInTriangle samples until one departs by more than -delta_ff
+				const int samples = (int)(area / V.sample_area);
+				const float4 nr = R.nrm[r];
+				const float nlen = __builtin_sqrtf(dot3<F>(nr.x, nr.y, nr.z, nr.x, nr.y, nr.z));
+				const int logdst = __float_as_int(nr.w);
+				float P[3][3];
+				for (int v = 0; v < 3; v++) {
+					P[v][0] = R.vx[3 * r + v];
+					P[v][1] = R.vy[3 * r + v];
+					P[v][2] = R.vz[3 * r + v];
 				}
-				__syncthreads();
-				bool brk = false;
-				float mydff = 0.0f;
-				if (lane < m) {
-					float w1 = (float)outs[3 * lane], w2 = (float)outs[3 * lane + 1], w3 = (float)outs[3 * lane + 2];
-					const float tot = w1 + w2 + w3;
-					w1 /= tot;
-					w2 /= tot;
-					w3 /= tot;
-					const float qx = dot3<F>(w1, w2, w3, P[0][0], P[1][0], P[2][0]);
-					const float qy = dot3<F>(w1, w2, w3, P[0][1], P[1][1], P[2][1]);
-					const float qz = dot3<F>(w1, w2, w3, P[0][2], P[1][2], P[2][2]);
-					const float g = dot3<F>(w1, w2, w3, f0, f1, f2);
-					const float fs = adapt_ff<F>(S, s, nr.x, nr.y, nr.z, nlen, logdst, qx, qy, qz, c);
-					mydff = ffabs(g - fs);
-					brk = mydff > V.delta_ff;
-				}
-				const unsigned long long bal = __ballot(brk);
-				if (bal) {
-					const int j = __ffsll((long long)bal) - 1;
-					dff = __shfl(mydff, j);
-					// the state after the 3 (j + 1) draws the reference made
-					__syncthreads();
-					if (lane < 34) st[lane] = snap[lane];
+				for (int base = 0; base < samples; base += kWave) {
+					const int m = min(kWave, samples - base);
+					if (lane < 34) snap[lane] = st[lane];
 					__syncthreads();
 					if (lane == 0) {
-						int k = snap_k;
-						for (int t = 0; t < 3 * (j + 1); t++) (void)rand_next(st, k);
+						snap_k = sk;
+						int k = sk;
+						for (int j = 0; j < 3 * m; j++) outs[j] = rand_next(st, k);
 						sk = k;
 					}
 					__syncthreads();
-					break;
+					bool brk = false;
+					float mydff = 0.0f;
+					if (lane < m) {
+						float w1 = (float)outs[3 * lane], w2 = (float)outs[3 * lane + 1], w3 = (float)outs[3 * lane + 2];
+						const float tot = w1 + w2 + w3;
+						w1 /= tot;
+						w2 /= tot;
+						w3 /= tot;
+						const float qx = dot3<F>(w1, w2, w3, P[0][0], P[1][0], P[2][0]);
+						const float qy = dot3<F>(w1, w2, w3, P[0][1], P[1][1], P[2][1]);
+						const float qz = dot3<F>(w1, w2, w3, P[0][2], P[1][2], P[2][2]);
+						const float g = dot3<F>(w1, w2, w3, f0, f1, f2);
+						const float fs = adapt_ff<F>(S, s, nr.x, nr.y, nr.z, nlen, logdst, qx, qy, qz, c);
+						mydff = ffabs(g - fs);
+						brk = mydff > V.delta_ff;
+					}
+					const unsigned long long bal = __ballot(brk);
+					if (bal) {
+						const int j = __ffsll((long long)bal) - 1;
+						dff = __shfl(mydff, j);
+						// the state after the 3 (j + 1) draws the reference made
+						__syncthreads();
+						if (lane < 34) st[lane] = snap[lane];
+						__syncthreads();
+						if (lane == 0) {
+							int k = snap_k;
+							for (int t = 0; t < 3 * (j + 1); t++) (void)rand_next(st, k);
+							sk = k;
+						}
+						__syncthreads();
+						break;
+					}
+					dff = __shfl(mydff, m - 1);
+					__syncthreads();
 				}
-				dff = __shfl(mydff, m - 1);
-				__syncthreads();
+				if (dff < V.delta_ff) continue;  // CalRadiosity with the vertices' form factors
 			}
-			if (dff < V.delta_ff) continue;  // CalRadiosity with the vertices' form factors
-		}
-		// PartitionTriangleWithCenterAndStore (shade.cpp:261-306)
-		if (trinum + 4 >= limit || (double)area < 1e-8) continue;
-		Patch src;
-		v3_load(V, R, r, src);
-		float maxl = 0.0f, minl = 1e+30f;
-		for (int v = 0; v < 3; v++) {
-			const int w = (v + 1) % 3;
-			const float lx = src.p[w][0] - src.p[v][0], ly = src.p[w][1] - src.p[v][1], lz = src.p[w][2] - src.p[v][2];
-			const float len = dot3<F>(lx, ly, lz, lx, ly, lz);
-			maxl = (maxl < len) ? len : maxl;
-			minl = (len < minl) ? len : minl;
-		}
-		if ((double)(minl / maxl) < 1e-1 || minl < 1) continue;
-		if (src.parent != r && r < V.n_orig) {
-			// the split would rewrite an original the octree lists (a slot
-			// pre-partition overwrote): the rays of the later receivers
-			// would change mid-pass; not supported
-			if (lane == 0) L->v3_error = r + 1;
-			break;
-		}
-		const int id[3] = {trinum, trinum + 1, trinum + 2};
-		Patch t[3];
-		for (int k = 0; k < 3; k++) {
-			t[k] = src;
-			for (int x = 0; x < 3; x++) t[k].p[k][x] = src.c[x];
-			v3_center<F>(t[k].p, t[k].c);
-			v3_area<F>(t[k]);
-		}
-		t[0].neighbor[0] = id[2], t[0].neighbor[2] = id[1];
-		t[1].neighbor[1] = id[0], t[1].neighbor[0] = id[2];
-		t[2].neighbor[2] = id[1], t[2].neighbor[1] = id[0];
-		if (lane == 0) {
-			v3_store(V, R, id[0], t[0]);
-			v3_store(V, R, id[1], t[1]);
-		}
-		if (src.parent == r) {
-			// an original becomes logical; its three parts are shaded next pass
+			// PartitionTriangleWithCenterAndStore (shade.cpp:261-306)
+			if (trinum + 4 >= limit || (double)area < 1e-8) continue;
+			Patch src;
+			v3_load(V, R, r, src);
+			float maxl = 0.0f, minl = 1e+30f;
+			for (int v = 0; v < 3; v++) {
+				const int w = (v + 1) % 3;
+				const float lx = src.p[w][0] - src.p[v][0], ly = src.p[w][1] - src.p[v][1], lz = src.p[w][2] - src.p[v][2];
+				const float len = dot3<F>(lx, ly, lz, lx, ly, lz);
+				maxl = (maxl < len) ? len : maxl;
+				minl = (len < minl) ? len : minl;
+			}
+			if ((double)(minl / maxl) < 1e-1 || minl < 1) continue;
+			if (src.parent != r && r < V.n_orig) {
+				// the split would rewrite an original the octree lists (a slot
+				// pre-partition overwrote): the rays of the later receivers
+				// would change mid-pass; not supported
+				if (lane == 0) L->v3_error = r + 1;
+				stop = true;
+				break;
+			}
+			const int id[3] = {trinum, trinum + 1, trinum + 2};
+			Patch t[3];
+			for (int k = 0; k < 3; k++) {
+				t[k] = src;
+				for (int x = 0; x < 3; x++) t[k].p[k][x] = src.c[x];
+				v3_center<F>(t[k].p, t[k].c);
+				v3_area<F>(t[k]);
+			}
+			t[0].neighbor[0] = id[2], t[0].neighbor[2] = id[1];
+			t[1].neighbor[1] = id[0], t[1].neighbor[0] = id[2];
+			t[2].neighbor[2] = id[1], t[2].neighbor[1] = id[0];
 			if (lane == 0) {
-				v3_store(V, R, id[2], t[2]);
-				src.parent = -1;
-				v3_store(V, R, r, src);
+				v3_store(V, R, id[0], t[0]);
+				v3_store(V, R, id[1], t[1]);
 			}
-			trinum += 3;
-			V.act[r] = V3_NONE;
-		} else {
-			// slot r holds the third part (the last slot is given back), its
-			// centroid corner shaded anew (shade.cpp:508-520)
-			if (lane == 0) v3_store(V, R, r, t[2]);
-			trinum += 2;
-			const Patch &t3 = t[2];
-			const float nlen = __builtin_sqrtf(dot3<F>(t3.n[0], t3.n[1], t3.n[2], t3.n[0], t3.n[1], t3.n[2]));
-			float f3 = 0.0f;
-			if (lane == 0)
-				f3 = adapt_ff<F>(S, s, t3.n[0], t3.n[1], t3.n[2], nlen, src.parent, t3.p[2][0], t3.p[2][1], t3.p[2][2], c);
-			if (lane == 0) V.ff[4 * (size_t)r + 2] = f3;
+			if (src.parent == r) {
+				// an original becomes logical; its three parts are shaded next pass
+				if (lane == 0) {
+					v3_store(V, R, id[2], t[2]);
+					src.parent = -1;
+					v3_store(V, R, r, src);
+				}
+				trinum += 3;
+				V.act[r] = V3_NONE;
+			} else {
+				// slot r holds the third part (the last slot is given back), its
+				// centroid corner shaded anew (shade.cpp:508-520)
+				if (lane == 0) v3_store(V, R, r, t[2]);
+				trinum += 2;
+				const Patch &t3 = t[2];
+				const float nlen = __builtin_sqrtf(dot3<F>(t3.n[0], t3.n[1], t3.n[2], t3.n[0], t3.n[1], t3.n[2]));
+				float f3 = 0.0f;
+				if (lane == 0)
+					f3 = adapt_ff<F>(S, s, t3.n[0], t3.n[1], t3.n[2], nlen, src.parent, t3.p[2][0], t3.p[2][1], t3.p[2][2], c);
+				if (lane == 0) V.ff[4 * (size_t)r + 2] = f3;
+			}
+			__syncthreads();
 		}
-		__syncthreads();
 	}
 	__syncthreads();
 	if (lane < 34) L->rng[lane] = st[lane];
