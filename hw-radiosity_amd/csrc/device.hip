@@ -65,6 +65,9 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 // k_trace's block: one wave, so that a wave whose rays walk far holds only
 // its own slot (rays' costs are heavy-tailed)
 constexpr int kTraceBlock = 64;
+// lanes the machine runs at once of the k_trace form that shares rays over
+// lanes (256 CUs x 4 SIMDs x 5 waves, its occupancy, x 64)
+constexpr long long kTraceSlots = 256LL * 4 * 5 * 64;
 constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
 constexpr int kGroups = 2;        // receiver groups per block
 constexpr int kMinQueue = 8;      // iterations queued per host check at least (up to -batch)
@@ -218,10 +221,11 @@ __device__ __forceinline__ int cell_coord(float q, float g0, const DScene &S)
 //    the side tests' rounding (at most 2^-21 of those magnitudes, moved by up
 //    to 16x at a corner), the distance of the reference's rounded hit point
 //    from the plane and |q_a - q|, with a factor of 4 to spare.
-template <bool F>
+template <bool F, bool SHARED = false>
 __device__ __forceinline__ void test_entry(const DScene &S, const float4 &a, const float4 &b, const float4 *se,
                                            float px, float py, float pz, float vx, float vy, float vz, int ban,
-                                           float ck, float cb, float &tmin, float &tmin_hi, int &best, Cnt &c)
+                                           float ck, float cb, float &tmin, float &tmin_hi, int &best, int e,
+                                           int &best_e, bool &nan_hit, Cnt &c)
 {
 	// The shortcuts are evaluated for every lane without branches (most
 	// waves hold lanes on every path, so branches would only add mask
@@ -253,6 +257,40 @@ __device__ __forceinline__ void test_entry(const DScene &S, const float4 &a, con
 	tmin = tt;
 	tmin_hi = tt * kGuardUp;
 	best = id;
+	if (SHARED) {  // what the lanes sharing a list need to combine
+		best_e = e;
+		nan_hit |= tt != tt;  // a NaN quotient passes every compare: accepted
+	}
+}
+
+// TriListHitted over entries sub, sub + L, sub + 2 L, ... of a leaf's list
+// (one lane's share when L lanes test the list of one ray; L = 1: the whole
+// list).  Two entries per trip, the next one loaded while the current one
+// is tested (no register copies between trips); the loads are not
+// conditional (the index is clamped to the share), so that the wait before
+// a test covers only the loads that entry needs.
+template <bool F, int L>
+__device__ __forceinline__ void test_list(const DScene &S, int begin, int cnt, int sub, float px, float py, float pz,
+                                          float vx, float vy, float vz, int ban, float ck, float cb, float &tmin,
+                                          float &tmin_hi, int &best, int &best_e, bool &nan_hit, Cnt &c)
+{
+	const int my = cnt > sub ? (cnt - sub + L - 1) / L : 0;
+	if (my <= 0) return;
+	const float4 *nc = S.lnc + 2 * (size_t)begin;
+	const float4 *se = S.lse + 3 * (size_t)begin;
+	float4 ea0 = nc[2 * sub], eb0 = nc[2 * sub + 1];
+	for (int m = 0; m < my; m += 2) {
+		const int e0 = sub + L * m, e1 = sub + L * min(m + 1, my - 1);
+		const float4 ea1 = nc[2 * e1], eb1 = nc[2 * e1 + 1];
+		test_entry<F, (L > 1)>(S, ea0, eb0, se + 3 * e0, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, e0,
+		                       best_e, nan_hit, c);
+		if (m + 1 >= my) break;
+		const int e2 = sub + L * min(m + 2, my - 1);
+		ea0 = nc[2 * e2];
+		eb0 = nc[2 * e2 + 1];
+		test_entry<F, (L > 1)>(S, ea1, eb1, se + 3 * e1, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, e1,
+		                       best_e, nan_hit, c);
+	}
 }
 
 // The walk of raycast.cpp:335-360 over the tabulated octree: find the leaf
@@ -260,8 +298,16 @@ __device__ __forceinline__ void test_entry(const DScene &S, const float4 &a, con
 // raycast.cpp:283-333) and return the nearest hit of the first leaf with any
 // hit; else step q to just past the leaf's nearest exit plane (NextPoint,
 // raycast.cpp:235-251).
-template <bool F>
-__device__ int walk(const DScene &S, float px, float py, float pz, float vx, float vy, float vz, int ban, Cnt &c)
+// With L > 1 the L lanes sub = 0 .. L-1 of an aligned group walk the same
+// ray together: each tests every L-th entry of a leaf's list, and the group
+// takes the nearest hit, the earliest entry on ties, as the sequential scan
+// does; where a lane accepted a NaN quotient (after which the sequential
+// scan accepts whatever follows) the group scans the leaf again in order.
+// The walk's other steps run in every lane of the group alike (so every
+// lane counts them; each counts only its own triangle tests).
+template <bool F, int L = 1>
+__device__ int walk(const DScene &S, float px, float py, float pz, float vx, float vy, float vz, int ban, Cnt &c,
+                    int sub = 0)
 {
 	float t = 0.f, qx = px, qy = py, qz = pz;
 	int prev = -1;
@@ -298,24 +344,55 @@ __device__ int walk(const DScene &S, float px, float py, float pz, float vx, flo
 		c.cells++;
 		c.levels += cd >> 24;
 		float tmin = 1000000.0f, tmin_hi = 1000000.0f * kGuardUp;
-		int best = -1;
-		const float4 *nc = S.lnc + 2 * (size_t)begin;
-		const float4 *se = S.lse + 3 * (size_t)begin;
-		// two entries per trip, the next one loaded while the current one is
-		// tested (no register copies between trips); the loads are not
-		// conditional (the index is clamped to the list), so that the wait
-		// before a test covers only the loads that entry needs
-		if (cnt > 0) {
-			float4 ea0 = nc[0], eb0 = nc[1];
-			for (int e = 0; e < cnt; e += 2) {
-				const int e1 = min(e + 1, cnt - 1);
-				const float4 ea1 = nc[2 * e1], eb1 = nc[2 * e1 + 1];
-				test_entry<F>(S, ea0, eb0, se + 3 * e, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, c);
-				if (e + 1 >= cnt) break;
-				const int e2 = min(e + 2, cnt - 1);
-				ea0 = nc[2 * e2];
-				eb0 = nc[2 * e2 + 1];
-				test_entry<F>(S, ea1, eb1, se + 3 * e + 3, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, c);
+		int best = -1, best_e = 0;
+		bool nan_hit = false;
+		if (L == 1) {
+			// two entries per trip, the next one loaded while the current one
+			// is tested (no register copies between trips); the loads are not
+			// conditional (the index is clamped to the list), so that the wait
+			// before a test covers only the loads that entry needs
+			const float4 *nc = S.lnc + 2 * (size_t)begin;
+			const float4 *se = S.lse + 3 * (size_t)begin;
+			if (cnt > 0) {
+				float4 ea0 = nc[0], eb0 = nc[1];
+				for (int e = 0; e < cnt; e += 2) {
+					const int e1 = min(e + 1, cnt - 1);
+					const float4 ea1 = nc[2 * e1], eb1 = nc[2 * e1 + 1];
+					test_entry<F>(S, ea0, eb0, se + 3 * e, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, e,
+					              best_e, nan_hit, c);
+					if (e + 1 >= cnt) break;
+					const int e2 = min(e + 2, cnt - 1);
+					ea0 = nc[2 * e2];
+					eb0 = nc[2 * e2 + 1];
+					test_entry<F>(S, ea1, eb1, se + 3 * e + 3, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best,
+					              e + 1, best_e, nan_hit, c);
+				}
+			}
+		} else {
+			test_list<F, L>(S, begin, cnt, sub, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, best_e, nan_hit,
+			                c);
+			// butterfly over the group: after the step with offset o each
+			// lane holds the combination of its aligned 2o lanes
+			bool redo = nan_hit;
+			for (int o = 1; o < L; o <<= 1) {
+				const float t2 = __shfl_xor(tmin, o);
+				const int b2 = __shfl_xor(best, o), e2 = __shfl_xor(best_e, o);
+				redo |= __shfl_xor((int)redo, o) != 0;
+				if (b2 >= 0 && (best < 0 || t2 < tmin || (t2 == tmin && e2 < best_e))) {
+					tmin = t2;
+					best = b2;
+					best_e = e2;
+				}
+			}
+			if (redo) {
+				// the whole list in order, in every lane of the group (its
+				// tests were counted by the shares)
+				Cnt tmp = c;
+				tmin = 1000000.0f;
+				tmin_hi = 1000000.0f * kGuardUp;
+				best = -1;
+				test_list<F, 1>(S, begin, cnt, 0, px, py, pz, vx, vy, vz, ban, ck, cb, tmin, tmin_hi, best, best_e,
+				                nan_hit, tmp);
 			}
 		}
 		if (best >= 0) return best;
@@ -740,16 +817,32 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(DScene S, DRecv R, int nv, 
 // neighbouring receivers toward the same point and every lane has a ray.
 // RayHitted (raycast.cpp:335-360); a ray that does not reach the source
 // zeroes its form factor (shade.cpp:124-126).  XCD-contiguous block order.
-template <bool F>
+// Lanes per ray for nr rays: where one lane per ray would leave most of the
+// machine's wave slots idle, 2 or 4 lanes share each ray's lists.
+__device__ __forceinline__ int lanes_per_ray(int nr)
+{
+	return 4LL * nr <= kTraceSlots ? 4 : 2LL * nr <= kTraceSlots ? 2 : 1;
+}
+
+// SPLIT = false takes one lane per ray (a grid of one lane per possible
+// ray), SPLIT = true 2 or 4 (a grid of kTraceSlots lanes, its own register
+// budget).  On scenes small enough for the shared form to pay (both_forms)
+// both are launched for every source and one of them returns at once; on
+// larger ones only SPLIT = false, which then takes every source.
+template <bool F, bool SPLIT>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(DScene S, DRecv R, Loop *L, int k, float *ffbuf,
-                                                       long long *bcnt, const int *rays)
+                                                       long long *bcnt, const int *rays, int both_forms)
 {
 	if (L->done || k >= L->nsrc) return;
+	const int nr = L->nrays;
+	const int lpr = SPLIT ? lanes_per_ray(nr) : 1;
+	if (SPLIT ? lpr == 1 : both_forms && lanes_per_ray(nr) != 1) return;
 	// the compacted list spread XCD-contiguously over the blocks that have work
-	const int nr = L->nrays, nb = (nr + kTraceBlock - 1) / kTraceBlock, per = (nb + 7) >> 3;
+	const int rpb = kTraceBlock / lpr;  // rays per block
+	const int nb = (nr + rpb - 1) / rpb, per = (nb + 7) >> 3;
 	const int xcd = blockIdx.x & 7;
 	const int vb = xcd * per + (blockIdx.x >> 3);
-	const int j = vb * kTraceBlock + threadIdx.x;
+	const int j = vb * rpb + (int)threadIdx.x / lpr, sub = (int)threadIdx.x % lpr;
 	Cnt c = {0, 0, 0, 0, 0, 0, 0};
 	if ((int)(blockIdx.x >> 3) < per && j < nr) {
 		const int i = rays[j];
@@ -758,9 +851,26 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(DScene S, DRecv R, Loop *
 		const float px = R.vx[sv], py = R.vy[sv], pz = R.vz[sv];
 		const int ban = __float_as_int(R.nrm[r].w);
 		const SrcPatch &sp = s.split ? s.half[h] : s.whole;
-		c.rays++;
-		c.ref_rays++;
-		if (walk<F>(S, px, py, pz, sp.c[0] - px, sp.c[1] - py, sp.c[2] - pz, ban, c) != s.log) ffbuf[i] = 0.0f;
+		const float vx = sp.c[0] - px, vy = sp.c[1] - py, vz = sp.c[2] - pz;
+		if (!SPLIT) {
+			c.rays++;
+			c.ref_rays++;
+			if (walk<F>(S, px, py, pz, vx, vy, vz, ban, c) != s.log) ffbuf[i] = 0.0f;
+		} else {
+			Cnt w = {0, 0, 0, 0, 0, 0, 0};
+			const int hit = lpr == 4 ? walk<F, 4>(S, px, py, pz, vx, vy, vz, ban, w, sub)
+			                         : walk<F, 2>(S, px, py, pz, vx, vy, vz, ban, w, sub);
+			// every lane of a group counts its own tests; the rest once
+			c.tests = w.tests;
+			if (sub == 0) {
+				c.rays = c.ref_rays = 1;
+				c.cells = w.cells;
+				c.levels = w.levels;
+				c.precision = w.precision;
+				c.stalled = w.stalled;
+				if (hit != s.log) ffbuf[i] = 0.0f;
+			}
+		}
 	}
 	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
 	add_block_counters<kTraceBlock / kWave>(vals, bcnt);
@@ -1129,17 +1239,35 @@ __global__ __launch_bounds__(kBlock) void k_pack_recv(const Patch *__restrict__ 
 	dB[i] = make_float4(p.deltaB[0], p.deltaB[1], p.deltaB[2], 0.f);
 }
 
-template <bool F>
+// RayHitted for n rays, L lanes per ray (as k_trace: each tests every L-th
+// entry of a leaf's list; cells from the group's first lane, tests summed)
+template <bool F, int L>
 __global__ __launch_bounds__(kBlock) void k_cast(DScene S, const RayIn *rays, int n, int *hit, int *cells, int *tests)
 {
-	const int i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= n) return;
+	const int g = blockIdx.x * kBlock + threadIdx.x;
+	const int i = g / L, sub = g % L;
+	if (i >= n) return;  // a whole group: its lanes share i
 	const RayIn r = rays[i];
 	Cnt c = {0, 0, 0, 0, 0, 0, 0};
-	const int h = walk<F>(S, r.p[0], r.p[1], r.p[2], r.v[0], r.v[1], r.v[2], r.ban, c);
-	hit[i] = h;
-	cells[i] = c.cells;
-	tests[i] = c.tests;
+	const int h = walk<F, L>(S, r.p[0], r.p[1], r.p[2], r.v[0], r.v[1], r.v[2], r.ban, c, sub);
+	int t = c.tests;
+	for (int o = 1; o < L; o <<= 1) t += __shfl_xor(t, o);
+	if (sub == 0) {
+		hit[i] = h;
+		cells[i] = c.cells;
+		tests[i] = t;
+	}
+}
+
+// k_cast over n rays with lanes per ray as k_trace would choose them
+template <bool F>
+void launch_cast(hipStream_t st, const DScene &S, const RayIn *rays, int n, int *hit, int *cells, int *tests)
+{
+	const int lpr = 4LL * n <= kTraceSlots ? 4 : 2LL * n <= kTraceSlots ? 2 : 1;
+	const dim3 g((unsigned)(((size_t)n * lpr + kBlock - 1) / kBlock)), b(kBlock);
+	if (lpr == 4) hipLaunchKernelGGL((k_cast<F, 4>), g, b, 0, st, S, rays, n, hit, cells, tests);
+	else if (lpr == 2) hipLaunchKernelGGL((k_cast<F, 2>), g, b, 0, st, S, rays, n, hit, cells, tests);
+	else hipLaunchKernelGGL((k_cast<F, 1>), g, b, 0, st, S, rays, n, hit, cells, tests);
 }
 
 // Scratch of rad_cast_static, the variant of rad_cast that allocates nothing:
@@ -1943,7 +2071,8 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	const int grid = ((d.nblocks + 7) / 8) * 8;  // a whole number of blocks per XCD
 	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
 	const int v3grid = d.v3 ? (int)(((size_t)cap * 4 + kBlock - 1) / kBlock) : 0;
-	std::vector<long long> zeros((size_t)std::max(std::max(grid, tgrid), v3grid) * 8, 0);
+	const int sgrid = (int)(kTraceSlots / kTraceBlock);  // k_trace<F, true>
+	std::vector<long long> zeros((size_t)std::max(std::max(std::max(grid, tgrid), v3grid), sgrid) * 8, 0);
 	if (d.bcnt.put(zeros.data(), zeros.size(), st, err)) return -1;
 	if (!d.lowest && d.parallel_src > 1 && d.cand.alloc((size_t)n + 1, err)) return -1;
 	if (d.live.alloc((size_t)n + 1, err) || d.ffbuf.alloc((size_t)n * 6 + 2, err) ||
@@ -2054,7 +2183,8 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 	}
 	const int grid = ((d.nblocks + 7) / 8) * 8;
 	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
-	if (d.bcnt.n < (size_t)std::max(grid, tgrid) * 8 || d.ffbuf.n < (size_t)d.R.nslots * 6 || d.live.n < (size_t)d.R.nslots ||
+	if (d.bcnt.n < (size_t)std::max(std::max(grid, tgrid), (int)(kTraceSlots / kTraceBlock)) * 8 ||
+	    d.ffbuf.n < (size_t)d.R.nslots * 6 || d.live.n < (size_t)d.R.nslots ||
 	    d.raycnt.n < (size_t)d.nblocks || d.rays.n < (size_t)d.R.nslots * 6) {
 		err = "device buffers smaller than the launch grids";
 		return -1;
@@ -2083,8 +2213,18 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 			hipLaunchKernelGGL(k_compact, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, d.order.p, 6 * d.R.nslots, d.raycnt.p,
 			                   d.nblocks, d.rkey.p, d.rays.p);
 			if (ev >= 0 && k == 0) (void)hipEventRecord(d.kev[2 * ev], st);
-			if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
-			else hipLaunchKernelGGL(k_trace<false>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
+			// the shared-lane form pays where a source's rays (about 2.4 per
+			// slot) leave the machine's slots idle: scenes of at most
+			// 4 kTraceSlots / 6 slots (218k)
+			const int both = (long long)d.R.nslots * 6 <= 4 * kTraceSlots;
+			const dim3 sg(kTraceSlots / kTraceBlock);
+			if (d.fma) {
+				hipLaunchKernelGGL((k_trace<true, false>), tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p, both);
+				if (both) hipLaunchKernelGGL((k_trace<true, true>), sg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p, both);
+			} else {
+				hipLaunchKernelGGL((k_trace<false, false>), tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p, both);
+				if (both) hipLaunchKernelGGL((k_trace<false, true>), sg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p, both);
+			}
 			if (ev >= 0 && k == 0) (void)hipEventRecord(d.kev[2 * ev + 1], st);
 #define UPD(F, FS, LO) hipLaunchKernelGGL((k_update<F, FS, LO>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.converge, d.bcnt.p, d.nblocks)
 			if (multi) { if (d.fma) UPD(true, false, false); else UPD(false, false, false); }
@@ -2207,10 +2347,9 @@ int DeviceEngine::cast(int n, const RayIn *rays, int *hit, int *cells, int *test
 	DevBuf<int> dh, dc, dt;
 	hipStream_t st = d.stream;
 	if (dr.put(rays, n, st, err) || dh.alloc(n, err) || dc.alloc(n, err) || dt.alloc(n, err)) return -1;
-	const int blocks = (n + kBlock - 1) / kBlock;
 	HIP_OK(hipEventRecord(d.ev0, st), "event");
-	if (d.fma) hipLaunchKernelGGL(k_cast<true>, dim3(blocks), dim3(kBlock), 0, st, d.S, dr.p, n, dh.p, dc.p, dt.p);
-	else hipLaunchKernelGGL(k_cast<false>, dim3(blocks), dim3(kBlock), 0, st, d.S, dr.p, n, dh.p, dc.p, dt.p);
+	if (d.fma) launch_cast<true>(st, d.S, dr.p, n, dh.p, dc.p, dt.p);
+	else launch_cast<false>(st, d.S, dr.p, n, dh.p, dc.p, dt.p);
 	HIP_OK(hipGetLastError(), "cast launch");
 	HIP_OK(hipEventRecord(d.ev1, st), "event");
 	HIP_OK(hipMemcpyAsync(hit, dh.p, n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");
@@ -2286,10 +2425,9 @@ int DeviceEngine::cast_static(int n, const RayIn *rays, int *hit, int *cells, in
 	int *dc = dh + kStaticRays, *dt = dc + kStaticRays;
 	hipStream_t st = d.stream;
 	HIP_OK(hipMemcpyAsync(dr, rays, (size_t)n * sizeof(RayIn), hipMemcpyHostToDevice, st), "upload");
-	const int blocks = (n + kBlock - 1) / kBlock;
 	HIP_OK(hipEventRecord(d.ev0, st), "event");
-	if (d.fma) hipLaunchKernelGGL(k_cast<true>, dim3(blocks), dim3(kBlock), 0, st, d.S, dr, n, dh, dc, dt);
-	else hipLaunchKernelGGL(k_cast<false>, dim3(blocks), dim3(kBlock), 0, st, d.S, dr, n, dh, dc, dt);
+	if (d.fma) launch_cast<true>(st, d.S, dr, n, dh, dc, dt);
+	else launch_cast<false>(st, d.S, dr, n, dh, dc, dt);
 	HIP_OK(hipGetLastError(), "cast launch");
 	HIP_OK(hipEventRecord(d.ev1, st), "event");
 	HIP_OK(hipMemcpyAsync(hit, dh, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, st), "cast");

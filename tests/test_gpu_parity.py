@@ -148,7 +148,10 @@ def test_hall_subset_matches_oracle(select):
 
 def test_hall_full_first_shots_match_oracle():
     """The 200k-triangle hall: the first 3 shots bit-identical to the oracle,
-    and random rays through the full octree identical."""
+    and random rays through the full octree identical: 200000 rays take one
+    lane each, the first 100000 of them cast alone two lanes each and the
+    first 50000 four (each lane tests every second or fourth list entry, and
+    the lanes combine), with the same hits, cells and tests."""
     s = scenes.hall_scene(2)
     opts = ["-converge", "1200", "-triangle", "1000000", "-quiet", "1"]
     print("hall full", s.n, flush=True)
@@ -165,6 +168,7 @@ def test_hall_full_first_shots_match_oracle():
         rays["ban"] = rec0["parent"][dst]
         rays["target"] = rec0["parent"][src]
         hit, cells, tests = eng.cast(rays)
+        shared = [eng.cast(rays[:m]) for m in (100000, 50000)]
         shots, _ = eng.shoot(3)
         rec, _ = eng.records()
     o = ol.Oracle(*(opts + ["-jobs", "16"]))
@@ -176,6 +180,9 @@ def test_hall_full_first_shots_match_oracle():
     assert rec0.tobytes() == orec0.tobytes()
     ohit, ocells, otests = o.raycast(rays.view(ol.RAY_DTYPE))
     assert np.array_equal(hit, ohit) and np.array_equal(cells, ocells) and np.array_equal(tests, otests)
+    for h2, c2, t2 in shared:
+        m = len(h2)
+        assert np.array_equal(h2, hit[:m]) and np.array_equal(c2, cells[:m]) and np.array_equal(t2, tests[:m])
     o.iterate(3)
     orec, _ = o.dump()
     assert shots == 3
