@@ -138,6 +138,125 @@ __global__ __launch_bounds__(64 * W) void k_gemm_lds(const E *__restrict__ A, co
 			}
 }
 
+// Wide right-hand sides with the operands staged by LDS DMA: eight waves,
+// each owning 16 RT = 32 rows and all 16 CT columns (the accumulators stay in
+// VGPRs: with more rows they would need AGPRs, which the register allocator
+// then copies every k-block); each k-block's A fragments (the wave's own,
+// 1 KB each) and B fragments (shared by the block, loaded by the first waves)
+// go straight from HBM into a ring of NS LDS slots (global_load_lds_dwordx4:
+// no registers held while in flight), NS - 1 k-blocks ahead of the MFMAs:
+// 64 KB of A per CU in flight, where a register pipeline holds 32 KB.  The
+// compiler treats any LDS read as dependent on every LDS DMA in flight and
+// would drain them all before it, so the slot is read by inline assembly
+// (its own lgkmcnt wait inside; early-clobber outputs, so no read overwrites
+// the address of a later one) after an explicit vmcnt wait for this
+// k-block's DMAs and a barrier; every trip issues the same DMAs per wave
+// (the index is clamped past the chunk's end; those slots are never read),
+// so the vmcnt bound holds to the last trip.
+template <typename E, int CT>
+__global__ __launch_bounds__(512) void k_gemm_dma(const E *__restrict__ A, const E *__restrict__ XT,
+                                                  float *__restrict__ part, int ld, int rp, int kchunk)
+{
+	typedef typename VecOf<E>::type V;
+	constexpr int RT = 2, W = 8, NS = 5;
+	constexpr int SLOT = (W * RT + CT) * 64;  // fragments (16 bytes) per ring slot
+	// B fragments per wave: waves w < BX load BW + 1, the others BW
+	constexpr int BW = CT / W, BX = CT % W;
+	static_assert(CT == 8 || CT == 12, "8 or 12 column tiles");
+	__shared__ V ring[NS * SLOT];
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const int rb0 = (blockIdx.x * W + w) * RT;
+	const int s = blockIdx.y, kb0 = s * (kchunk >> 5), kb1 = kb0 + (kchunk >> 5);
+	const int nkb = ld >> 5;
+	const V *Ap = (const V *)A + lane, *Bp = (const V *)XT + lane;
+	const bool extra = w < BX;
+	const int c0 = w * BW + (extra ? w : BX);  // this wave's first B fragment
+	f4 acc[RT][CT];
+#pragma unroll
+	for (int r = 0; r < RT; r++)
+#pragma unroll
+		for (int c = 0; c < CT; c++) acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
+	auto issue = [&](int kb, int slot) {
+		const int k = min(kb, kb1 - 1);
+		V *base = ring + slot * SLOT;
+#pragma unroll
+		for (int r = 0; r < RT; r++)
+			__builtin_amdgcn_global_load_lds((const void *)(Ap + ((size_t)(rb0 + r) * nkb + k) * 64),
+			                                 (__attribute__((address_space(3))) void *)(base + (w * RT + r) * 64), 16, 0, 0);
+#pragma unroll
+		for (int m = 0; m < BW; m++)
+			__builtin_amdgcn_global_load_lds((const void *)(Bp + ((size_t)(c0 + m) * nkb + k) * 64),
+			                                 (__attribute__((address_space(3))) void *)(base + (W * RT + c0 + m) * 64), 16, 0, 0);
+		if (extra)
+			__builtin_amdgcn_global_load_lds((const void *)(Bp + ((size_t)(c0 + BW) * nkb + k) * 64),
+			                                 (__attribute__((address_space(3))) void *)(base + (W * RT + c0 + BW) * 64), 16, 0, 0);
+	};
+	const unsigned lbase = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void *)ring + lane * 16;
+	for (int i = 0; i < NS - 1; i++) issue(kb0 + i, i);
+	int slot = 0;
+	auto step = [&](int kb) {
+		// this k-block's DMAs: NS - 2 k-blocks' DMAs (RT + BW, + 1 on the
+		// first BX waves) were issued after them
+		static_assert(NS == 5 && RT == 2 && W == 8, "the vmcnt bounds below are 3 (RT + BW [+ 1])");
+		if (CT == 12) {
+			if (extra) asm volatile("s_waitcnt vmcnt(12)" ::: "memory");
+			else asm volatile("s_waitcnt vmcnt(9)" ::: "memory");
+		} else {
+			asm volatile("s_waitcnt vmcnt(9)" ::: "memory");
+		}
+		__builtin_amdgcn_s_barrier();
+		// every wave is past its reads of the previous trip's slot: refill it
+		issue(kb + NS - 1, slot == 0 ? NS - 1 : slot - 1);
+		const unsigned ab = lbase + (unsigned)(slot * SLOT * 16 + w * RT * 1024);
+		const unsigned bb = lbase + (unsigned)(slot * SLOT * 16 + W * RT * 1024);
+		V a[RT], b[CT];
+		if constexpr (CT == 12) {
+			asm volatile(
+				"ds_read_b128 %0, %14\n ds_read_b128 %1, %14 offset:1024\n"
+				"ds_read_b128 %2, %15\n ds_read_b128 %3, %15 offset:1024\n"
+				"ds_read_b128 %4, %15 offset:2048\n ds_read_b128 %5, %15 offset:3072\n"
+				"ds_read_b128 %6, %15 offset:4096\n ds_read_b128 %7, %15 offset:5120\n"
+				"ds_read_b128 %8, %15 offset:6144\n ds_read_b128 %9, %15 offset:7168\n"
+				"ds_read_b128 %10, %15 offset:8192\n ds_read_b128 %11, %15 offset:9216\n"
+				"ds_read_b128 %12, %15 offset:10240\n ds_read_b128 %13, %15 offset:11264\n"
+				"s_waitcnt lgkmcnt(0)"
+				: "=&v"(a[0]), "=&v"(a[1]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]),
+				  "=&v"(b[5]), "=&v"(b[6]), "=&v"(b[7]), "=&v"(b[8]), "=&v"(b[9]), "=&v"(b[10]), "=&v"(b[11])
+				: "v"(ab), "v"(bb)
+				: "memory");
+		} else {
+			asm volatile(
+				"ds_read_b128 %0, %10\n ds_read_b128 %1, %10 offset:1024\n"
+				"ds_read_b128 %2, %11\n ds_read_b128 %3, %11 offset:1024\n"
+				"ds_read_b128 %4, %11 offset:2048\n ds_read_b128 %5, %11 offset:3072\n"
+				"ds_read_b128 %6, %11 offset:4096\n ds_read_b128 %7, %11 offset:5120\n"
+				"ds_read_b128 %8, %11 offset:6144\n ds_read_b128 %9, %11 offset:7168\n"
+				"s_waitcnt lgkmcnt(0)"
+				: "=&v"(a[0]), "=&v"(a[1]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]),
+				  "=&v"(b[5]), "=&v"(b[6]), "=&v"(b[7])
+				: "v"(ab), "v"(bb)
+				: "memory");
+		}
+#pragma unroll
+		for (int r = 0; r < RT; r++)
+#pragma unroll
+			for (int c = 0; c < CT; c++) acc[r][c] = mfma(a[r], b[c], acc[r][c]);
+		slot = slot == NS - 1 ? 0 : slot + 1;
+	};
+	for (int kb = kb0; kb < kb1; kb++) step(kb);
+	asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // no DMA may outlive the block's LDS
+	const int lr = lane & 15;
+#pragma unroll
+	for (int r = 0; r < RT; r++)
+#pragma unroll
+		for (int c = 0; c < CT; c++)
+#pragma unroll
+			for (int j = 0; j < 4; j++) {
+				const int row = (rb0 + r) * 16 + (lane >> 4) * 4 + j;
+				part[((size_t)s * ld + row) * rp + c * 16 + lr] = acc[r][c][j];
+			}
+}
+
 // d = rhs + rho o (inv_s * sum_s part); XT <- d (fragment order).
 // Padding of XT (rows >= nrhs, k >= n) is never written and stays zero.
 // RHO_IN (the adjoint's sweep d <- r + F^T (rho o d)): v = rhs + inv_s * sum
@@ -549,9 +668,9 @@ int launch_gemm(int rt, int ct, const E *A, const E *XT, float *part, int ld, in
 	if (rt == 1) {
 		GR(1)
 	} else if (rt == 2 && ct == 8) {
-		hipLaunchKernelGGL((k_gemm_lds<E, 2, 8, 8>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
+		hipLaunchKernelGGL((k_gemm_dma<E, 8>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
 	} else if (rt == 2 && ct == 12) {
-		hipLaunchKernelGGL((k_gemm_lds<E, 2, 12, 8>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
+		hipLaunchKernelGGL((k_gemm_dma<E, 12>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
 	} else if (rt == 2) {
 		GR(2)
 	} else if (rt == 4 && ct <= 4) {
@@ -575,13 +694,14 @@ int launch_gemm(int rt, int ct, const E *A, const E *XT, float *part, int ld, in
 
 bool valid_tile(int rt, int ct, int S, int ld)
 {
-	// k-blocks per pipelined loop trip: 2 * GemmUnroll<ct>::U, or 2 * U of
-	// the LDS kernels (8 or 12 column tiles: U = 4 or 2); rows per block:
-	// 64 rt, or 256 for the eight-wave LDS kernel (rt = 2)
-	const bool lds = ct >= 8 && (rt == 4 || rt == 2);
-	const int u2 = lds ? (ct >= 12 ? 4 : 8) : 2 * (ct <= 4 ? 4 : 2);
-	const int rows = (lds && rt == 2) ? 256 : 64 * rt;
-	return ld % rows == 0 && ld % (32 * S) == 0 && (ld / 32 / S) % u2 == 0;
+	// k-blocks per pipelined loop trip: 2 * GemmUnroll<ct>::U, 2 * U of the
+	// four-wave LDS kernel (8 or 12 column tiles, rt = 4: U = 4 or 2), or 1
+	// for the eight-wave LDS DMA kernel (8 or 12 column tiles, rt = 2);
+	// rows per block: 64 rt, or 256 for the eight-wave kernel
+	const bool lds = ct >= 8 && rt == 4, dma = ct >= 8 && rt == 2;
+	const int u2 = dma ? 1 : lds ? (ct >= 12 ? 4 : 8) : 2 * (ct <= 4 ? 4 : 2);
+	const int rows = dma ? 256 : 64 * rt;
+	return ld % rows == 0 && ld % (32 * S) == 0 && (ld / 32 / S) % u2 == 0 && ld / 32 / S >= 1;
 }
 
 double power_sum(const std::vector<float> &a, const double *v, int n, int nrhs)
