@@ -1056,7 +1056,9 @@ __device__ bool seg_occluded(const DFine &G, float px, float py, float pz, float
 			last = leaf;
 			c.cells++;
 			const int e0 = G.begin[leaf], e1 = G.begin[leaf + 1];
-			// the whole 32-byte record at once, the next one in flight
+			// the whole 32-byte record at once, the next one in flight; the
+			// load is not conditional (the index is clamped to the list), so
+			// that the wait before a test covers only the loads it needs
 			float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nb = na;
 			if (e0 < e1) {
 				na = G.nc[2 * e0];
@@ -1064,10 +1066,9 @@ __device__ bool seg_occluded(const DFine &G, float px, float py, float pz, float
 			}
 			for (int e = e0; e < e1; e++) {
 				const float4 a = na, b = nb;
-				if (e + 1 < e1) {
-					na = G.nc[2 * e + 2];
-					nb = G.nc[2 * e + 3];
-				}
+				const int en = min(e + 1, e1 - 1);
+				na = G.nc[2 * en];
+				nb = G.nc[2 * en + 1];
 				const int id = __float_as_int(b.z);
 				if (id == ban0 || id == ban1) continue;
 				c.tests++;
