@@ -2,7 +2,8 @@
 -converge 1200; or, with --linear, the explicit
 matrix workload: the 32768-patch room's form factors, FP16 gathers with 3
 and 192 right-hand sides, 2000 plain shots; with --gemm, the same matrix and
-five FP16 sweeps with 192 right-hand sides at the tile RT = 4, split-K 2;
+five FP16 sweeps with 192 right-hand sides at the tile RT = 2 (the LDS DMA
+kernel), split-K 2;
 with --batch, 256 rooms of 500-8000 patches, the autotuned FP16 batch solve
 twice in one graph launch; with --adjoint, the 16384-patch room's FP16
 adjoint solve twice), then a plain 256 MiB device copy that calibrates the
@@ -48,7 +49,7 @@ def gemm():
     e.set_scene(sc).prepare_host().upload()
     lin = radgpu.LinearSystem(n)
     print("form factors %.1f ms" % lin.from_scene(e), flush=True)
-    lin.set_tile(192, "f16", 4, 2)
+    lin.set_tile(192, "f16", 2, 2)
     print("gemm 192 f16 %.3f ms" % lin.gemm_ms(192, "f16", reps=5), flush=True)
     lin.close()
     e.close()
