@@ -65,8 +65,8 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 // k_trace's block: one wave, so that a wave whose rays walk far holds only
 // its own slot (rays' costs are heavy-tailed)
 constexpr int kTraceBlock = 64;
-// lanes the machine runs at once of the k_trace form that shares rays over
-// lanes (256 CUs x 4 SIMDs x 5 waves, its occupancy, x 64)
+// k_trace lanes the machine runs at once (256 CUs x 4 SIMDs x 5 waves, its
+// occupancy, x 64)
 constexpr long long kTraceSlots = 256LL * 4 * 5 * 64;
 constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
 constexpr int kGroups = 2;        // receiver groups per block
@@ -812,32 +812,27 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(DScene S, DRecv R, int nv, 
 	rkey[sv] = (unsigned char)key;
 }
 
-// Source k's rays, part 2: one lane per ray of the compacted list, which
-// keeps the receivers' spatial order, so a wave's rays start from
-// neighbouring receivers toward the same point and every lane has a ray.
-// RayHitted (raycast.cpp:335-360); a ray that does not reach the source
-// zeroes its form factor (shade.cpp:124-126).  XCD-contiguous block order.
-// Lanes per ray for nr rays: where one lane per ray would leave most of the
-// machine's wave slots idle, 2 or 4 lanes share each ray's lists.
+// Source k's rays, part 2: the compacted list, which keeps the receivers'
+// spatial order, so a wave's rays start from neighbouring receivers toward
+// the same point and every lane has a ray.  RayHitted (raycast.cpp:335-360);
+// a ray that does not reach the source zeroes its form factor
+// (shade.cpp:124-126).  XCD-contiguous block order.  Two lanes share each
+// ray's leaf lists (each tests every second entry), or four where even that
+// would leave most of the machine's wave slots idle: measured on the hall
+// against one lane per ray (242 us) and four (239 us), two take 227 us, the
+// list loop's lanes being busier than the walk's doubled steps cost.
 __device__ __forceinline__ int lanes_per_ray(int nr)
 {
-	return 4LL * nr <= kTraceSlots ? 4 : 2LL * nr <= kTraceSlots ? 2 : 1;
+	return 4LL * nr <= kTraceSlots ? 4 : 2;
 }
 
-// SPLIT = false takes one lane per ray (a grid of one lane per possible
-// ray), SPLIT = true 2 or 4 (a grid of kTraceSlots lanes, its own register
-// budget).  On scenes small enough for the shared form to pay (both_forms)
-// both are launched for every source and one of them returns at once; on
-// larger ones only SPLIT = false, which then takes every source.
-template <bool F, bool SPLIT>
+template <bool F>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(DScene S, DRecv R, Loop *L, int k, float *ffbuf,
-                                                       long long *bcnt, const int *rays, int both_forms)
+                                                       long long *bcnt, const int *rays)
 {
 	if (L->done || k >= L->nsrc) return;
 	const int nr = L->nrays;
-	const int lpr = SPLIT ? lanes_per_ray(nr) : 1;
-	if (SPLIT ? lpr == 1 : both_forms && lanes_per_ray(nr) != 1) return;
-	// the compacted list spread XCD-contiguously over the blocks that have work
+	const int lpr = lanes_per_ray(nr);
 	const int rpb = kTraceBlock / lpr;  // rays per block
 	const int nb = (nr + rpb - 1) / rpb, per = (nb + 7) >> 3;
 	const int xcd = blockIdx.x & 7;
@@ -852,24 +847,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(DScene S, DRecv R, Loop *
 		const int ban = __float_as_int(R.nrm[r].w);
 		const SrcPatch &sp = s.split ? s.half[h] : s.whole;
 		const float vx = sp.c[0] - px, vy = sp.c[1] - py, vz = sp.c[2] - pz;
-		if (!SPLIT) {
-			c.rays++;
-			c.ref_rays++;
-			if (walk<F>(S, px, py, pz, vx, vy, vz, ban, c) != s.log) ffbuf[i] = 0.0f;
-		} else {
-			Cnt w = {0, 0, 0, 0, 0, 0, 0};
-			const int hit = lpr == 4 ? walk<F, 4>(S, px, py, pz, vx, vy, vz, ban, w, sub)
-			                         : walk<F, 2>(S, px, py, pz, vx, vy, vz, ban, w, sub);
-			// every lane of a group counts its own tests; the rest once
-			c.tests = w.tests;
-			if (sub == 0) {
-				c.rays = c.ref_rays = 1;
-				c.cells = w.cells;
-				c.levels = w.levels;
-				c.precision = w.precision;
-				c.stalled = w.stalled;
-				if (hit != s.log) ffbuf[i] = 0.0f;
-			}
+		Cnt w = {0, 0, 0, 0, 0, 0, 0};
+		const int hit = lpr == 4 ? walk<F, 4>(S, px, py, pz, vx, vy, vz, ban, w, sub)
+		                         : walk<F, 2>(S, px, py, pz, vx, vy, vz, ban, w, sub);
+		// every lane of a group counts its own tests; the rest once
+		c.tests = w.tests;
+		if (sub == 0) {
+			c.rays = c.ref_rays = 1;
+			c.cells = w.cells;
+			c.levels = w.levels;
+			c.precision = w.precision;
+			c.stalled = w.stalled;
+			if (hit != s.log) ffbuf[i] = 0.0f;
 		}
 	}
 	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
@@ -2068,12 +2057,13 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	// grid sizes first: every kernel that adds into bcnt indexes it by its
 	// hardware block index, so bcnt must cover the largest grid
 	d.nblocks = (n + kRecvPerBlock - 1) / kRecvPerBlock;
-	d.trace_nblocks = (int)(((size_t)n * 6 + kTraceBlock - 1) / kTraceBlock);
+	// k_trace: two lanes for each of up to 6 rays per slot, or four lanes on
+	// up to kTraceSlots lanes
+	d.trace_nblocks = (int)std::max(((size_t)n * 12 + kTraceBlock - 1) / kTraceBlock, (size_t)(kTraceSlots / kTraceBlock));
 	const int grid = ((d.nblocks + 7) / 8) * 8;  // a whole number of blocks per XCD
 	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
 	const int v3grid = d.v3 ? (int)(((size_t)cap * 4 + kBlock - 1) / kBlock) : 0;
-	const int sgrid = (int)(kTraceSlots / kTraceBlock);  // k_trace<F, true>
-	std::vector<long long> zeros((size_t)std::max(std::max(std::max(grid, tgrid), v3grid), sgrid) * 8, 0);
+	std::vector<long long> zeros((size_t)std::max(std::max(grid, tgrid), v3grid) * 8, 0);
 	if (d.bcnt.put(zeros.data(), zeros.size(), st, err)) return -1;
 	if (!d.lowest && d.parallel_src > 1 && d.cand.alloc((size_t)n + 1, err)) return -1;
 	if (d.live.alloc((size_t)n + 1, err) || d.ffbuf.alloc((size_t)n * 6 + 2, err) ||
@@ -2184,7 +2174,7 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 	}
 	const int grid = ((d.nblocks + 7) / 8) * 8;
 	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
-	if (d.bcnt.n < (size_t)std::max(std::max(grid, tgrid), (int)(kTraceSlots / kTraceBlock)) * 8 ||
+	if (d.bcnt.n < (size_t)std::max(grid, tgrid) * 8 ||
 	    d.ffbuf.n < (size_t)d.R.nslots * 6 || d.live.n < (size_t)d.R.nslots ||
 	    d.raycnt.n < (size_t)d.nblocks || d.rays.n < (size_t)d.R.nslots * 6) {
 		err = "device buffers smaller than the launch grids";
@@ -2214,18 +2204,8 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 			hipLaunchKernelGGL(k_compact, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, d.order.p, 6 * d.R.nslots, d.raycnt.p,
 			                   d.nblocks, d.rkey.p, d.rays.p);
 			if (ev >= 0 && k == 0) (void)hipEventRecord(d.kev[2 * ev], st);
-			// the shared-lane form pays where a source's rays (about 2.4 per
-			// slot) leave the machine's slots idle: scenes of at most
-			// 4 kTraceSlots / 6 slots (218k)
-			const int both = (long long)d.R.nslots * 6 <= 4 * kTraceSlots;
-			const dim3 sg(kTraceSlots / kTraceBlock);
-			if (d.fma) {
-				hipLaunchKernelGGL((k_trace<true, false>), tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p, both);
-				if (both) hipLaunchKernelGGL((k_trace<true, true>), sg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p, both);
-			} else {
-				hipLaunchKernelGGL((k_trace<false, false>), tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p, both);
-				if (both) hipLaunchKernelGGL((k_trace<false, true>), sg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p, both);
-			}
+			if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
+			else hipLaunchKernelGGL(k_trace<false>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
 			if (ev >= 0 && k == 0) (void)hipEventRecord(d.kev[2 * ev + 1], st);
 #define UPD(F, FS, LO) hipLaunchKernelGGL((k_update<F, FS, LO>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.converge, d.bcnt.p, d.nblocks)
 			if (multi) { if (d.fma) UPD(true, false, false); else UPD(false, false, false); }
