@@ -47,130 +47,108 @@ __global__ __launch_bounds__(kB) void k_gemm(const E *__restrict__ A, const E *_
 	gemm_tile<E, RT, CT>(A, XT, part + (size_t)s * ld * rp, ld, rp, (blockIdx.x * 4 + w) * RT, kb0, kb0 + (kchunk >> 5));
 }
 
-// Wide right-hand sides (CT = 8 or 12 column tiles): the B fragments of
-// each group of U k-blocks are loaded once per block, cooperatively, into LDS
-// (double-buffered) and read there by the W waves, so the B operand costs
-// one L2 read per block instead of one per wave.  A wave owns 16*RT rows;
-// its A fragments for the next group stream from HBM while the current group
-// feeds the MFMAs.  W = 8 with RT = 2 keeps two waves on each SIMD, so one
-// computes while the other waits.  Needs (k-blocks per chunk) % (2U) == 0
-// (PER is whole for U * CT a multiple of W).
-template <typename E, int RT, int CT, int W>
-__global__ __launch_bounds__(64 * W) void k_gemm_lds(const E *__restrict__ A, const E *__restrict__ XT,
-                                                     float *__restrict__ part, int ld, int rp, int kchunk)
+// Wide right-hand sides (NCT = 8 or 12 column tiles) with the operands
+// staged by LDS DMA.  A block is eight waves over 256 rows: WC = 1 gives each
+// wave 16 RT = 32 rows and every column, WC = 2 gives it 64 rows and half the
+// columns, which cuts the block's LDS reads per k-block from 112 to 80 KB (a
+// wave with 64 rows and every column would need its accumulators in AGPRs,
+// which the register allocator then copies every k-block).  Each k-block's A
+// fragments (1 KB each, those of a row group split over its WC waves) and B
+// fragments (spread over the waves) go straight from HBM into a ring of NS
+// LDS slots (global_load_lds_dwordx4: no registers held while in flight),
+// NS - 1 k-blocks ahead of the MFMAs: 64 KB of A per CU in flight, where a
+// register pipeline holds 32 KB.  The compiler treats any LDS read as
+// dependent on every LDS DMA in flight and would drain them all before it,
+// so the slot is read by inline assembly (its own lgkmcnt wait inside;
+// early-clobber outputs, so no read overwrites the address of a later one)
+// after an explicit vmcnt wait for this k-block's DMAs and a barrier; every
+// trip issues the same DMAs per wave (the index is clamped past the chunk's
+// end; those slots are never read), so the vmcnt bound holds to the last
+// trip.  The second wave on each SIMD (waves 4-7) runs half a k-block
+// behind: after each barrier it multiplies the previous k-block while the
+// first wave reads, then reads while the first wave multiplies (0.433 ->
+// 0.389 ms for 192 columns at n = 32768, where all eight waves had read and
+// then multiplied in step).
+// cache policy operand of global_load_lds: nt (non-temporal)
+constexpr int kCpolNT = 2;
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
 {
-	typedef typename VecOf<E>::type V;
-	constexpr int NT = 64 * W;
-	constexpr int U = CT >= 12 ? 2 : 4;  // k-blocks per group (registers: RT x CT accumulators)
-	constexpr int PER = U * CT * 64 / NT;     // B fragments (16 bytes) per thread per group
-	static_assert(U * CT * 64 % NT == 0, "B fragments split evenly over the block");
-	__shared__ V sb[2][U * CT * 64];
-	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-	const int rb0 = (blockIdx.x * W + w) * RT;
-	const int s = blockIdx.y, kb0 = s * (kchunk >> 5), kb1 = kb0 + (kchunk >> 5);
-	const int nkb = ld >> 5;
-	const V *Ap = (const V *)A + lane;
-	const V *Bg = (const V *)XT;
-	f4 acc[RT][CT];
-#pragma unroll
-	for (int r = 0; r < RT; r++)
-#pragma unroll
-		for (int c = 0; c < CT; c++) acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
-	V a0[U][RT], a1[U][RT], bs[PER];
-	// group g covers k-blocks kb .. kb+U-1; LDS slot (u * CT + c) * 64 + lane
-	auto load_b = [&](int kb) {
-#pragma unroll
-		for (int m = 0; m < PER; m++) {
-			const int q = threadIdx.x + m * NT;
-			const int l2 = q & 63, uc = q >> 6, u = uc / CT, c = uc % CT;
-			bs[m] = Bg[((size_t)c * nkb + kb + u) * 64 + l2];
-		}
-	};
-	auto store_b = [&](int buf) {
-#pragma unroll
-		for (int m = 0; m < PER; m++) sb[buf][threadIdx.x + m * NT] = bs[m];
-	};
-	auto load_a = [&](V (&a)[U][RT], int kb) {
-#pragma unroll
-		for (int u = 0; u < U; u++)
-#pragma unroll
-			for (int r = 0; r < RT; r++) a[u][r] = __builtin_nontemporal_load(Ap + ((size_t)(rb0 + r) * nkb + kb + u) * 64);
-	};
-	auto mma = [&](const V (&a)[U][RT], int buf) {
-#pragma unroll
-		for (int u = 0; u < U; u++)
-#pragma unroll
-			for (int c = 0; c < CT; c++) {
-				const V b = sb[buf][(u * CT + c) * 64 + lane];
-#pragma unroll
-				for (int r = 0; r < RT; r++) acc[r][c] = mfma(a[u][r], b, acc[r][c]);
-			}
-	};
-	load_b(kb0);
-	load_a(a0, kb0);
-	store_b(0);
-	__syncthreads();
-	for (int kb = kb0; kb < kb1; kb += 2 * U) {
-		// group kb (buffer 0, a0) computes while group kb + U loads
-		load_b(kb + U);
-		load_a(a1, kb + U);
-		mma(a0, 0);
-		store_b(1);
-		__syncthreads();
-		const bool more = kb + 2 * U < kb1;
-		if (more) {
-			load_b(kb + 2 * U);
-			load_a(a0, kb + 2 * U);
-		}
-		mma(a1, 1);
-		if (more) store_b(0);
-		__syncthreads();
-	}
-	const int lr = lane & 15;
-#pragma unroll
-	for (int r = 0; r < RT; r++)
-#pragma unroll
-		for (int c = 0; c < CT; c++)
-#pragma unroll
-			for (int j = 0; j < 4; j++) {
-				const int row = (rb0 + r) * 16 + (lane >> 4) * 4 + j;
-				part[((size_t)s * ld + row) * rp + c * 16 + lr] = acc[r][c][j];
-			}
+	static_assert(N >= 0 && N < 64, "vmcnt is six bits");
+	asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
-// Wide right-hand sides with the operands staged by LDS DMA: eight waves,
-// each owning 16 RT = 32 rows and all 16 CT columns (the accumulators stay in
-// VGPRs: with more rows they would need AGPRs, which the register allocator
-// then copies every k-block); each k-block's A fragments (the wave's own,
-// 1 KB each) and B fragments (shared by the block, loaded by the first waves)
-// go straight from HBM into a ring of NS LDS slots (global_load_lds_dwordx4:
-// no registers held while in flight), NS - 1 k-blocks ahead of the MFMAs:
-// 64 KB of A per CU in flight, where a register pipeline holds 32 KB.  The
-// compiler treats any LDS read as dependent on every LDS DMA in flight and
-// would drain them all before it, so the slot is read by inline assembly
-// (its own lgkmcnt wait inside; early-clobber outputs, so no read overwrites
-// the address of a later one) after an explicit vmcnt wait for this
-// k-block's DMAs and a barrier; every trip issues the same DMAs per wave
-// (the index is clamped past the chunk's end; those slots are never read),
-// so the vmcnt bound holds to the last trip.
-template <typename E, int CT>
+// RT A fragments (offsets 0, 1024, ...) from ab, then CT B fragments from bb,
+// completed before the block ends
+#define RGPU_DS1(i, base, off) "ds_read_b128 %" #i ", %" #base " offset:" #off "\n"
+template <int RT, int CT, typename V>
+__device__ __forceinline__ void read_slot(V (&a)[RT], V (&b)[CT], unsigned ab, unsigned bb)
+{
+	if constexpr (RT == 2 && CT == 12) {
+		asm volatile(RGPU_DS1(0, 14, 0) RGPU_DS1(1, 14, 1024) RGPU_DS1(2, 15, 0) RGPU_DS1(3, 15, 1024)
+		             RGPU_DS1(4, 15, 2048) RGPU_DS1(5, 15, 3072) RGPU_DS1(6, 15, 4096) RGPU_DS1(7, 15, 5120)
+		             RGPU_DS1(8, 15, 6144) RGPU_DS1(9, 15, 7168) RGPU_DS1(10, 15, 8192) RGPU_DS1(11, 15, 9216)
+		             RGPU_DS1(12, 15, 10240) RGPU_DS1(13, 15, 11264) "s_waitcnt lgkmcnt(0)"
+		             : "=&v"(a[0]), "=&v"(a[1]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]),
+		               "=&v"(b[5]), "=&v"(b[6]), "=&v"(b[7]), "=&v"(b[8]), "=&v"(b[9]), "=&v"(b[10]), "=&v"(b[11])
+		             : "v"(ab), "v"(bb)
+		             : "memory");
+	} else if constexpr (RT == 2 && CT == 8) {
+		asm volatile(RGPU_DS1(0, 10, 0) RGPU_DS1(1, 10, 1024) RGPU_DS1(2, 11, 0) RGPU_DS1(3, 11, 1024)
+		             RGPU_DS1(4, 11, 2048) RGPU_DS1(5, 11, 3072) RGPU_DS1(6, 11, 4096) RGPU_DS1(7, 11, 5120)
+		             RGPU_DS1(8, 11, 6144) RGPU_DS1(9, 11, 7168) "s_waitcnt lgkmcnt(0)"
+		             : "=&v"(a[0]), "=&v"(a[1]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]),
+		               "=&v"(b[5]), "=&v"(b[6]), "=&v"(b[7])
+		             : "v"(ab), "v"(bb)
+		             : "memory");
+	} else if constexpr (RT == 4 && CT == 6) {
+		asm volatile(RGPU_DS1(0, 10, 0) RGPU_DS1(1, 10, 1024) RGPU_DS1(2, 10, 2048) RGPU_DS1(3, 10, 3072)
+		             RGPU_DS1(4, 11, 0) RGPU_DS1(5, 11, 1024) RGPU_DS1(6, 11, 2048) RGPU_DS1(7, 11, 3072)
+		             RGPU_DS1(8, 11, 4096) RGPU_DS1(9, 11, 5120) "s_waitcnt lgkmcnt(0)"
+		             : "=&v"(a[0]), "=&v"(a[1]), "=&v"(a[2]), "=&v"(a[3]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]),
+		               "=&v"(b[3]), "=&v"(b[4]), "=&v"(b[5])
+		             : "v"(ab), "v"(bb)
+		             : "memory");
+	} else if constexpr (RT == 4 && CT == 4) {
+		asm volatile(RGPU_DS1(0, 8, 0) RGPU_DS1(1, 8, 1024) RGPU_DS1(2, 8, 2048) RGPU_DS1(3, 8, 3072)
+		             RGPU_DS1(4, 9, 0) RGPU_DS1(5, 9, 1024) RGPU_DS1(6, 9, 2048) RGPU_DS1(7, 9, 3072)
+		             "s_waitcnt lgkmcnt(0)"
+		             : "=&v"(a[0]), "=&v"(a[1]), "=&v"(a[2]), "=&v"(a[3]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]),
+		               "=&v"(b[3])
+		             : "v"(ab), "v"(bb)
+		             : "memory");
+	} else {
+		static_assert(RT < 0, "no slot reader for this tile");
+	}
+}
+#undef RGPU_DS1
+
+template <typename E, int NCT, int RT, int WC>
 __global__ __launch_bounds__(512) void k_gemm_dma(const E *__restrict__ A, const E *__restrict__ XT,
                                                   float *__restrict__ part, int ld, int rp, int kchunk)
 {
 	typedef typename VecOf<E>::type V;
-	constexpr int RT = 2, W = 8, NS = 5;
-	constexpr int SLOT = (W * RT + CT) * 64;  // fragments (16 bytes) per ring slot
-	// B fragments per wave: waves w < BX load BW + 1, the others BW
-	constexpr int BW = CT / W, BX = CT % W;
-	static_assert(CT == 8 || CT == 12, "8 or 12 column tiles");
+	constexpr int NS = 5, W = 8, WR = W / WC, CT = NCT / WC;
+	constexpr int SLOT = (WR * RT + NCT) * 64;  // fragments (16 bytes) per ring slot
+	// DMAs per wave and k-block: RT / WC of its row group's A fragments, and
+	// BW of the B fragments, one more on waves w < BX
+	constexpr int AW = RT / WC, BW = NCT / W, BX = NCT % W;
+	static_assert(NCT == 8 || NCT == 12, "8 or 12 column tiles");
+	static_assert(WR * RT == 16 && AW * WC == RT && CT * WC == NCT, "256 rows per block, whole splits");
 	__shared__ V ring[NS * SLOT];
-	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-	const int rb0 = (blockIdx.x * W + w) * RT;
+	const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int wr = w % WR, wc = w / WR;  // this wave's row group and column group
+	const int rb0 = (blockIdx.x * WR + wr) * RT;
+	// waves 4-7 (the second wave on each SIMD) run half a k-block behind:
+	// their MFMAs on the previous k-block overlap the first four waves' LDS
+	// reads, and their reads the others' MFMAs
+	const bool late = w >= 4;
 	const int s = blockIdx.y, kb0 = s * (kchunk >> 5), kb1 = kb0 + (kchunk >> 5);
 	const int nkb = ld >> 5;
 	const V *Ap = (const V *)A + lane, *Bp = (const V *)XT + lane;
 	const bool extra = w < BX;
-	const int c0 = w * BW + (extra ? w : BX);  // this wave's first B fragment
+	const int c0 = w * BW + (extra ? w : BX);  // the first B fragment this wave loads
 	f4 acc[RT][CT];
 #pragma unroll
 	for (int r = 0; r < RT; r++)
@@ -179,71 +157,71 @@ __global__ __launch_bounds__(512) void k_gemm_dma(const E *__restrict__ A, const
 	auto issue = [&](int kb, int slot) {
 		const int k = min(kb, kb1 - 1);
 		V *base = ring + slot * SLOT;
+		// A streams once: non-temporal (0.389 -> 0.364 ms at 192 columns);
+		// B is read by every block and stays in L2
 #pragma unroll
-		for (int r = 0; r < RT; r++)
+		for (int r = wc * AW; r < (wc + 1) * AW; r++)
 			__builtin_amdgcn_global_load_lds((const void *)(Ap + ((size_t)(rb0 + r) * nkb + k) * 64),
-			                                 (__attribute__((address_space(3))) void *)(base + (w * RT + r) * 64), 16, 0, 0);
+			                                 (__attribute__((address_space(3))) void *)(base + (wr * RT + r) * 64), 16, 0,
+			                                 kCpolNT);
 #pragma unroll
 		for (int m = 0; m < BW; m++)
 			__builtin_amdgcn_global_load_lds((const void *)(Bp + ((size_t)(c0 + m) * nkb + k) * 64),
-			                                 (__attribute__((address_space(3))) void *)(base + (W * RT + c0 + m) * 64), 16, 0, 0);
-		if (extra)
+			                                 (__attribute__((address_space(3))) void *)(base + (WR * RT + c0 + m) * 64), 16, 0, 0);
+		if (BX && extra)
 			__builtin_amdgcn_global_load_lds((const void *)(Bp + ((size_t)(c0 + BW) * nkb + k) * 64),
-			                                 (__attribute__((address_space(3))) void *)(base + (W * RT + c0 + BW) * 64), 16, 0, 0);
+			                                 (__attribute__((address_space(3))) void *)(base + (WR * RT + c0 + BW) * 64), 16, 0, 0);
 	};
 	const unsigned lbase = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void *)ring + lane * 16;
 	for (int i = 0; i < NS - 1; i++) issue(kb0 + i, i);
 	int slot = 0;
-	auto step = [&](int kb) {
-		// this k-block's DMAs: NS - 2 k-blocks' DMAs (RT + BW, + 1 on the
-		// first BX waves) were issued after them
-		static_assert(NS == 5 && RT == 2 && W == 8, "the vmcnt bounds below are 3 (RT + BW [+ 1])");
-		if (CT == 12) {
-			if (extra) asm volatile("s_waitcnt vmcnt(12)" ::: "memory");
-			else asm volatile("s_waitcnt vmcnt(9)" ::: "memory");
-		} else {
-			asm volatile("s_waitcnt vmcnt(9)" ::: "memory");
-		}
-		__builtin_amdgcn_s_barrier();
-		// every wave is past its reads of the previous trip's slot: refill it
-		issue(kb + NS - 1, slot == 0 ? NS - 1 : slot - 1);
-		const unsigned ab = lbase + (unsigned)(slot * SLOT * 16 + w * RT * 1024);
-		const unsigned bb = lbase + (unsigned)(slot * SLOT * 16 + W * RT * 1024);
-		V a[RT], b[CT];
-		if constexpr (CT == 12) {
-			asm volatile(
-				"ds_read_b128 %0, %14\n ds_read_b128 %1, %14 offset:1024\n"
-				"ds_read_b128 %2, %15\n ds_read_b128 %3, %15 offset:1024\n"
-				"ds_read_b128 %4, %15 offset:2048\n ds_read_b128 %5, %15 offset:3072\n"
-				"ds_read_b128 %6, %15 offset:4096\n ds_read_b128 %7, %15 offset:5120\n"
-				"ds_read_b128 %8, %15 offset:6144\n ds_read_b128 %9, %15 offset:7168\n"
-				"ds_read_b128 %10, %15 offset:8192\n ds_read_b128 %11, %15 offset:9216\n"
-				"ds_read_b128 %12, %15 offset:10240\n ds_read_b128 %13, %15 offset:11264\n"
-				"s_waitcnt lgkmcnt(0)"
-				: "=&v"(a[0]), "=&v"(a[1]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]),
-				  "=&v"(b[5]), "=&v"(b[6]), "=&v"(b[7]), "=&v"(b[8]), "=&v"(b[9]), "=&v"(b[10]), "=&v"(b[11])
-				: "v"(ab), "v"(bb)
-				: "memory");
-		} else {
-			asm volatile(
-				"ds_read_b128 %0, %10\n ds_read_b128 %1, %10 offset:1024\n"
-				"ds_read_b128 %2, %11\n ds_read_b128 %3, %11 offset:1024\n"
-				"ds_read_b128 %4, %11 offset:2048\n ds_read_b128 %5, %11 offset:3072\n"
-				"ds_read_b128 %6, %11 offset:4096\n ds_read_b128 %7, %11 offset:5120\n"
-				"ds_read_b128 %8, %11 offset:6144\n ds_read_b128 %9, %11 offset:7168\n"
-				"s_waitcnt lgkmcnt(0)"
-				: "=&v"(a[0]), "=&v"(a[1]), "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]),
-				  "=&v"(b[5]), "=&v"(b[6]), "=&v"(b[7])
-				: "v"(ab), "v"(bb)
-				: "memory");
-		}
+	auto mma = [&](const V (&a)[RT], const V (&b)[CT]) {
 #pragma unroll
 		for (int r = 0; r < RT; r++)
 #pragma unroll
 			for (int c = 0; c < CT; c++) acc[r][c] = mfma(a[r], b[c], acc[r][c]);
+	};
+	// one k-block: its DMAs waited for, the barrier, the refill, then the
+	// wave's own reads and MFMAs (body)
+	auto step = [&](int kb, auto body) {
+		// this k-block's DMAs: NS - 2 k-blocks' DMAs were issued after them
+		constexpr int D = AW + BW;
+		if (BX && extra) wait_vmcnt<(NS - 2) * (D + 1)>();
+		else wait_vmcnt<(NS - 2) * D>();
+		__builtin_amdgcn_s_barrier();
+		// every wave is past its reads of the previous trip's slot: refill it
+		issue(kb + NS - 1, slot == 0 ? NS - 1 : slot - 1);
+		body(lbase + (unsigned)(slot * SLOT * 16 + wr * RT * 1024),
+		     lbase + (unsigned)(slot * SLOT * 16 + (WR * RT + wc * CT) * 1024));
 		slot = slot == NS - 1 ? 0 : slot + 1;
 	};
-	for (int kb = kb0; kb < kb1; kb++) step(kb);
+	if (!late) {
+		for (int kb = kb0; kb < kb1; kb++)
+			step(kb, [&](unsigned ab, unsigned bb) {
+				V a[RT], b[CT];
+				read_slot<RT, CT>(a, b, ab, bb);
+				mma(a, b);
+			});
+	} else {
+		// the previous k-block's MFMAs, then this one's reads, into the
+		// other of two fragment sets: no read overwrites an operand of an
+		// MFMA issued in the same trip
+		V a0[RT], b0[CT], a1[RT], b1[CT];
+		step(kb0, [&](unsigned ab, unsigned bb) { read_slot<RT, CT>(a0, b0, ab, bb); });
+		for (int kb = kb0 + 1; kb < kb1; kb += 2) {
+			step(kb, [&](unsigned ab, unsigned bb) {
+				mma(a0, b0);
+				read_slot<RT, CT>(a1, b1, ab, bb);
+			});
+			if (kb + 1 < kb1)
+				step(kb + 1, [&](unsigned ab, unsigned bb) {
+					mma(a1, b1);
+					read_slot<RT, CT>(a0, b0, ab, bb);
+				});
+		}
+		if ((kb1 - kb0) & 1) mma(a0, b0);  // the last k-block went to set 0
+		else mma(a1, b1);
+	}
 	asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // no DMA may outlive the block's LDS
 	const int lr = lane & 15;
 #pragma unroll
@@ -253,7 +231,7 @@ __global__ __launch_bounds__(512) void k_gemm_dma(const E *__restrict__ A, const
 #pragma unroll
 			for (int j = 0; j < 4; j++) {
 				const int row = (rb0 + r) * 16 + (lane >> 4) * 4 + j;
-				part[((size_t)s * ld + row) * rp + c * 16 + lr] = acc[r][c][j];
+				part[((size_t)s * ld + row) * rp + (wc * CT + c) * 16 + lr] = acc[r][c][j];
 			}
 }
 
@@ -668,9 +646,9 @@ int launch_gemm(int rt, int ct, const E *A, const E *XT, float *part, int ld, in
 	if (rt == 1) {
 		GR(1)
 	} else if (rt == 2 && ct == 8) {
-		hipLaunchKernelGGL((k_gemm_dma<E, 8>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
+		hipLaunchKernelGGL((k_gemm_dma<E, 8, 2, 1>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
 	} else if (rt == 2 && ct == 12) {
-		hipLaunchKernelGGL((k_gemm_dma<E, 12>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
+		hipLaunchKernelGGL((k_gemm_dma<E, 12, 2, 1>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
 	} else if (rt == 2) {
 		GR(2)
 	} else if (rt == 4 && ct <= 4) {
@@ -681,9 +659,9 @@ int launch_gemm(int rt, int ct, const E *A, const E *XT, float *part, int ld, in
 		default: return -1;
 		}
 	} else if (rt == 4 && ct == 8) {
-		hipLaunchKernelGGL((k_gemm_lds<E, 4, 8, 4>), grid, blk, 0, st, A, XT, part, ld, rp, kchunk);
+		hipLaunchKernelGGL((k_gemm_dma<E, 8, 4, 2>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
 	} else if (rt == 4 && ct == 12) {
-		hipLaunchKernelGGL((k_gemm_lds<E, 4, 12, 4>), grid, blk, 0, st, A, XT, part, ld, rp, kchunk);
+		hipLaunchKernelGGL((k_gemm_dma<E, 12, 4, 2>), dim3(ld / 256, S), dim3(512), 0, st, A, XT, part, ld, rp, kchunk);
 	} else {
 		return -1;
 	}
@@ -698,8 +676,8 @@ bool valid_tile(int rt, int ct, int S, int ld)
 	// four-wave LDS kernel (8 or 12 column tiles, rt = 4: U = 4 or 2), or 1
 	// for the eight-wave LDS DMA kernel (8 or 12 column tiles, rt = 2);
 	// rows per block: 64 rt, or 256 for the eight-wave kernel
-	const bool lds = ct >= 8 && rt == 4, dma = ct >= 8 && rt == 2;
-	const int u2 = dma ? 1 : lds ? (ct >= 12 ? 4 : 8) : 2 * (ct <= 4 ? 4 : 2);
+	const bool dma = ct >= 8 && (rt == 2 || rt == 4);
+	const int u2 = dma ? 1 : 2 * (ct <= 4 ? 4 : 2);
 	const int rows = dma ? 256 : 64 * rt;
 	return ld % rows == 0 && ld % (32 * S) == 0 && (ld / 32 / S) % u2 == 0 && ld / 32 / S >= 1;
 }

@@ -49,7 +49,7 @@ def gemm():
     e.set_scene(sc).prepare_host().upload()
     lin = radgpu.LinearSystem(n)
     print("form factors %.1f ms" % lin.from_scene(e), flush=True)
-    lin.set_tile(192, "f16", 2, 2)
+    lin.set_tile(192, "f16", 4, 2)
     print("gemm 192 f16 %.3f ms" % lin.gemm_ms(192, "f16", reps=5), flush=True)
     lin.close()
     e.close()
