@@ -504,8 +504,10 @@ def main():
     ap.add_argument("--no-e2e", action="store_true",
                     help="skip every run but the timed steps (so a kernel trace holds their launches only)")
     ap.add_argument("--no-graph", action="store_true",
-                    help="the timed steps launch every kernel from the host (-graph 0) instead of replaying "
-                         "captured graphs: the same kernels, for profilers that do not follow graph launches")
+                    help="the timed steps launch every kernel from the host (-graph 0, the engine's default) "
+                         "even where --graph is given")
+    ap.add_argument("--graph", action="store_true",
+                    help="the timed steps replay captured graphs of shooting iterations (-graph 1)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -542,7 +544,7 @@ def main():
         log("end-to-end solve (init .. .fin): %.3f s, %d iterations" % (e2e_s, iters))
 
     # the timed steps: no per-kernel events, no host wait inside a batch
-    eng = radgpu.Engine(*(OPTS + ["-quiet", "1"] + (["-graph", "0"] if args.no_graph else [])))
+    eng = radgpu.Engine(*(OPTS + ["-quiet", "1"] + (["-graph", "1"] if args.graph and not args.no_graph else [])))
     eng.set_scene(scene).prepare_host()
     rec, _ = eng.records()
     slots = len(rec)

@@ -61,7 +61,7 @@ struct Options {
 	int quiet = 0;
 	int batch = 32;         // shooting iterations queued per host check
 	int kernel_timing = 0;  // time every shoot kernel with HIP events
-	int graph = 1;          // replay shooting iterations from a captured HIP graph
+	int graph = 0;          // replay shooting iterations from a captured HIP graph
 	int neighbors_host = 0; // InitRadiosity's neighbour search on the host instead of the device
 	int octree_host = 0;    // BuildTree on the host instead of the device
 	int v3 = 0;             // the reference's V3 build: Delta-ff splitting live (config.h without DONT_PARTITION)

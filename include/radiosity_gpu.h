@@ -52,9 +52,9 @@ enum {
  *   -old_format 0|1   per-vertex output of the OLD_FORMAT build (rad.cpp:150-159)
  *   -batch N          shooting iterations queued between host checks (default 32)
  *   -kernel_timing 0|1  time every shoot kernel with HIP events (rad_timings)
- *   -graph 0|1        replay shooting iterations from a captured HIP graph
- *                     (default 1; 0 launches every kernel from the host, as
- *                     -kernel_timing 1 does; results are the same)
+ *   -graph 0|1        1: replay shooting iterations from a captured HIP
+ *                     graph; 0 (default): launch every kernel from the host,
+ *                     as -kernel_timing 1 does; results are the same
  *   -quiet 0|1        no log lines
  *   -neighbors device|host  where InitRadiosity's neighbour search runs (default device)
  *   -octree device|host     where BuildTree runs (default device)

@@ -362,17 +362,17 @@ def test_parallel_src_zero_shoots_nothing(inputs, tmp_path):
 
 @pytest.mark.parametrize("scene", ["c1", "hall_subset"])
 def test_graph_replays_equal_eager_launches(scene):
-    """The shooting loop replays captured graphs of 8 iterations between host
-    checks; with -kernel_timing 1 or -graph 0 it launches every kernel eagerly
-    instead.  Solves in pieces (graph replays, eager tails and host
-    synchronisations interleaved) give the same shots and records as the
-    eager solves."""
+    """With -graph 1 the shooting loop replays captured graphs of 8
+    iterations between host checks; by default (-graph 0), and with
+    -kernel_timing 1, it launches every kernel eagerly instead.  Solves in
+    pieces (graph replays, eager tails and host synchronisations interleaved)
+    give the same shots and records as the eager solves."""
     if scene == "c1":
         s, opts = scenes.room_scene(1), ["-converge", "100", "-triangle", "20000", "-quiet", "1"]
     else:
         s, opts = scenes.hall_scene(2, n_boxes=40), ["-converge", "175", "-triangle", "1000000", "-quiet", "1"]
     print("graph vs eager", scene, flush=True)
-    with radgpu.Engine(*(opts + ["-kernel_timing", "1"])) as a, radgpu.Engine(*opts) as b:
+    with radgpu.Engine(*(opts + ["-kernel_timing", "1"])) as a, radgpu.Engine(*(opts + ["-graph", "1"])) as b:
         a.set_scene(s).prepare_host().upload()
         b.set_scene(s).prepare_host().upload()
         na, ca = a.shoot()
