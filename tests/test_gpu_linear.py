@@ -137,7 +137,9 @@ def test_mfma_product_matches_fp64_of_rounded_operands(nrhs):
         X16 = X.astype(np.float32).astype(np.float16).astype(np.float64)
         for prec, Fr, Xr, sc in (("f16", F16, X16, s), ("bf16", _to_bf16(F), _to_bf16(X.astype(np.float32)), 1.0)):
             want = (Fr @ Xr) / sc
-            for rt, sk in ((1, 1), (2, 1), (2, 2), (4, 1), (4, 2), (1, 8)):
+            # split-K 8 at n = 700 (ld 768): 3 k-blocks per chunk, an odd
+            # count for the wide kernels' two fragment sets
+            for rt, sk in ((1, 1), (2, 1), (2, 2), (4, 1), (4, 2), (1, 8), (2, 8), (4, 8)):
                 try:
                     lin.set_tile(nrhs, prec, rt, sk)
                 except radgpu.RadError:
