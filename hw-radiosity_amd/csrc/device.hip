@@ -236,14 +236,15 @@ __device__ __forceinline__ void test_entry(const DScene &S, const float4 &a, con
 	const float a1 = dot3<F>(vx, vy, vz, a.x, a.y, a.z);
 	const float a2 = dot3<F>(a.w - px, b.x - py, b.y - pz, a.x, a.y, a.z);
 	const float fa1 = fabsf(a1);
-	const bool est = fa1 >= 0x1p-100f && fa1 <= 0x1p100f && fabsf(a2) <= 0x1p100f;
+	// & and | rather than && and ||: no branch per condition
+	const bool est = (fa1 >= 0x1p-100f) & (fa1 <= 0x1p100f) & (fabsf(a2) <= 0x1p100f);
 	const float ta = a2 * __builtin_amdgcn_rcpf(a1);
 	const float qx = __builtin_fmaf(ta, vx, px) - a.w, qy = __builtin_fmaf(ta, vy, py) - b.x,
 	            qz = __builtin_fmaf(ta, vz, pz) - b.y;
 	const float d2 = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx));
 	const float lim = __builtin_fmaf(b.w, 1.0f + 0x1p-11f, __builtin_fmaf(fabsf(ta), ck, cb));
-	const bool rej = est && (ta < S.thr_t_lo || ta > tmin_hi || d2 > lim * lim);
-	if (own || rej) return;
+	const bool rej = est & ((ta < S.thr_t_lo) | (ta > tmin_hi) | (d2 > lim * lim));
+	if (own | rej) return;
 	const float tt = a2 / a1;
 	// TriHitted: t < 0.0001 (in double) or t >= t_min rejects
 	if (tt < S.thr_t || tt >= tmin) return;
