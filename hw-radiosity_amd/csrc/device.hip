@@ -818,13 +818,14 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(DScene S, DRecv R, int nv, 
 // the same point and every lane has a ray.  RayHitted (raycast.cpp:335-360);
 // a ray that does not reach the source zeroes its form factor
 // (shade.cpp:124-126).  XCD-contiguous block order.  Two lanes share each
-// ray's leaf lists (each tests every second entry), or four where even that
-// would leave most of the machine's wave slots idle: measured on the hall
-// against one lane per ray (242 us) and four (239 us), two take 227 us, the
-// list loop's lanes being busier than the walk's doubled steps cost.
-__device__ __forceinline__ int lanes_per_ray(int nr)
+// ray's leaf lists (each tests every second entry), or four or eight where
+// fewer would leave most of the machine's wave slots idle: measured on the
+// hall against one lane per ray (242 us) and four (239 us), two take 227 us,
+// the list loop's lanes being busier than the walk's doubled steps cost; on
+// the 2120-slot room eight lanes take an iteration from 49.4 to 48.4 us.
+__host__ __device__ __forceinline__ int lanes_per_ray(int nr)
 {
-	return 4LL * nr <= kTraceSlots ? 4 : 2;
+	return 8LL * nr <= kTraceSlots ? 8 : 4LL * nr <= kTraceSlots ? 4 : 2;
 }
 
 template <bool F>
@@ -849,8 +850,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(DScene S, DRecv R, Loop *
 		const SrcPatch &sp = s.split ? s.half[h] : s.whole;
 		const float vx = sp.c[0] - px, vy = sp.c[1] - py, vz = sp.c[2] - pz;
 		Cnt w = {0, 0, 0, 0, 0, 0, 0};
-		const int hit = lpr == 4 ? walk<F, 4>(S, px, py, pz, vx, vy, vz, ban, w, sub)
-		                         : walk<F, 2>(S, px, py, pz, vx, vy, vz, ban, w, sub);
+		const int hit = lpr == 8   ? walk<F, 8>(S, px, py, pz, vx, vy, vz, ban, w, sub)
+		                : lpr == 4 ? walk<F, 4>(S, px, py, pz, vx, vy, vz, ban, w, sub)
+		                           : walk<F, 2>(S, px, py, pz, vx, vy, vz, ban, w, sub);
 		// every lane of a group counts its own tests; the rest once
 		c.tests = w.tests;
 		if (sub == 0) {
@@ -1254,11 +1256,11 @@ __global__ __launch_bounds__(kBlock) void k_cast(DScene S, const RayIn *rays, in
 template <bool F>
 void launch_cast(hipStream_t st, const DScene &S, const RayIn *rays, int n, int *hit, int *cells, int *tests)
 {
-	const int lpr = 4LL * n <= kTraceSlots ? 4 : 2LL * n <= kTraceSlots ? 2 : 1;
+	const int lpr = lanes_per_ray(n);
 	const dim3 g((unsigned)(((size_t)n * lpr + kBlock - 1) / kBlock)), b(kBlock);
-	if (lpr == 4) hipLaunchKernelGGL((k_cast<F, 4>), g, b, 0, st, S, rays, n, hit, cells, tests);
-	else if (lpr == 2) hipLaunchKernelGGL((k_cast<F, 2>), g, b, 0, st, S, rays, n, hit, cells, tests);
-	else hipLaunchKernelGGL((k_cast<F, 1>), g, b, 0, st, S, rays, n, hit, cells, tests);
+	if (lpr == 8) hipLaunchKernelGGL((k_cast<F, 8>), g, b, 0, st, S, rays, n, hit, cells, tests);
+	else if (lpr == 4) hipLaunchKernelGGL((k_cast<F, 4>), g, b, 0, st, S, rays, n, hit, cells, tests);
+	else hipLaunchKernelGGL((k_cast<F, 2>), g, b, 0, st, S, rays, n, hit, cells, tests);
 }
 
 // Scratch of rad_cast_static, the variant of rad_cast that allocates nothing:
