@@ -1978,7 +1978,23 @@ int DeviceEngine::warm(std::string &err)
 	hipLaunchKernelGGL(k_warm, dim3(1), dim3(64), 0, d_->stream, p);
 	hipError_t e = hipMemcpyAsync(p, h.data(), 4096, hipMemcpyHostToDevice, d_->stream);
 	if (e == hipSuccess) e = hipMemcpyAsync(h.data(), p, 4096, hipMemcpyDeviceToHost, d_->stream);
+	// k_trace keeps a few values in scratch: its first dispatch in a process
+	// has the runtime set up scratch memory (about 3 ms), paid here.  A loop
+	// state marked done makes it return at once.
+	Loop *lp = nullptr;
+	std::vector<unsigned char> zero(sizeof(Loop), 0);
+	reinterpret_cast<Loop *>(zero.data())->done = 1;
+	if (e == hipSuccess) e = hipMalloc(&lp, sizeof(Loop));
+	if (e == hipSuccess) e = hipMemcpyAsync(lp, zero.data(), sizeof(Loop), hipMemcpyHostToDevice, d_->stream);
+	if (e == hipSuccess) {
+		const DScene S{};
+		const DRecv R{};
+		hipLaunchKernelGGL(k_trace<true>, dim3(1), dim3(kTraceBlock), 0, d_->stream, S, R, lp, 0, nullptr, nullptr, nullptr);
+		hipLaunchKernelGGL(k_trace<false>, dim3(1), dim3(kTraceBlock), 0, d_->stream, S, R, lp, 0, nullptr, nullptr, nullptr);
+		e = hipGetLastError();
+	}
 	if (e == hipSuccess) e = hipStreamSynchronize(d_->stream);
+	(void)hipFree(lp);
 	(void)hipFree(p);
 	HIP_OK(e, "warm");
 	return 0;
