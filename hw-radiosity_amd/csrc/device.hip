@@ -49,6 +49,7 @@
 #include <cstdint>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "device.h"
@@ -69,10 +70,14 @@ constexpr int kTraceBlock = 64;
 // occupancy, x 64)
 constexpr long long kTraceSlots = 256LL * 4 * 6 * 64;
 constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
-constexpr int kGroups = 2;        // receiver groups per block
+// receiver groups per block of k_gen, k_compact and k_update: 2, or 1 below
+// kOneGroupSlots slots (two 2-group blocks per CU), where halving a block's
+// sequential work pays (the 2120-slot room: 48.6 -> 42.0 us per iteration;
+// the 253,728-slot hall: 0.4475 s with 2, 0.4530 s with 1)
+constexpr int kOneGroupSlots = 2 * 256 * 2 * 4 * 21;
 constexpr int kMinQueue = 8;      // iterations queued per host check at least (up to -batch)
 constexpr int kGraphIters = 8;    // shooting iterations per captured graph
-constexpr int kRecvPerBlock = kGroups * kWavesPerBlock * kRecvPerWave;
+template <int G> constexpr int kRecvPerBlock = G * kWavesPerBlock * kRecvPerWave;
 constexpr int kMaxSrc = 256;
 constexpr int kRayBuckets = 16;   // ray keys k_compact groups by
 constexpr int kMaxWalk = 1 << 24;
@@ -611,7 +616,7 @@ template <bool F> __global__ __launch_bounds__(1024) void k_topk(DRecv R, Loop *
 // block, 3 lanes (vertices) per receiver, 21 receivers per wave.  Blocks
 // are XCD-aware: the hardware deals consecutive blocks to different XCDs,
 // so each XCD is given a contiguous run of receivers.
-struct RecvMap {
+template <int G> struct RecvMap {
 	int vb, lane, wid, v, base;
 	__device__ RecvMap(int nblocks)
 	{
@@ -625,7 +630,7 @@ struct RecvMap {
 	}
 	__device__ int slot(int g) const
 	{
-		return ((vb * kGroups + g) * kWavesPerBlock + wid) * kRecvPerWave + lane / 3;
+		return ((vb * G + g) * kWavesPerBlock + wid) * kRecvPerWave + lane / 3;
 	}
 	__device__ bool valid(int g, int nslots, int nblocks) const
 	{
@@ -654,16 +659,16 @@ template <int W = kWavesPerBlock> __device__ void add_block_counters(const int (
 // rad.cpp:420-428) and CalFF up to the ray (shade.cpp:107-123) for each
 // (receiver, vertex, half) into ffbuf[2 * (3 * slot + vertex) + half]; 0
 // where no ray is cast.
-template <bool F>
+template <bool F, int G>
 __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigned char *live, float *ffbuf,
                                                 long long *bcnt, int nblocks, int *raycnt)
 {
 	if (L->done || k >= L->nsrc) return;
-	const RecvMap m(nblocks);
+	const RecvMap<G> m(nblocks);
 	const SrcDesc &s = L->src[k];
 	int ref_rays = 0, nray = 0;
 #pragma unroll 1
-	for (int g = 0; g < kGroups; g++) {
+	for (int g = 0; g < G; g++) {
 		const bool valid = m.valid(g, R.nslots, nblocks);
 		const int r = valid ? R.order[m.slot(g)] : -1;
 		bool lv;
@@ -703,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 		ffbuf[2 * sv + 1] = f1;
 		nray += (f0 != 0.0f) + (f1 != 0.0f);
 	}
-	// rays of this block's contiguous lane range [vb * 6 * kRecvPerBlock, +6 * kRecvPerBlock)
+	// rays of this block's contiguous lane range [vb * 6 * kRecvPerBlock<G>, +6 * kRecvPerBlock<G>)
 	__shared__ int wray[kWavesPerBlock];
 	for (int off = 32; off > 0; off >>= 1) nray += __shfl_xor(nray, off);
 	if ((threadIdx.x & 63) == 0) wray[threadIdx.x >> 6] = nray;
@@ -718,7 +723,7 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 }
 
 // The rays of source k as a dense list: block b (one per k_gen range) takes
-// positions q of [b * 6 * kRecvPerBlock, +6 * kRecvPerBlock), lane i = 6 *
+// positions q of [b * 6 * kRecvPerBlock<G>, +6 * kRecvPerBlock<G>), lane i = 6 *
 // order[q / 6] + q % 6, and places those whose form-factor prefix is
 // non-zero at offset sum(raycnt[0..b)), grouped by their origin's key (the
 // list length of the leaf holding the receiver vertex, in kRayBuckets
@@ -727,12 +732,13 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 // grouping the rays of one neighbourhood by that key makes each wave's 64
 // rays alike (less divergence: k_trace 272 -> 255 us on the hall); which
 // rays there are, and each one's result, do not depend on the order.
+template <int G>
 __global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float *ffbuf, const int *order, int nlanes,
                                                     const int *raycnt, int nblocks, const unsigned char *rkey,
                                                     int *rays)
 {
 	if (L->done || k >= L->nsrc) return;
-	constexpr int kRounds = (6 * kRecvPerBlock + kBlock - 1) / kBlock;
+	constexpr int kRounds = (6 * kRecvPerBlock<G> + kBlock - 1) / kBlock;
 	__shared__ int wsum[kWavesPerBlock];
 	__shared__ int slot[kRounds][kWavesPerBlock][kRayBuckets];  // rays per (round, wave, key), then their first slot
 	__shared__ int ktot[kRayBuckets], kfirst[kRayBuckets];
@@ -742,7 +748,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float 
 	for (int j = threadIdx.x; j < b; j += kBlock) off += raycnt[j];
 	for (int o = 32; o > 0; o >>= 1) off += __shfl_xor(off, o);
 	if (lane == 0) wsum[w] = off;
-	constexpr int chunk = 6 * kRecvPerBlock;
+	constexpr int chunk = 6 * kRecvPerBlock<G>;
 	const int i0 = b * chunk, i1 = min(nlanes, i0 + chunk);
 	const unsigned long long below = (1ull << lane) - 1;
 	int ri[kRounds], rk[kRounds];
@@ -878,20 +884,20 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
 // three vertex lanes combined by shuffles in the reference's order; after
 // the last source the next iteration's candidate key (listCanditate,
 // rad.cpp:387-404) by wave, block and one 64-bit atomicMax per block.
-template <bool F, bool FUSED_SELECT, bool LOWEST>
+template <bool F, bool FUSED_SELECT, bool LOWEST, int G>
 __global__ __launch_bounds__(kBlock) void k_update(DRecv R, Loop *L, int k, const unsigned char *live,
                                                    const float *ffbuf, float converge, long long *bcnt, int nblocks)
 {
 	if (L->done || k >= L->nsrc) return;
 	__shared__ unsigned long long wmax[kWavesPerBlock];
 	__shared__ int wn[kWavesPerBlock];
-	const RecvMap m(nblocks);
+	const RecvMap<G> m(nblocks);
 	const SrcDesc &s = L->src[k];
 	const bool last = k == L->nsrc - 1;
 	unsigned long long key = 0;
 	int recv = 0, ncand = 0;
 #pragma unroll 1
-	for (int g = 0; g < kGroups; g++) {
+	for (int g = 0; g < G; g++) {
 		const int r = m.slot(g);
 		const bool valid = m.valid(g, R.nslots, nblocks);
 		const bool lv = valid && live[r];
@@ -1877,6 +1883,7 @@ struct DeviceEngine::Impl {
 	DRecv R;
 	Loop *host_loop = nullptr;  // pinned mirror of the loop header
 	int nblocks = 0, n_orig = 0, trace_nblocks = 0;
+	int groups = 2;  // receiver groups per block (kOneGroupSlots)
 	bool fma = true, lowest = false;
 	int parallel_src = 1, batch = 32;
 	float converge = 1200.f;
@@ -2081,7 +2088,9 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	HIP_OK(hipGetLastError(), "pack launch");
 	// grid sizes first: every kernel that adds into bcnt indexes it by its
 	// hardware block index, so bcnt must cover the largest grid
-	d.nblocks = (n + kRecvPerBlock - 1) / kRecvPerBlock;
+	d.groups = n <= kOneGroupSlots ? 1 : 2;
+	const int rpb = d.groups == 1 ? kRecvPerBlock<1> : kRecvPerBlock<2>;
+	d.nblocks = (n + rpb - 1) / rpb;
 	// k_trace: two lanes for each of up to 6 rays per slot, or four lanes on
 	// up to kTraceSlots lanes
 	d.trace_nblocks = (int)std::max(((size_t)n * 12 + kTraceBlock - 1) / kTraceBlock, (size_t)(kTraceSlots / kTraceBlock));
@@ -2224,25 +2233,35 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 		const int nk = multi ? d.parallel_src : 1;
 		for (int k = 0; grid > 0 && k < nk; k++) {
 			const dim3 g(grid), blk(kBlock), tg(tgrid);
-			if (d.fma) hipLaunchKernelGGL(k_gen<true>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
-			else hipLaunchKernelGGL(k_gen<false>, g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
-			hipLaunchKernelGGL(k_compact, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, d.order.p, 6 * d.R.nslots, d.raycnt.p,
-			                   d.nblocks, d.rkey.p, d.rays.p);
+			// k_gen and k_compact for G receiver groups per block
+			auto gen = [&](auto gc) {
+				constexpr int G = decltype(gc)::value;
+				if (d.fma) hipLaunchKernelGGL((k_gen<true, G>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
+				else hipLaunchKernelGGL((k_gen<false, G>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
+				hipLaunchKernelGGL(k_compact<G>, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, d.order.p, 6 * d.R.nslots,
+				                   d.raycnt.p, d.nblocks, d.rkey.p, d.rays.p);
+			};
+			if (d.groups == 1) gen(std::integral_constant<int, 1>{});
+			else gen(std::integral_constant<int, 2>{});
 			if (ev >= 0 && k == 0) (void)hipEventRecord(d.kev[2 * ev], st);
 			if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
 			else hipLaunchKernelGGL(k_trace<false>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
 			if (ev >= 0 && k == 0) (void)hipEventRecord(d.kev[2 * ev + 1], st);
-#define UPD(F, FS, LO) hipLaunchKernelGGL((k_update<F, FS, LO>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.converge, d.bcnt.p, d.nblocks)
-			if (multi) { if (d.fma) UPD(true, false, false); else UPD(false, false, false); }
-			else if (d.lowest) { if (d.fma) UPD(true, true, true); else UPD(false, true, true); }
-			else { if (d.fma) UPD(true, true, false); else UPD(false, true, false); }
+			auto update = [&](auto gc) {
+				constexpr int G = decltype(gc)::value;
+#define UPD(F, FS, LO) hipLaunchKernelGGL((k_update<F, FS, LO, G>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.converge, d.bcnt.p, d.nblocks)
+				if (multi) { if (d.fma) UPD(true, false, false); else UPD(false, false, false); }
+				else if (d.lowest) { if (d.fma) UPD(true, true, true); else UPD(false, true, true); }
+				else { if (d.fma) UPD(true, true, false); else UPD(false, true, false); }
 #undef UPD
+			};
+			if (d.groups == 1) update(std::integral_constant<int, 1>{});
+			else update(std::integral_constant<int, 2>{});
 		}
 	};
-	// Without kernel timing, runs of kGraphIters iterations replay one
-	// captured graph (the kernels exit at once after convergence, and none
-	// of the launches is a memory set), so the loop is not bound by the
-	// host's launch rate on small scenes.
+	// With -graph 1 and without kernel timing, runs of kGraphIters
+	// iterations replay one captured graph (the kernels exit at once after
+	// convergence, and none of the launches is a memory set).
 	const bool replay = graphs && !timing;
 	if (replay && !d.graph_exec) {
 		HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
