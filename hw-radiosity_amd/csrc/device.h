@@ -75,7 +75,7 @@ public:
 	long long kernel_timed() const { return kernel_timed_; }
 	double cast_ms() const { return cast_ms_; }  // device time of the last cast kernel
 	bool timing = false;  // time each shoot kernel with events (for the roofline)
-	bool graphs = true;   // untimed shooting replays captured graphs of kGraphIters iterations
+	bool graphs = false;  // untimed shooting replays captured graphs of kGraphIters iterations (-graph 1)
 
 private:
 	DeviceEngine() = default;
