@@ -1422,12 +1422,38 @@ __global__ __launch_bounds__(kBlock) void k_v3_gen(DScene S, DRecv R, V3Dev V, L
 
 // glibc rand() (TYPE_3, degree 31, separation 3): the next output of the
 // state st[34] (the oldest word at st[k])
-__device__ __forceinline__ int rand_next(unsigned *st, int &k)
+// glibc rand() (TYPE_3): r[i] = r[i - 3] + r[i - 31] (mod 2^32), output
+// r[i] >> 1; the state is the last 34 values, the oldest at st[k].
+//
+// v3_draw: the next n (<= 3 kWave) values on one wave into seq[34 ..
+// 34 + n), after the state's 34 in order in seq[0, 34).  In a block of 31
+// consecutive values every r[i - 31] is already known, so within the block
+// r[b + t] = r[b + t % 3 - 3] + (the sum of known_u over u <= t, u = t mod
+// 3): three interleaved prefix sums, four shuffle steps.
+constexpr int kV3Seq = 34 + 3 * kWave + 31;
+__device__ __forceinline__ void v3_draw(const unsigned *st, int sk, unsigned *seq, int n, int lane)
 {
-	const unsigned v = st[(k + 3) % 34] + st[(k + 31) % 34];
-	st[k] = v;
-	k = (k + 1) % 34;
-	return (int)(v >> 1);
+	if (lane < 34) seq[lane] = st[(sk + lane) % 34];
+	__syncthreads();
+	for (int b = 34; b < 34 + n; b += 31) {
+		unsigned x = lane < 31 ? seq[b + lane - 31] : 0u;
+		for (int d = 3; d < 31; d <<= 1) {  // 3, 6, 12, 24: within a residue class
+			const unsigned y = __shfl_up(x, d);
+			if (lane >= d) x += y;
+		}
+		if (lane < 31) x += seq[b + lane % 3 - 3];
+		__syncthreads();
+		if (lane < 31) seq[b + lane] = x;
+		__syncthreads();
+	}
+}
+// the state after the first n values of seq's draws (k = 0: oldest first)
+__device__ __forceinline__ void v3_advance(unsigned *st, int &sk, const unsigned *seq, int n, int lane)
+{
+	__syncthreads();
+	if (lane < 34) st[lane] = seq[n + lane];
+	if (lane == 0) sk = 0;
+	__syncthreads();
 }
 
 template <bool F> __device__ void v3_center(float p[3][3], float *c)
@@ -1453,9 +1479,8 @@ template <bool F> __device__ void v3_area(Patch &t)
 template <bool F>
 __global__ __launch_bounds__(kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V, Loop *L, int r0, int r1, long long *bcnt)
 {
-	__shared__ unsigned st[34], snap[34];
-	__shared__ int outs[3 * kWave];
-	__shared__ int sk, snap_k;
+	__shared__ unsigned st[34], seq[kV3Seq];
+	__shared__ int sk;
 	const int lane = threadIdx.x;
 	Cnt c = {0, 0, 0, 0, 0, 0, 0};
 	if (L->done || L->v3_error) return;
@@ -1499,19 +1524,12 @@ __global__ __launch_bounds__(kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V,
 				}
 				for (int base = 0; base < samples; base += kWave) {
 					const int m = min(kWave, samples - base);
-					if (lane < 34) snap[lane] = st[lane];
-					__syncthreads();
-					if (lane == 0) {
-						snap_k = sk;
-						int k = sk;
-						for (int j = 0; j < 3 * m; j++) outs[j] = rand_next(st, k);
-						sk = k;
-					}
-					__syncthreads();
+					v3_draw(st, sk, seq, 3 * m, lane);
 					bool brk = false;
 					float mydff = 0.0f;
 					if (lane < m) {
-						float w1 = (float)outs[3 * lane], w2 = (float)outs[3 * lane + 1], w3 = (float)outs[3 * lane + 2];
+						const unsigned *o = seq + 34 + 3 * lane;
+						float w1 = (float)(int)(o[0] >> 1), w2 = (float)(int)(o[1] >> 1), w3 = (float)(int)(o[2] >> 1);
 						const float tot = w1 + w2 + w3;
 						w1 /= tot;
 						w2 /= tot;
@@ -1529,19 +1547,11 @@ __global__ __launch_bounds__(kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V,
 						const int j = __ffsll((long long)bal) - 1;
 						dff = __shfl(mydff, j);
 						// the state after the 3 (j + 1) draws the reference made
-						__syncthreads();
-						if (lane < 34) st[lane] = snap[lane];
-						__syncthreads();
-						if (lane == 0) {
-							int k = snap_k;
-							for (int t = 0; t < 3 * (j + 1); t++) (void)rand_next(st, k);
-							sk = k;
-						}
-						__syncthreads();
+						v3_advance(st, sk, seq, 3 * (j + 1), lane);
 						break;
 					}
 					dff = __shfl(mydff, m - 1);
-					__syncthreads();
+					v3_advance(st, sk, seq, 3 * m, lane);
 				}
 				if (dff < V.delta_ff) continue;  // CalRadiosity with the vertices' form factors
 			}
