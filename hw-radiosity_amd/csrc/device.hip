@@ -1344,6 +1344,42 @@ __device__ float adapt_ff(const DScene &S, const SrcDesc &s, float nx, float ny,
 	return f1 + f2;
 }
 
+// adapt_ff on the four lanes q = 0..3 of an aligned group: lanes 0-1 walk
+// the first half (or the whole source), lanes 2-3 the second half, two
+// lanes sharing each ray's lists (walk<F, 2>); every lane of the group
+// returns the form factor, the halves summed in adapt_ff's order.  Tests
+// are counted by every lane, the rest once per ray (its first lane).  The
+// four lanes call it together.
+template <bool F>
+__device__ float adapt_ff4(const DScene &S, const SrcDesc &s, float nx, float ny, float nz, float nlen, int logdst,
+                           float px, float py, float pz, int q, Cnt &c)
+{
+	const int h = q >> 1, sub = q & 1;
+	float f = 0.0f;
+	if (s.split || h == 0) {
+		const SrcPatch &sp = s.split ? s.half[h] : s.whole;
+		if (s.split && q == 0 && ff_prefix<F>(s.whole, nx, ny, nz, nlen, px, py, pz) != 0.0f) c.ref_rays++;
+		const float ff = ff_prefix<F>(sp, nx, ny, nz, nlen, px, py, pz);
+		if (ff != 0.0f) {
+			Cnt w = {0, 0, 0, 0, 0, 0, 0};
+			const int hit = walk<F, 2>(S, px, py, pz, sp.c[0] - px, sp.c[1] - py, sp.c[2] - pz, logdst, w, sub);
+			c.tests += w.tests;
+			if (sub == 0) {
+				c.rays++;
+				c.ref_rays++;
+				c.cells += w.cells;
+				c.levels += w.levels;
+				c.precision += w.precision;
+				c.stalled += w.stalled;
+			}
+			f = hit == s.log ? ff : 0.0f;
+		}
+	}
+	const int g0 = (int)(threadIdx.x & 63) & ~3;
+	const float fa = __shfl(f, g0), fb = __shfl(f, g0 + 2);
+	return s.split ? fa + fb : fa;
+}
+
 __device__ __forceinline__ float ffabs(float x) { return (float)fabs((double)x); }
 
 // Slot r as the reference holds it: static fields from the record, the rest
@@ -1425,34 +1461,34 @@ __global__ __launch_bounds__(kBlock) void k_v3_gen(DScene S, DRecv R, V3Dev V, L
 // glibc rand() (TYPE_3): r[i] = r[i - 3] + r[i - 31] (mod 2^32), output
 // r[i] >> 1; the state is the last 34 values, the oldest at st[k].
 //
-// v3_draw: the next n (<= 3 kWave) values on one wave into seq[34 ..
-// 34 + n), after the state's 34 in order in seq[0, 34).  In a block of 31
-// consecutive values every r[i - 31] is already known, so within the block
-// r[b + t] = r[b + t % 3 - 3] + (the sum of known_u over u <= t, u = t mod
-// 3): three interleaved prefix sums, four shuffle steps.
+// v3_draw: the next n (<= 3 kWave) values, drawn by the block's first wave
+// into seq[34 .. 34 + n), after the state's 34 in order in seq[0, 34).  In
+// a block of 31 consecutive values every r[i - 31] is already known, so
+// within the block r[b + t] = r[b + t % 3 - 3] + (the sum of known_u over
+// u <= t, u = t mod 3): three interleaved prefix sums, four shuffle steps.
 constexpr int kV3Seq = 34 + 3 * kWave + 31;
-__device__ __forceinline__ void v3_draw(const unsigned *st, int sk, unsigned *seq, int n, int lane)
+__device__ __forceinline__ void v3_draw(const unsigned *st, int sk, unsigned *seq, int n, int tid)
 {
-	if (lane < 34) seq[lane] = st[(sk + lane) % 34];
-	__syncthreads();
-	for (int b = 34; b < 34 + n; b += 31) {
-		unsigned x = lane < 31 ? seq[b + lane - 31] : 0u;
-		for (int d = 3; d < 31; d <<= 1) {  // 3, 6, 12, 24: within a residue class
-			const unsigned y = __shfl_up(x, d);
-			if (lane >= d) x += y;
+	if (tid < kWave) {  // one wave: its LDS accesses stay in order
+		const int lane = tid;
+		if (lane < 34) seq[lane] = st[(sk + lane) % 34];
+		for (int b = 34; b < 34 + n; b += 31) {
+			unsigned x = lane < 31 ? seq[b + lane - 31] : 0u;
+			for (int d = 3; d < 31; d <<= 1) {  // 3, 6, 12, 24: within a residue class
+				const unsigned y = __shfl_up(x, d);
+				if (lane >= d) x += y;
+			}
+			if (lane < 31) seq[b + lane] = x + seq[b + lane % 3 - 3];
 		}
-		if (lane < 31) x += seq[b + lane % 3 - 3];
-		__syncthreads();
-		if (lane < 31) seq[b + lane] = x;
-		__syncthreads();
 	}
+	__syncthreads();
 }
 // the state after the first n values of seq's draws (k = 0: oldest first)
-__device__ __forceinline__ void v3_advance(unsigned *st, int &sk, const unsigned *seq, int n, int lane)
+__device__ __forceinline__ void v3_advance(unsigned *st, int &sk, const unsigned *seq, int n, int tid)
 {
 	__syncthreads();
-	if (lane < 34) st[lane] = seq[n + lane];
-	if (lane == 0) sk = 0;
+	if (tid < 34) st[tid] = seq[n + tid];
+	if (tid == 0) sk = 0;
 	__syncthreads();
 }
 
@@ -1475,17 +1511,24 @@ template <bool F> __device__ void v3_area(Patch &t)
 	t.area = __builtin_sqrtf(dot3<F>(t.n[0], t.n[1], t.n[2], t.n[0], t.n[1], t.n[2])) / 2;
 }
 
-// The receivers [r0, r1) in order, one wave (see above).
+// The receivers [r0, r1) in order (see above) on one block of four waves:
+// every wave follows the same decisions; a chunk's 64 sample points take
+// four lanes each (adapt_ff4), the first wave draws the random numbers,
+// thread 0 writes the splits.
 template <bool F>
-__global__ __launch_bounds__(kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V, Loop *L, int r0, int r1, long long *bcnt)
+__global__ __launch_bounds__(4 * kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V, Loop *L, int r0, int r1,
+                                                         long long *bcnt)
 {
 	__shared__ unsigned st[34], seq[kV3Seq];
 	__shared__ int sk;
-	const int lane = threadIdx.x;
+	__shared__ unsigned long long pmask[4];  // each wave's departing points (bit 4k: its point k)
+	__shared__ float pdff[kWave];            // each point's |g - fs|
+	const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+	const int pt = tid >> 2, q = tid & 3;  // a chunk's point and the lane in its group of four
 	Cnt c = {0, 0, 0, 0, 0, 0, 0};
 	if (L->done || L->v3_error) return;
-	if (lane < 34) st[lane] = L->rng[lane];
-	if (lane == 0) sk = L->rng_k;
+	if (tid < 34) st[tid] = L->rng[tid];
+	if (tid == 0) sk = L->rng_k;
 	__syncthreads();
 	const SrcDesc &s = L->src[0];
 	const int limit = V.cap;
@@ -1524,11 +1567,10 @@ __global__ __launch_bounds__(kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V,
 				}
 				for (int base = 0; base < samples; base += kWave) {
 					const int m = min(kWave, samples - base);
-					v3_draw(st, sk, seq, 3 * m, lane);
+					v3_draw(st, sk, seq, 3 * m, tid);
 					bool brk = false;
-					float mydff = 0.0f;
-					if (lane < m) {
-						const unsigned *o = seq + 34 + 3 * lane;
+					if (pt < m) {
+						const unsigned *o = seq + 34 + 3 * pt;
 						float w1 = (float)(int)(o[0] >> 1), w2 = (float)(int)(o[1] >> 1), w3 = (float)(int)(o[2] >> 1);
 						const float tot = w1 + w2 + w3;
 						w1 /= tot;
@@ -1538,20 +1580,28 @@ __global__ __launch_bounds__(kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V,
 						const float qy = dot3<F>(w1, w2, w3, P[0][1], P[1][1], P[2][1]);
 						const float qz = dot3<F>(w1, w2, w3, P[0][2], P[1][2], P[2][2]);
 						const float g = dot3<F>(w1, w2, w3, f0, f1, f2);
-						const float fs = adapt_ff<F>(S, s, nr.x, nr.y, nr.z, nlen, logdst, qx, qy, qz, c);
-						mydff = ffabs(g - fs);
+						const float fs = adapt_ff4<F>(S, s, nr.x, nr.y, nr.z, nlen, logdst, qx, qy, qz, q, c);
+						const float mydff = ffabs(g - fs);
 						brk = mydff > V.delta_ff;
+						if (q == 0) pdff[pt] = mydff;
 					}
-					const unsigned long long bal = __ballot(brk);
+					const unsigned long long bw = __ballot(brk && q == 0);
+					if (lane == 0) pmask[wv] = bw;
+					__syncthreads();
+					unsigned long long bal = 0;  // bit p: point p departs
+					for (int w = 0; w < 4; w++) {
+						const unsigned long long x = pmask[w];
+						for (int k = 0; k < 16; k++) bal |= ((x >> (4 * k)) & 1ull) << (16 * w + k);
+					}
 					if (bal) {
 						const int j = __ffsll((long long)bal) - 1;
-						dff = __shfl(mydff, j);
+						dff = pdff[j];
 						// the state after the 3 (j + 1) draws the reference made
-						v3_advance(st, sk, seq, 3 * (j + 1), lane);
+						v3_advance(st, sk, seq, 3 * (j + 1), tid);
 						break;
 					}
-					dff = __shfl(mydff, m - 1);
-					v3_advance(st, sk, seq, 3 * m, lane);
+					dff = pdff[m - 1];
+					v3_advance(st, sk, seq, 3 * m, tid);
 				}
 				if (dff < V.delta_ff) continue;  // CalRadiosity with the vertices' form factors
 			}
@@ -1572,7 +1622,7 @@ __global__ __launch_bounds__(kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V,
 				// the split would rewrite an original the octree lists (a slot
 				// pre-partition overwrote): the rays of the later receivers
 				// would change mid-pass; not supported
-				if (lane == 0) L->v3_error = r + 1;
+				if (tid == 0) L->v3_error = r + 1;
 				stop = true;
 				break;
 			}
@@ -1587,13 +1637,13 @@ __global__ __launch_bounds__(kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V,
 			t[0].neighbor[0] = id[2], t[0].neighbor[2] = id[1];
 			t[1].neighbor[1] = id[0], t[1].neighbor[0] = id[2];
 			t[2].neighbor[2] = id[1], t[2].neighbor[1] = id[0];
-			if (lane == 0) {
+			if (tid == 0) {
 				v3_store(V, R, id[0], t[0]);
 				v3_store(V, R, id[1], t[1]);
 			}
 			if (src.parent == r) {
 				// an original becomes logical; its three parts are shaded next pass
-				if (lane == 0) {
+				if (tid == 0) {
 					v3_store(V, R, id[2], t[2]);
 					src.parent = -1;
 					v3_store(V, R, r, src);
@@ -1603,26 +1653,27 @@ __global__ __launch_bounds__(kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V,
 			} else {
 				// slot r holds the third part (the last slot is given back), its
 				// centroid corner shaded anew (shade.cpp:508-520)
-				if (lane == 0) v3_store(V, R, r, t[2]);
+				if (tid == 0) v3_store(V, R, r, t[2]);
 				trinum += 2;
 				const Patch &t3 = t[2];
 				const float nlen = __builtin_sqrtf(dot3<F>(t3.n[0], t3.n[1], t3.n[2], t3.n[0], t3.n[1], t3.n[2]));
 				float f3 = 0.0f;
-				if (lane == 0)
-					f3 = adapt_ff<F>(S, s, t3.n[0], t3.n[1], t3.n[2], nlen, src.parent, t3.p[2][0], t3.p[2][1], t3.p[2][2], c);
-				if (lane == 0) V.ff[4 * (size_t)r + 2] = f3;
+				if (tid < 4)
+					f3 = adapt_ff4<F>(S, s, t3.n[0], t3.n[1], t3.n[2], nlen, src.parent, t3.p[2][0], t3.p[2][1], t3.p[2][2],
+					                  q, c);
+				if (tid == 0) V.ff[4 * (size_t)r + 2] = f3;
 			}
 			__syncthreads();
 		}
 	}
 	__syncthreads();
-	if (lane < 34) L->rng[lane] = st[lane];
-	if (lane == 0) {
+	if (tid < 34) L->rng[tid] = st[tid];
+	if (tid == 0) {
 		L->rng_k = sk;
 		L->trinum = trinum;
 	}
 	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
-	add_block_counters<1>(vals, bcnt);
+	add_block_counters<4>(vals, bcnt);
 }
 
 // CalRadiosity (shade.cpp:244-259) for the receivers [r0, r1) that keep
@@ -2182,10 +2233,10 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 				const unsigned ga = (unsigned)((n - prev + kBlock - 1) / kBlock);
 				if (d.fma) {
 					hipLaunchKernelGGL(k_v3_gen<true>, dim3(g), dim3(kBlock), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
-					hipLaunchKernelGGL(k_v3_commit<true>, dim3(1), dim3(kWave), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
+					hipLaunchKernelGGL(k_v3_commit<true>, dim3(1), dim3(4 * kWave), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
 				} else {
 					hipLaunchKernelGGL(k_v3_gen<false>, dim3(g), dim3(kBlock), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
-					hipLaunchKernelGGL(k_v3_commit<false>, dim3(1), dim3(kWave), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
+					hipLaunchKernelGGL(k_v3_commit<false>, dim3(1), dim3(4 * kWave), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
 				}
 				hipLaunchKernelGGL(k_v3_apply, dim3(ga), dim3(kBlock), 0, st, d.R, d.V, d.loop.p, prev, n);
 				HIP_OK(hipGetLastError(), "v3 launch");
