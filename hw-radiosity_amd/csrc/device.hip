@@ -1315,7 +1315,7 @@ __global__ __launch_bounds__(kBlock) void k_ff_pairs(DScene S, DRecv R, const in
 // three (PartitionTriangleWithCenterAndStore, shade.cpp:261-306); the new
 // slots are shaded in further passes (rad.cpp:406-451).  On the device:
 // k_v3_gen computes every receiver's form factors (its vertices and centroid)
-// in parallel; k_v3_commit walks the receivers in order on one wave, keeping
+// in parallel; k_v3_commit walks the receivers in order on one block, keeping
 // the pool count and the rand() state, drawing each receiver's samples 64 at
 // a time across the lanes, and writing the splits; k_v3_apply then runs
 // CalRadiosity (shade.cpp:244-259) for the receivers that keep their form
@@ -1332,22 +1332,13 @@ struct V3Dev {
 	float adapt_area, sample_area, delta_ff;
 };
 
-// AdaptCalFF(CalFF(..)) at a point of a receiver with normal n (shade.cpp:141-168)
-template <bool F>
-__device__ float adapt_ff(const DScene &S, const SrcDesc &s, float nx, float ny, float nz, float nlen, int logdst,
-                          float px, float py, float pz, Cnt &c)
-{
-	if (!s.split) return point_ff<F>(S, s.whole, s.log, nx, ny, nz, nlen, logdst, px, py, pz, c);
-	if (ff_prefix<F>(s.whole, nx, ny, nz, nlen, px, py, pz) != 0.0f) c.ref_rays++;  // computed and discarded
-	const float f1 = point_ff<F>(S, s.half[0], s.log, nx, ny, nz, nlen, logdst, px, py, pz, c);
-	const float f2 = point_ff<F>(S, s.half[1], s.log, nx, ny, nz, nlen, logdst, px, py, pz, c);
-	return f1 + f2;
-}
-
-// adapt_ff on the four lanes q = 0..3 of an aligned group: lanes 0-1 walk
-// the first half (or the whole source), lanes 2-3 the second half, two
-// lanes sharing each ray's lists (walk<F, 2>); every lane of the group
-// returns the form factor, the halves summed in adapt_ff's order.  Tests
+// AdaptCalFF(CalFF(..)) at a point of a receiver with normal n
+// (shade.cpp:141-168): the whole source's form factor, or, for a split
+// source, the discarded whole one and then the sum of the two halves'.  On
+// the four lanes q = 0..3 of an aligned group: lanes 0-1 walk the first half
+// (or the whole source), lanes 2-3 the second half, two lanes sharing each
+// ray's lists (walk<F, 2>); every lane of the group returns the form
+// factor, the halves summed in AdaptCalFF's order.  Tests
 // are counted by every lane, the rest once per ray (its first lane).  The
 // four lanes call it together.
 template <bool F>
@@ -1421,8 +1412,9 @@ template <bool F>
 __global__ __launch_bounds__(kBlock) void k_v3_gen(DScene S, DRecv R, V3Dev V, Loop *L, int r0, int r1, long long *bcnt)
 {
 	Cnt c = {0, 0, 0, 0, 0, 0, 0};
-	const int q = blockIdx.x * kBlock + threadIdx.x;
-	const int r = r0 + q / 4, w = q % 4;
+	// four lanes per form factor (adapt_ff4), four form factors per receiver
+	const int t = blockIdx.x * kBlock + threadIdx.x;
+	const int r = r0 + t / 16, w = (t / 4) % 4, g4 = t % 4;
 	if (!L->done && r < r1) {
 		const SrcDesc &s = L->src[0];
 		const float4 nr = R.nrm[r];
@@ -1433,7 +1425,7 @@ __global__ __launch_bounds__(kBlock) void k_v3_gen(DScene S, DRecv R, V3Dev V, L
 			sat = sat && !(a.x < 255.0f || a.y < 255.0f || a.z < 255.0f);
 		}
 		const bool live = parent >= 0 && r != s.phys && !sat;
-		if (w == 0) V.act[r] = live ? V3_APPLY : V3_NONE;
+		if (w == 0 && g4 == 0) V.act[r] = live ? V3_APPLY : V3_NONE;
 		float f = 0.0f;
 		const float4 ca = R.cA[r];
 		if (live && (w < 3 || ca.w >= V.adapt_area)) {
@@ -1448,16 +1440,14 @@ __global__ __launch_bounds__(kBlock) void k_v3_gen(DScene S, DRecv R, V3Dev V, L
 				py = ca.y;
 				pz = ca.z;
 			}
-			f = adapt_ff<F>(S, s, nr.x, nr.y, nr.z, nlen, parent, px, py, pz, c);
+			f = adapt_ff4<F>(S, s, nr.x, nr.y, nr.z, nlen, parent, px, py, pz, g4, c);
 		}
-		V.ff[4 * (size_t)r + w] = f;
+		if (g4 == 0) V.ff[4 * (size_t)r + w] = f;
 	}
 	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
 	add_block_counters(vals, bcnt);
 }
 
-// glibc rand() (TYPE_3, degree 31, separation 3): the next output of the
-// state st[34] (the oldest word at st[k])
 // glibc rand() (TYPE_3): r[i] = r[i - 3] + r[i - 31] (mod 2^32), output
 // r[i] >> 1; the state is the last 34 values, the oldest at st[k].
 //
@@ -2229,7 +2219,7 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 			if (d.host_loop->done) break;
 			int prev = 0, n = d.host_loop->trinum;
 			while (prev < n) {
-				const unsigned g = (unsigned)(((size_t)(n - prev) * 4 + kBlock - 1) / kBlock);
+				const unsigned g = (unsigned)(((size_t)(n - prev) * 16 + kBlock - 1) / kBlock);
 				const unsigned ga = (unsigned)((n - prev + kBlock - 1) / kBlock);
 				if (d.fma) {
 					hipLaunchKernelGGL(k_v3_gen<true>, dim3(g), dim3(kBlock), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
