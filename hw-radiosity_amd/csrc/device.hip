@@ -77,6 +77,7 @@ constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
 constexpr int kOneGroupSlots = 2 * 256 * 2 * 4 * 21;
 constexpr int kMinQueue = 8;      // iterations queued per host check at least (up to -batch)
 constexpr int kGraphIters = 8;    // shooting iterations per captured graph
+constexpr int kV3Batch = 32;      // -variant v3: iteration slots queued per host check
 template <int G> constexpr int kRecvPerBlock = G * kWavesPerBlock * kRecvPerWave;
 constexpr int kMaxSrc = 256;
 constexpr int kRayBuckets = 16;   // ray keys k_compact groups by
@@ -140,8 +141,10 @@ struct Loop {
 	int trinum;
 	int v3_error;
 	int rng_k;
+	int v3_pending;  // the last pass made new slots: another pass before the selection
 	unsigned rng[34];
 	SrcDesc src[kMaxSrc];
+	int v3_range[2][2];  // V3: the slots [r0, r1) of the pass of each parity
 };
 
 struct Cnt {
@@ -555,6 +558,28 @@ template <bool F, bool LOWEST> __global__ void k_prep(DRecv R, Loop *L)
 	L->nsrc = 1;
 	L->shots++;
 	L->ncand = 0;
+}
+
+// k_prep for -variant v3: nothing while a split is still being shaded
+// (v3_pending: the pass of parity par continues it); else the next source
+// and a first pass over every slot in use
+template <bool F> __global__ void k_v3_prep(DRecv R, Loop *L, int par)
+{
+	if (threadIdx.x != 0 || L->done || L->v3_error || L->v3_pending) return;
+	const unsigned long long key = L->key;
+	L->key = 0;
+	if (key == 0) {
+		L->done = 1;
+		return;
+	}
+	const int idx = key_slot<false>(key);
+	make_source<F>(R, idx, L->src[0]);
+	R.dB[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+	L->nsrc = 1;
+	L->shots++;
+	L->ncand = 0;
+	L->v3_range[par][0] = 0;
+	L->v3_range[par][1] = L->trinum;
 }
 
 // Multi-source selection (-parallel_src k > 1): collect candidate keys, then
@@ -1409,13 +1434,14 @@ __device__ void v3_store(const V3Dev &V, const DRecv &R, int r, const Patch &t)
 // (shade.cpp:469-470); act = V3_APPLY for each receiver the reference
 // shades (parent >= 0, not the source, not saturated: rad.cpp:419-428).
 template <bool F>
-__global__ __launch_bounds__(kBlock) void k_v3_gen(DScene S, DRecv R, V3Dev V, Loop *L, int r0, int r1, long long *bcnt)
+__global__ __launch_bounds__(kBlock) void k_v3_gen(DScene S, DRecv R, V3Dev V, Loop *L, int par, long long *bcnt)
 {
 	Cnt c = {0, 0, 0, 0, 0, 0, 0};
-	// four lanes per form factor (adapt_ff4), four form factors per receiver
-	const int t = blockIdx.x * kBlock + threadIdx.x;
-	const int r = r0 + t / 16, w = (t / 4) % 4, g4 = t % 4;
-	if (!L->done && r < r1) {
+	const int r0 = L->v3_range[par][0], r1 = L->v3_range[par][1];
+	// four lanes per form factor (adapt_ff4), four form factors per receiver;
+	// the grid strides over the pass's slots
+	for (int t = blockIdx.x * kBlock + threadIdx.x; !L->done && t < 16 * (r1 - r0); t += gridDim.x * kBlock) {
+		const int r = r0 + t / 16, w = (t / 4) % 4, g4 = t % 4;
 		const SrcDesc &s = L->src[0];
 		const float4 nr = R.nrm[r];
 		const int parent = __float_as_int(nr.w);
@@ -1506,7 +1532,7 @@ template <bool F> __device__ void v3_area(Patch &t)
 // four lanes each (adapt_ff4), the first wave draws the random numbers,
 // thread 0 writes the splits.
 template <bool F>
-__global__ __launch_bounds__(4 * kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V, Loop *L, int r0, int r1,
+__global__ __launch_bounds__(4 * kWave) void k_v3_commit(DScene S, DRecv R, V3Dev V, Loop *L, int par,
                                                          long long *bcnt)
 {
 	__shared__ unsigned st[34], seq[kV3Seq];
@@ -1517,6 +1543,7 @@ __global__ __launch_bounds__(4 * kWave) void k_v3_commit(DScene S, DRecv R, V3De
 	const int pt = tid >> 2, q = tid & 3;  // a chunk's point and the lane in its group of four
 	Cnt c = {0, 0, 0, 0, 0, 0, 0};
 	if (L->done || L->v3_error) return;
+	const int r0 = L->v3_range[par][0], r1 = L->v3_range[par][1];
 	if (tid < 34) st[tid] = L->rng[tid];
 	if (tid == 0) sk = L->rng_k;
 	__syncthreads();
@@ -1661,18 +1688,17 @@ __global__ __launch_bounds__(4 * kWave) void k_v3_commit(DScene S, DRecv R, V3De
 	if (tid == 0) {
 		L->rng_k = sk;
 		L->trinum = trinum;
+		// the next pass: the slots this one made
+		L->v3_range[par ^ 1][0] = r1;
+		L->v3_range[par ^ 1][1] = trinum;
 	}
 	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
 	add_block_counters<4>(vals, bcnt);
 }
 
-// CalRadiosity (shade.cpp:244-259) for the receivers [r0, r1) that keep
-// their form factors, one lane each, in the reference's order
-__global__ __launch_bounds__(kBlock) void k_v3_apply(DRecv R, V3Dev V, Loop *L, int r0, int r1)
+// CalRadiosity (shade.cpp:244-259) of receiver r with its form factors V.ff
+__device__ __forceinline__ void v3_receive(const DRecv &R, const V3Dev &V, const SrcDesc &s, int r)
 {
-	const int r = r0 + blockIdx.x * kBlock + threadIdx.x;
-	if (L->done || L->v3_error || r >= r1 || V.act[r] != V3_APPLY) return;
-	const SrcDesc &s = L->src[0];
 	const float4 rho = R.rho[r];
 	const float kr = (float)(0.5 / 255.0);
 	const float lo[3] = {rho.x * kr * s.dB[0] / 3, rho.y * kr * s.dB[1] / 3, rho.z * kr * s.dB[2] / 3};
@@ -1694,21 +1720,38 @@ __global__ __launch_bounds__(kBlock) void k_v3_apply(DRecv R, V3Dev V, Loop *L, 
 	R.dB[r] = make_float4(dbv[0], dbv[1], dbv[2], 0.f);
 }
 
+// ... for the receivers of the pass that keep their form factors, one lane
+// each
+__global__ __launch_bounds__(kBlock) void k_v3_apply(DRecv R, V3Dev V, Loop *L, int par)
+{
+	if (L->done || L->v3_error) return;
+	const int r0 = L->v3_range[par][0], r1 = L->v3_range[par][1];
+	for (int r = r0 + blockIdx.x * kBlock + threadIdx.x; r < r1; r += gridDim.x * kBlock)
+		if (V.act[r] == V3_APPLY) v3_receive(R, V, L->src[0], r);
+}
+
 // the best candidate over the slots in use (listCanditate with one source,
-// rad.cpp:387-404: the highest slot on ties)
-__global__ __launch_bounds__(kBlock) void k_v3_select(DRecv R, Loop *L, float converge)
+// rad.cpp:387-404: the highest slot on ties), once the passes are over: the
+// next pass (parity par) has no slot; else v3_pending makes the next
+// iteration's k_v3_prep leave the source as it is, for that pass
+__global__ __launch_bounds__(kBlock) void k_v3_select(DRecv R, Loop *L, float converge, int par)
 {
 	__shared__ unsigned long long wmax[kWavesPerBlock];
 	__shared__ int wn[kWavesPerBlock];
-	const int r = blockIdx.x * kBlock + threadIdx.x;
+	if (L->done || L->v3_error) return;
+	const bool pending = L->v3_range[par][0] < L->v3_range[par][1];
+	if (blockIdx.x == 0 && threadIdx.x == 0) L->v3_pending = pending;
+	if (pending) return;
 	unsigned long long key = 0;
 	int n = 0;
-	if (r < L->trinum && __float_as_int(R.nrm[r].w) >= 0) {
+	for (int r = blockIdx.x * kBlock + threadIdx.x; r < L->trinum; r += gridDim.x * kBlock) {
+		if (__float_as_int(R.nrm[r].w) < 0) continue;
 		const float4 db = R.dB[r];
 		const float sum = db.x + db.y + db.z;
 		if (sum > converge) {
-			key = cand_key<false>(sum, r);
-			n = 1;
+			const unsigned long long k = cand_key<false>(sum, r);
+			key = k > key ? k : key;
+			n++;
 		}
 	}
 	key = wave_max(key);
@@ -1935,6 +1978,7 @@ struct DeviceEngine::Impl {
 	Loop *host_loop = nullptr;  // pinned mirror of the loop header
 	int nblocks = 0, n_orig = 0, trace_nblocks = 0;
 	int groups = 2;  // receiver groups per block (kOneGroupSlots)
+	int v3_par = 0;  // parity of the next V3 pass
 	bool fma = true, lowest = false;
 	int parallel_src = 1, batch = 32;
 	float converge = 1200.f;
@@ -2207,49 +2251,60 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 	}
 	const int start = d.host_loop->shots;
 	if (d.v3) {
-		// one iteration at a time: the source, then passes over the receivers
-		// until a pass makes no new slot (rad.cpp:414-444), then the next source
+		// An iteration is the source, then passes over the receivers until a
+		// pass makes no new slot (rad.cpp:414-444), then the next source.
+		// The device runs it from queued slots of five launches: the source
+		// (kept while a split is still being shaded), one pass, and the
+		// selection once no pass is left (v3_pending).  Each pass's slots
+		// are in Loop by parity; the grids stride over them.  Batches of
+		// kV3Batch slots between host checks; a call ends between iterations.
 		float ms_total = 0.f;
-		for (int it = 0; it < max_shots; it++) {
-			HIP_OK(hipEventRecord(d.ev0, st), "event");
-			if (d.fma) hipLaunchKernelGGL((k_prep<true, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
-			else hipLaunchKernelGGL((k_prep<false, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
-			HIP_OK(hipMemcpyAsync(d.host_loop, d.loop.p, 64, hipMemcpyDeviceToHost, st), "loop read");
-			HIP_OK(hipStreamSynchronize(st), "v3 source");
-			if (d.host_loop->done) break;
-			int prev = 0, n = d.host_loop->trinum;
-			while (prev < n) {
-				const unsigned g = (unsigned)(((size_t)(n - prev) * 16 + kBlock - 1) / kBlock);
-				const unsigned ga = (unsigned)((n - prev + kBlock - 1) / kBlock);
-				if (d.fma) {
-					hipLaunchKernelGGL(k_v3_gen<true>, dim3(g), dim3(kBlock), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
-					hipLaunchKernelGGL(k_v3_commit<true>, dim3(1), dim3(4 * kWave), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
-				} else {
-					hipLaunchKernelGGL(k_v3_gen<false>, dim3(g), dim3(kBlock), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
-					hipLaunchKernelGGL(k_v3_commit<false>, dim3(1), dim3(4 * kWave), 0, st, d.S, d.R, d.V, d.loop.p, prev, n, d.bcnt.p);
-				}
-				hipLaunchKernelGGL(k_v3_apply, dim3(ga), dim3(kBlock), 0, st, d.R, d.V, d.loop.p, prev, n);
-				HIP_OK(hipGetLastError(), "v3 launch");
-				HIP_OK(hipMemcpyAsync(d.host_loop, d.loop.p, 64, hipMemcpyDeviceToHost, st), "loop read");
-				HIP_OK(hipStreamSynchronize(st), "v3 pass");
-				if (d.host_loop->v3_error) {
-					err = "-variant v3: the split of slot " + std::to_string(d.host_loop->v3_error - 1) +
-					      " would rewrite an original the octree lists (pre-partition overwrote it); not supported";
-					return -1;
-				}
-				prev = n;
-				n = d.host_loop->trinum;
+		auto slot = [&]() {
+			const int p = d.v3_par;
+			const size_t nt = (size_t)std::max(d.host_loop->trinum, 1);  // as last read back
+			const dim3 g((unsigned)std::min<size_t>((nt * 16 + kBlock - 1) / kBlock, 4096));
+			const dim3 ga((unsigned)std::min<size_t>((nt + kBlock - 1) / kBlock, 1024)), blk(kBlock);
+			if (d.fma) {
+				hipLaunchKernelGGL(k_v3_prep<true>, dim3(1), dim3(64), 0, st, d.R, d.loop.p, p);
+				hipLaunchKernelGGL(k_v3_gen<true>, g, blk, 0, st, d.S, d.R, d.V, d.loop.p, p, d.bcnt.p);
+				hipLaunchKernelGGL(k_v3_commit<true>, dim3(1), dim3(4 * kWave), 0, st, d.S, d.R, d.V, d.loop.p, p, d.bcnt.p);
+			} else {
+				hipLaunchKernelGGL(k_v3_prep<false>, dim3(1), dim3(64), 0, st, d.R, d.loop.p, p);
+				hipLaunchKernelGGL(k_v3_gen<false>, g, blk, 0, st, d.S, d.R, d.V, d.loop.p, p, d.bcnt.p);
+				hipLaunchKernelGGL(k_v3_commit<false>, dim3(1), dim3(4 * kWave), 0, st, d.S, d.R, d.V, d.loop.p, p, d.bcnt.p);
 			}
-			const unsigned gs = (unsigned)((n + kBlock - 1) / kBlock);
-			hipLaunchKernelGGL(k_v3_select, dim3(gs > 0 ? gs : 1), dim3(kBlock), 0, st, d.R, d.loop.p, d.converge);
-			HIP_OK(hipGetLastError(), "v3 select");
+			hipLaunchKernelGGL(k_v3_apply, ga, blk, 0, st, d.R, d.V, d.loop.p, p);
+			hipLaunchKernelGGL(k_v3_select, ga, blk, 0, st, d.R, d.loop.p, d.converge, p ^ 1);
+			d.v3_par = p ^ 1;
+		};
+		auto read_back = [&]() -> int {
+			HIP_OK(hipGetLastError(), "v3 launch");
+			HIP_OK(hipMemcpyAsync(d.host_loop, d.loop.p, 64, hipMemcpyDeviceToHost, st), "loop read");
+			HIP_OK(hipStreamSynchronize(st), "v3 batch");
+			if (d.host_loop->v3_error) {
+				err = "-variant v3: the split of slot " + std::to_string(d.host_loop->v3_error - 1) +
+				      " would rewrite an original the octree lists (pre-partition overwrote it); not supported";
+				return -1;
+			}
+			return 0;
+		};
+		for (;;) {
+			const int left = max_shots - (d.host_loop->shots - start);
+			if (left <= 0 || d.host_loop->done) break;
+			const int b = std::min(kV3Batch, left);  // slots that pick a source: at most b shots
+			HIP_OK(hipEventRecord(d.ev0, st), "event");
+			for (int i = 0; i < b; i++) slot();
 			HIP_OK(hipEventRecord(d.ev1, st), "event");
-			HIP_OK(hipEventSynchronize(d.ev1), "v3 iteration");
+			if (read_back()) return -1;
 			float ms = 0;
 			(void)hipEventElapsedTime(&ms, d.ev0, d.ev1);
 			ms_total += ms;
-			kernel_launches_++;
 		}
+		while (!d.host_loop->done && d.host_loop->v3_pending) {  // finish the iteration under way
+			slot();
+			if (read_back()) return -1;
+		}
+		kernel_launches_ += d.host_loop->shots - start;
 		HIP_OK(hipMemcpyAsync(d.host_loop, d.loop.p, 64, hipMemcpyDeviceToHost, st), "loop read");
 		HIP_OK(hipStreamSynchronize(st), "v3 shoot");
 		shoot_ms_ += ms_total;
