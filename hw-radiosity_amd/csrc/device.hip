@@ -452,6 +452,35 @@ __device__ __forceinline__ float ff_prefix(const SrcPatch &s, float nx, float ny
 	return ff <= 0.0f ? 0.0f : ff;
 }
 
+// ff_prefix(...) != 0, for counting the full-source ray AdaptCalFF casts and
+// discards.  After ff_prefix's own sign test, estimates from hardware
+// reciprocals (relative error below 2^-18 in all) of |ds| / dl, |dd| / dl,
+// both cosines, their product and the whole prefix: when every one lies in
+// [2^-100, 2^100], each exact intermediate is a positive normal float (or a
+// double of that size), so the prefix is non-zero; anything else (zero,
+// underflow, overflow, NaN) takes ff_prefix itself.
+template <bool F>
+__device__ __forceinline__ bool ff_prefix_nonzero(const SrcPatch &s, float nx, float ny, float nz, float nlen,
+                                                  float px, float py, float pz)
+{
+	const float dx = s.c[0] - px, dy = s.c[1] - py, dz = s.c[2] - pz;
+	const float ds = dot3<F>(dx, dy, dz, s.n[0], s.n[1], s.n[2]);
+	const float dd = -dot3<F>(dx, dy, dz, nx, ny, nz);
+	if ((ds < 0) != (dd < 0)) return false;
+	const float r2 = dot3<F>(dx, dy, dz, dx, dy, dz);
+	const float ir = __builtin_amdgcn_rsqf(r2);
+	const float q1 = fabsf(ds) * ir, q2 = fabsf(dd) * ir;
+	const float c1 = q1 * __builtin_amdgcn_rcpf(s.nlen), c2 = q2 * __builtin_amdgcn_rcpf(nlen);
+	const float cc = c1 * c2;
+	const float e = cc * (s.area * __builtin_amdgcn_rcpf(r2 * (float)kPiD));
+	const float lo = fminf(fminf(fminf(q1, q2), fminf(c1, c2)), fminf(cc, e));
+	const float hi = fmaxf(fmaxf(fmaxf(q1, q2), fmaxf(c1, c2)), fmaxf(cc, e));
+	// fminf/fmaxf drop a NaN operand: test each estimate for NaN through e and lo/hi's own
+	const bool sure = lo >= 0x1p-100f && hi <= 0x1p100f && e == e && q1 == q1 && q2 == q2;
+	if (sure) return true;
+	return ff_prefix<F>(s, nx, ny, nz, nlen, px, py, pz) != 0.0f;
+}
+
 // Whole CalFF: prefix, then the ray (shade.cpp:124).
 template <bool F>
 __device__ float point_ff(const DScene &S, const SrcPatch &s, int logsrc, float nx, float ny, float nz, float nlen,
@@ -681,12 +710,17 @@ template <int W = kWavesPerBlock> __device__ void add_block_counters(const int (
 
 // Source k's rays, part 1: which receivers take part (fixed at the first
 // source: parent >= 0, not a source, not all nine accB >= 255,
-// rad.cpp:420-428) and CalFF up to the ray (shade.cpp:107-123) for each
+// rad.cpp:420-428; bit 0 of live[slot], bit 1 holding parent >= 0 alone,
+// which k_update's candidate test needs) and CalFF up to the ray (shade.cpp:107-123) for each
 // (receiver, vertex, half) into ffbuf[2 * (3 * slot + vertex) + half]; 0
-// where no ray is cast.
+// where no ray is cast.  For k_compact, per receiver vertex in position
+// order (rcode[3 p + vertex], p the receiver's position in R.order): bit h
+// set where half h casts a ray, and the vertex's key (rkey, same order)
+// above.
 template <bool F, int G>
 __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigned char *live, float *ffbuf,
-                                                long long *bcnt, int nblocks, int *raycnt)
+                                                const unsigned char *rkey, unsigned char *rcode, long long *bcnt,
+                                                int nblocks, int *raycnt)
 {
 	if (L->done || k >= L->nsrc) return;
 	const RecvMap<G> m(nblocks);
@@ -695,7 +729,10 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 #pragma unroll 1
 	for (int g = 0; g < G; g++) {
 		const bool valid = m.valid(g, R.nslots, nblocks);
-		const int r = valid ? R.order[m.slot(g)] : -1;
+		const int pos = m.slot(g);
+		const int r = valid ? R.order[pos] : -1;
+		// loaded before the stores below, which a byte pointer may alias
+		const int key = valid ? rkey[3 * pos + m.v] : 0;
 		bool lv;
 		if (k == 0) {
 			int parent = -1;
@@ -709,9 +746,9 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 			bool chosen = false;
 			for (int j = 0; j < L->nsrc; j++) chosen |= (L->src[j].phys == r);
 			lv = valid && parent >= 0 && !chosen && !full;
-			if (valid && m.v == 0) live[r] = lv;
+			if (valid && m.v == 0) live[r] = (unsigned char)(lv | (parent >= 0) << 1);
 		} else {
-			lv = valid && live[r];
+			lv = valid && (live[r] & 1);
 		}
 		if (!valid) continue;
 		float f0 = 0.f, f1 = 0.f;
@@ -722,7 +759,7 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 			const float px = R.vx[sv], py = R.vy[sv], pz = R.vz[sv];
 			if (s.split) {
 				// the full-source ray AdaptCalFF computes and then discards
-				if (ff_prefix<F>(s.whole, nr.x, nr.y, nr.z, nlen, px, py, pz) != 0.0f) ref_rays++;
+				if (ff_prefix_nonzero<F>(s.whole, nr.x, nr.y, nr.z, nlen, px, py, pz)) ref_rays++;
 				f0 = ff_prefix<F>(s.half[0], nr.x, nr.y, nr.z, nlen, px, py, pz);
 				f1 = ff_prefix<F>(s.half[1], nr.x, nr.y, nr.z, nlen, px, py, pz);
 			} else {
@@ -731,7 +768,9 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 		}
 		ffbuf[2 * sv] = f0;
 		ffbuf[2 * sv + 1] = f1;
-		nray += (f0 != 0.0f) + (f1 != 0.0f);
+		const int bits = (f0 != 0.0f) | (f1 != 0.0f) << 1;
+		rcode[3 * pos + m.v] = (unsigned char)(bits | key << 2);
+		nray += (bits & 1) + (bits >> 1);
 	}
 	// rays of this block's contiguous lane range [vb * 6 * kRecvPerBlock<G>, +6 * kRecvPerBlock<G>)
 	__shared__ int wray[kWavesPerBlock];
@@ -750,7 +789,8 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 // The rays of source k as a dense list: block b (one per k_gen range) takes
 // positions q of [b * 6 * kRecvPerBlock<G>, +6 * kRecvPerBlock<G>), lane i = 6 *
 // order[q / 6] + q % 6, and places those whose form-factor prefix is
-// non-zero at offset sum(raycnt[0..b)), grouped by their origin's key (the
+// non-zero (bit q % 2 of rcode[q / 2]; both it and order are read in q's
+// order) at offset sum(raycnt[0..b)), grouped by their origin's key (the
 // list length of the leaf holding the receiver vertex, in kRayBuckets
 // steps), smallest key first, in ascending q within a key; the last block
 // stores the total.  A ray's cost follows mostly the lists it tests, so
@@ -758,9 +798,8 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 // rays alike (less divergence: k_trace 272 -> 255 us on the hall); which
 // rays there are, and each one's result, do not depend on the order.
 template <int G>
-__global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float *ffbuf, const int *order, int nlanes,
-                                                    const int *raycnt, int nblocks, const unsigned char *rkey,
-                                                    int *rays)
+__global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const unsigned char *rcode, const int *order,
+                                                    int nlanes, const int *raycnt, int nblocks, int *rays)
 {
 	if (L->done || k >= L->nsrc) return;
 	constexpr int kRounds = (6 * kRecvPerBlock<G> + kBlock - 1) / kBlock;
@@ -781,8 +820,12 @@ __global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float 
 #pragma unroll
 	for (int r = 0; r < kRounds; r++) {
 		const int q = i0 + r * kBlock + threadIdx.x;
-		const int i = q < i1 ? 6 * order[q / 6] + q % 6 : 0;
-		const int key = (q < i1 && ffbuf[i] != 0.0f) ? rkey[i >> 1] : -1;  // -1: no ray
+		int i = 0, key = -1;  // -1: no ray
+		if (q < i1) {
+			const int cd = rcode[q >> 1];
+			i = 6 * order[q / 6] + q % 6;
+			key = (cd >> (q & 1)) & 1 ? cd >> 2 : -1;
+		}
 		unsigned long long mine = 0;
 #pragma unroll
 		for (int kb = 0; kb < kRayBuckets; kb++) {
@@ -829,11 +872,13 @@ __global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const float 
 
 // Key of each receiver vertex for k_compact's grouping: the list length of
 // the leaf that holds the vertex (the first leaf every ray from it tests),
-// in steps of 8, capped at kRayBuckets - 1; 0 outside the grid.
+// in steps of 8, capped at kRayBuckets - 1; 0 outside the grid.  Stored in
+// position order: rkey[3 p + v] for vertex v of slot R.order[p].
 __global__ __launch_bounds__(kBlock) void k_ray_keys(DScene S, DRecv R, int nv, unsigned char *rkey)
 {
-	const int sv = blockIdx.x * kBlock + threadIdx.x;
-	if (sv >= nv) return;
+	const int t = blockIdx.x * kBlock + threadIdx.x;
+	if (t >= nv) return;
+	const int sv = 3 * R.order[t / 3] + t % 3;
 	const float px = R.vx[sv], py = R.vy[sv], pz = R.vz[sv];
 	int key = 0;
 	if (!(px < S.g0x || px > S.g1x || py < S.g0y || py > S.g1y || pz < S.g0z || pz > S.g1z)) {
@@ -844,7 +889,7 @@ __global__ __launch_bounds__(kBlock) void k_ray_keys(DScene S, DRecv R, int nv, 
 			key = min(kRayBuckets - 1, cnt >> 3);
 		}
 	}
-	rkey[sv] = (unsigned char)key;
+	rkey[t] = (unsigned char)key;
 }
 
 // Source k's rays, part 2: the compacted list, which keeps the receivers'
@@ -921,11 +966,12 @@ __global__ __launch_bounds__(kBlock) void k_update(DRecv R, Loop *L, int k, cons
 	const bool last = k == L->nsrc - 1;
 	unsigned long long key = 0;
 	int recv = 0, ncand = 0;
-#pragma unroll 1
+#pragma unroll
 	for (int g = 0; g < G; g++) {
 		const int r = m.slot(g);
 		const bool valid = m.valid(g, R.nslots, nblocks);
-		const bool lv = valid && live[r];
+		const int lb = valid ? live[r] : 0;  // bit 0: takes part; bit 1: parent >= 0
+		const bool lv = lb & 1;
 		const int sv = 3 * r + m.v;
 		float ff = 0.f;
 		float4 rho = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -956,7 +1002,7 @@ __global__ __launch_bounds__(kBlock) void k_update(DRecv R, Loop *L, int k, cons
 				recv++;
 			}
 		}
-		if (last && valid && m.v == 0 && __float_as_int(R.nrm[r].w) >= 0) {
+		if (last && valid && m.v == 0 && (lb & 2)) {
 			const float sum = db.x + db.y + db.z;
 			if (sum > converge) {
 				ncand++;
@@ -1968,7 +2014,8 @@ struct DeviceEngine::Impl {
 	DevBuf<unsigned char> live;
 	DevBuf<float> ffbuf;
 	DevBuf<int> raycnt, rays;  // per-k_gen-block ray counts, compacted ray list
-	DevBuf<unsigned char> rkey;  // per receiver vertex: k_compact's grouping key
+	DevBuf<unsigned char> rkey;  // per receiver vertex in position order: k_compact's grouping key
+	DevBuf<unsigned char> rcode; // the same order: which of the vertex's rays exist, and its key (k_gen)
 	DevBuf<int> order;         // receiver processing order (DRecv::order)
 	DevBuf<Patch> pool;        // the uploaded slot records (packed by k_pack_items / k_pack_recv)
 	DevBuf<int> items;         // the octree's list entries (original slots)
@@ -2215,7 +2262,7 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	             (double)t.min_grid, 1.0 / (double)t.min_grid, t.gt, thr_t, thr_t * kGuardDown,
 	             largest_float_at_most(1e-6), cull_cl};
 	d.R = DRecv{d.vx.p, d.vy.p, d.vz.p, d.nrm.p, d.cA.p, d.rho.p, d.acc.p, d.dacc.p, d.dB.p, n, d.order.p};
-	if (d.rkey.alloc((size_t)n * 3 + 1, err)) return -1;
+	if (d.rkey.alloc((size_t)n * 3 + 1, err) || d.rcode.alloc((size_t)n * 3 + 1, err)) return -1;
 	if (n > 0)
 		hipLaunchKernelGGL(k_ray_keys, dim3((unsigned)(((size_t)n * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d.S,
 		                   d.R, n * 3, d.rkey.p);
@@ -2342,10 +2389,10 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 			// k_gen and k_compact for G receiver groups per block
 			auto gen = [&](auto gc) {
 				constexpr int G = decltype(gc)::value;
-				if (d.fma) hipLaunchKernelGGL((k_gen<true, G>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
-				else hipLaunchKernelGGL((k_gen<false, G>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.bcnt.p, d.nblocks, d.raycnt.p);
-				hipLaunchKernelGGL(k_compact<G>, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.ffbuf.p, d.order.p, 6 * d.R.nslots,
-				                   d.raycnt.p, d.nblocks, d.rkey.p, d.rays.p);
+				if (d.fma) hipLaunchKernelGGL((k_gen<true, G>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.rkey.p, d.rcode.p, d.bcnt.p, d.nblocks, d.raycnt.p);
+				else hipLaunchKernelGGL((k_gen<false, G>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.rkey.p, d.rcode.p, d.bcnt.p, d.nblocks, d.raycnt.p);
+				hipLaunchKernelGGL(k_compact<G>, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.rcode.p, d.order.p, 6 * d.R.nslots,
+				                   d.raycnt.p, d.nblocks, d.rays.p);
 			};
 			if (d.groups == 1) gen(std::integral_constant<int, 1>{});
 			else gen(std::integral_constant<int, 2>{});
