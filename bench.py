@@ -163,8 +163,18 @@ def reference_beside(threads):
             for who, exe, extra in runs:
                 if who == "oracle" and name.startswith("c1"):
                     continue
-                row[who] = _run_timed(exe, opts + extra + ["-write_cycle", "1000000", "-jobs", "1" if v3 else str(threads),
-                                                           tri, "-o", os.path.join(tmp, who)])
+                args = opts + extra + ["-write_cycle", "1000000", "-jobs", "1" if v3 else str(threads),
+                                       tri, "-o", os.path.join(tmp, who)]
+                if who != "engine":
+                    row[who] = _run_timed(exe, args)
+                    continue
+                # the engine's span is tens of ms, where a single fresh
+                # process can be off by a few ms: the median of three runs
+                tries = [_run_timed(exe, args) for _ in range(3)]
+                good = sorted((t for t in tries if t.get("method_s")), key=lambda t: t["method_s"])
+                row[who] = dict(good[len(good) // 2]) if good else tries[0]
+                row[who]["method_s_runs"] = [t.get("method_s") for t in tries]
+                row[who]["fin_md5_all_equal"] = len({t.get("fin_md5") for t in tries}) == 1
             ref, eng = row["reference"], row["engine"]
             if ref.get("method_s") and eng.get("method_s"):
                 row["speedup_method_span"] = ref["method_s"] / eng["method_s"]

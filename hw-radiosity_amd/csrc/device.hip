@@ -70,6 +70,12 @@ constexpr int kTraceBlock = 64;
 // occupancy, x 64)
 constexpr long long kTraceSlots = 256LL * 4 * 6 * 64;
 constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
+// k_trace with many rays: the last kTailPct percent of each XCD's share of
+// the list in blocks of 64 / kTailLanes rays at kTailLanes lanes per ray
+// (hall, per launch: 214 us without; 209, 205.7, 204.0, 204.3, 205.7 us at
+// 10, 15, 20, 25, 30 percent; eight lanes: 217 us at 20, 232 at 30)
+constexpr int kTailPct = 20;
+constexpr int kTailLanes = 4;
 // receiver groups per block of k_gen, k_compact and k_update: 2, or 1 below
 // kOneGroupSlots slots (two 2-group blocks per CU), where halving a block's
 // sequential work pays (the 2120-slot room: 48.6 -> 42.0 us per iteration;
@@ -907,6 +913,29 @@ __host__ __device__ __forceinline__ int lanes_per_ray(int nr)
 	return 8LL * nr <= kTraceSlots ? 8 : 4LL * nr <= kTraceSlots ? 4 : 2;
 }
 
+// Blocks of k_trace with two lanes per ray (nr rays, nr > kTraceSlots / 4):
+// XCD x's segment [nr x / 8, nr (x + 1) / 8) of the list (XCD-contiguous, as
+// the receivers' spatial order) is split at sb = end - len * kTailPct / 100.
+// Its head [start, sb) goes to blocks 8 b + x, b < head, 32 rays each at two
+// lanes per ray; its tail [sb, end) to blocks 8 (head + b) + x, b < tail,
+// 64 / kTailLanes rays each at kTailLanes lanes per ray.  Dispatched last,
+// the tail's waves are short, so the launch does not end on long waves
+// started just before the list ran out (timed per wave on the hall without
+// the split: every slot busy until 62 % of the launch, then 40 % of it
+// draining waves of 47 us on average, up to 130 us).  The host sizes the
+// grid for 6 n rays; blocks past the split of the actual count exit.
+struct TraceSplit {
+	int head, tail;  // blocks per XCD
+};
+__host__ __device__ __forceinline__ TraceSplit trace_split(long long nr)
+{
+	const long long sm = nr >> 3;                                            // shortest segment
+	const long long hmax = (sm + 1) - sm * kTailPct / 100;                  // longest head
+	const long long tmax = (sm + 1) * kTailPct / 100;                       // longest tail
+	const int rt = kTraceBlock / kTailLanes;
+	return TraceSplit{(int)((hmax + 31) / 32), (int)((tmax + rt - 1) / rt)};
+}
+
 // Six waves per SIMD (80 VGPRs, a 28-byte spill per leaf step) against five
 // (82 VGPRs, none): 223.6 -> 213.4 us per launch on the hall; on the 2120-slot
 // room, where latency rules, 48.2 -> 48.5 us per iteration.
@@ -916,15 +945,32 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
 {
 	if (L->done || k >= L->nsrc) return;
 	const int nr = L->nrays;
-	const int lpr = lanes_per_ray(nr);
-	const int rpb = kTraceBlock / lpr;  // rays per block
-	const int nb = (nr + rpb - 1) / rpb, per = (nb + 7) >> 3;
 	const int xcd = blockIdx.x & 7;
-	const int vb = xcd * per + (blockIdx.x >> 3);
-	const int j = vb * rpb + (int)threadIdx.x / lpr, sub = (int)threadIdx.x % lpr;
+	int b = blockIdx.x >> 3;
+	// this lane group's ray j (-1: none) and its lanes per ray
+	int lanes = lanes_per_ray(nr), j = -1;
+	if (lanes > 2) {
+		// few rays: 64 / lanes rays per block, XCD-contiguous
+		const int rpb = kTraceBlock / lanes;
+		const int nb = (nr + rpb - 1) / rpb, per = (nb + 7) >> 3;
+		const int jj = (xcd * per + b) * rpb + (int)threadIdx.x / lanes;
+		if (b < per && jj < nr) j = jj;
+	} else {
+		const TraceSplit ts = trace_split(nr);
+		const int s0 = (int)((long long)nr * xcd >> 3), s1 = (int)((long long)nr * (xcd + 1) >> 3);
+		const int sb = s1 - (int)((long long)(s1 - s0) * kTailPct / 100);
+		if (b < ts.head) {
+			const int jj = s0 + b * (kTraceBlock / 2) + (int)threadIdx.x / 2;
+			if (jj < sb) j = jj;
+		} else if ((b -= ts.head) < ts.tail) {
+			lanes = kTailLanes;
+			const int jj = sb + b * (kTraceBlock / kTailLanes) + (int)threadIdx.x / kTailLanes;
+			if (jj < s1) j = jj;
+		}
+	}
 	Cnt c = {0, 0, 0, 0, 0, 0, 0};
-	if ((int)(blockIdx.x >> 3) < per && j < nr) {
-		const int i = rays[j];
+	if (j >= 0) {
+		const int i = rays[j], sub = (int)threadIdx.x & (lanes - 1);
 		const SrcDesc &s = L->src[k];
 		const int sv = i >> 1, h = i & 1, r = sv / 3;
 		const float px = R.vx[sv], py = R.vy[sv], pz = R.vz[sv];
@@ -932,9 +978,9 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
 		const SrcPatch &sp = s.split ? s.half[h] : s.whole;
 		const float vx = sp.c[0] - px, vy = sp.c[1] - py, vz = sp.c[2] - pz;
 		Cnt w = {0, 0, 0, 0, 0, 0, 0};
-		const int hit = lpr == 8   ? walk<F, 8>(S, px, py, pz, vx, vy, vz, ban, w, sub)
-		                : lpr == 4 ? walk<F, 4>(S, px, py, pz, vx, vy, vz, ban, w, sub)
-		                           : walk<F, 2>(S, px, py, pz, vx, vy, vz, ban, w, sub);
+		const int hit = lanes == 8   ? walk<F, 8>(S, px, py, pz, vx, vy, vz, ban, w, sub)
+		                : lanes == 4 ? walk<F, 4>(S, px, py, pz, vx, vy, vz, ban, w, sub)
+		                             : walk<F, 2>(S, px, py, pz, vx, vy, vz, ban, w, sub);
 		// every lane of a group counts its own tests; the rest once
 		c.tests = w.tests;
 		if (sub == 0) {
@@ -2235,7 +2281,12 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 	d.nblocks = (n + rpb - 1) / rpb;
 	// k_trace: two lanes for each of up to 6 rays per slot, or four lanes on
 	// up to kTraceSlots lanes
-	d.trace_nblocks = (int)std::max(((size_t)n * 12 + kTraceBlock - 1) / kTraceBlock, (size_t)(kTraceSlots / kTraceBlock));
+	{
+		// k_trace's grid: every split of up to 6 n rays (two per receiver
+		// vertex), or the machine's slots for the few-ray mapping
+		const TraceSplit ts = trace_split((long long)n * 6);
+		d.trace_nblocks = (int)std::max(8LL * (ts.head + ts.tail), (long long)(kTraceSlots / kTraceBlock));
+	}
 	const int grid = ((d.nblocks + 7) / 8) * 8;  // a whole number of blocks per XCD
 	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
 	const int v3grid = d.v3 ? (int)(((size_t)cap * 4 + kBlock - 1) / kBlock) : 0;
