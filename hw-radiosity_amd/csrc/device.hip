@@ -70,12 +70,15 @@ constexpr int kTraceBlock = 64;
 // occupancy, x 64)
 constexpr long long kTraceSlots = 256LL * 4 * 6 * 64;
 constexpr int kRecvPerWave = 21;  // 3 lanes per receiver, lane 63 idle
-// k_trace with many rays: the last kTailPct percent of each XCD's share of
-// the list in blocks of 64 / kTailLanes rays at kTailLanes lanes per ray
-// (hall, per launch: 214 us without; 209, 205.7, 204.0, 204.3, 205.7 us at
-// 10, 15, 20, 25, 30 percent; eight lanes: 217 us at 20, 232 at 30)
+// k_trace with many rays: kTailPct percent of each XCD's share of the list
+// in blocks of 64 / kTailLanes rays at kTailLanes lanes per ray, then the
+// last kEndPct percent in blocks of 8 rays at eight lanes (hall, per launch:
+// 214 us without; 209, 205.7, 204.0, 204.3, 205.7 us at 10, 15, 20, 25, 30
+// percent at four lanes; eight lanes alone 217 us at 20, 232 at 30; 20 at
+// four then 3 at eight 200.8 us, then 5 at eight 201.4)
 constexpr int kTailPct = 20;
 constexpr int kTailLanes = 4;
+constexpr int kEndPct = 3;
 // receiver groups per block of k_gen, k_compact and k_update: 2, or 1 below
 // kOneGroupSlots slots (two 2-group blocks per CU), where halving a block's
 // sequential work pays (the 2120-slot room: 48.6 -> 42.0 us per iteration;
@@ -915,25 +918,28 @@ __host__ __device__ __forceinline__ int lanes_per_ray(int nr)
 
 // Blocks of k_trace with two lanes per ray (nr rays, nr > kTraceSlots / 4):
 // XCD x's segment [nr x / 8, nr (x + 1) / 8) of the list (XCD-contiguous, as
-// the receivers' spatial order) is split at sb = end - len * kTailPct / 100.
-// Its head [start, sb) goes to blocks 8 b + x, b < head, 32 rays each at two
-// lanes per ray; its tail [sb, end) to blocks 8 (head + b) + x, b < tail,
-// 64 / kTailLanes rays each at kTailLanes lanes per ray.  Dispatched last,
-// the tail's waves are short, so the launch does not end on long waves
+// the receivers' spatial order) is split at sc = end - len * kEndPct / 100
+// and sb = sc - len * kTailPct / 100.  Its head [start, sb) goes to blocks
+// 8 b + x, b < head, 32 rays each at two lanes per ray; its tail [sb, sc)
+// to blocks 8 (head + b) + x, b < tail, 64 / kTailLanes rays each at
+// kTailLanes lanes per ray; its end [sc, end) to the next `end` blocks of
+// the XCD, 8 rays each at eight lanes.  Dispatched last, the tail's waves
+// are short, so the launch does not end on long waves
 // started just before the list ran out (timed per wave on the hall without
 // the split: every slot busy until 62 % of the launch, then 40 % of it
 // draining waves of 47 us on average, up to 130 us).  The host sizes the
 // grid for 6 n rays; blocks past the split of the actual count exit.
 struct TraceSplit {
-	int head, tail;  // blocks per XCD
+	int head, tail, end;  // blocks per XCD
 };
 __host__ __device__ __forceinline__ TraceSplit trace_split(long long nr)
 {
 	const long long sm = nr >> 3;                                            // shortest segment
-	const long long hmax = (sm + 1) - sm * kTailPct / 100;                  // longest head
+	const long long hmax = (sm + 1) - sm * kTailPct / 100 - sm * kEndPct / 100;  // longest head
 	const long long tmax = (sm + 1) * kTailPct / 100;                       // longest tail
+	const long long emax = (sm + 1) * kEndPct / 100;
 	const int rt = kTraceBlock / kTailLanes;
-	return TraceSplit{(int)((hmax + 31) / 32), (int)((tmax + rt - 1) / rt)};
+	return TraceSplit{(int)((hmax + 31) / 32), (int)((tmax + rt - 1) / rt), (int)((emax + 7) / 8)};
 }
 
 // Six waves per SIMD (80 VGPRs, a 28-byte spill per leaf step) against five
@@ -958,13 +964,18 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
 	} else {
 		const TraceSplit ts = trace_split(nr);
 		const int s0 = (int)((long long)nr * xcd >> 3), s1 = (int)((long long)nr * (xcd + 1) >> 3);
-		const int sb = s1 - (int)((long long)(s1 - s0) * kTailPct / 100);
+		const int sc = s1 - (int)((long long)(s1 - s0) * kEndPct / 100);
+		const int sb = sc - (int)((long long)(s1 - s0) * kTailPct / 100);
 		if (b < ts.head) {
 			const int jj = s0 + b * (kTraceBlock / 2) + (int)threadIdx.x / 2;
 			if (jj < sb) j = jj;
 		} else if ((b -= ts.head) < ts.tail) {
 			lanes = kTailLanes;
 			const int jj = sb + b * (kTraceBlock / kTailLanes) + (int)threadIdx.x / kTailLanes;
+			if (jj < sc) j = jj;
+		} else if ((b -= ts.tail) < ts.end) {
+			lanes = 8;
+			const int jj = sc + b * (kTraceBlock / 8) + (int)threadIdx.x / 8;
 			if (jj < s1) j = jj;
 		}
 	}
@@ -2285,7 +2296,7 @@ int DeviceEngine::upload(const HostScene &s, const FlatOctree &t, std::string &e
 		// k_trace's grid: every split of up to 6 n rays (two per receiver
 		// vertex), or the machine's slots for the few-ray mapping
 		const TraceSplit ts = trace_split((long long)n * 6);
-		d.trace_nblocks = (int)std::max(8LL * (ts.head + ts.tail), (long long)(kTraceSlots / kTraceBlock));
+		d.trace_nblocks = (int)std::max(8LL * (ts.head + ts.tail + ts.end), (long long)(kTraceSlots / kTraceBlock));
 	}
 	const int grid = ((d.nblocks + 7) / 8) * 8;  // a whole number of blocks per XCD
 	const int tgrid = ((d.trace_nblocks + 7) / 8) * 8;
