@@ -726,14 +726,40 @@ template <int W = kWavesPerBlock> __device__ void add_block_counters(const int (
 // order (rcode[3 p + vertex], p the receiver's position in R.order): bit h
 // set where half h casts a ray, and the vertex's key (rkey, same order)
 // above.
-template <bool F, int G>
+// PREP 1 or 2 (one source per iteration): k_prep's selection here instead,
+// from the key the last k_update left (2: the lowest slot on ties).  Every
+// block builds the source descriptor in LDS; block 0 publishes it and counts
+// the shot; k_compact, after every block has read the source's unshot
+// energy, zeroes it and the key.
+template <bool F, int G, int PREP>
 __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigned char *live, float *ffbuf,
                                                 const unsigned char *rkey, unsigned char *rcode, long long *bcnt,
                                                 int nblocks, int *raycnt)
 {
-	if (L->done || k >= L->nsrc) return;
+	__shared__ SrcDesc sd;
+	const SrcDesc *sp;
+	if (PREP) {
+		if (L->done) return;
+		const unsigned long long key = L->key;
+		if (key == 0) {
+			if (blockIdx.x == 0 && threadIdx.x == 0) L->done = 1;
+			return;
+		}
+		if (threadIdx.x == 0) make_source<F>(R, key_slot<PREP == 2>(key), sd);
+		__syncthreads();
+		if (blockIdx.x == 0 && threadIdx.x == 0) {
+			L->src[0] = sd;
+			L->nsrc = 1;
+			L->shots++;
+			L->ncand = 0;
+		}
+		sp = &sd;
+	} else {
+		if (L->done || k >= L->nsrc) return;
+		sp = &L->src[k];
+	}
+	const SrcDesc &s = *sp;
 	const RecvMap<G> m(nblocks);
-	const SrcDesc &s = L->src[k];
 	int ref_rays = 0, nray = 0;
 #pragma unroll 1
 	for (int g = 0; g < G; g++) {
@@ -753,7 +779,9 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 			const int vfull = !(acc.x < 255.0f || acc.y < 255.0f || acc.z < 255.0f);
 			const bool full = __shfl(vfull, m.base) && __shfl(vfull, m.base + 1) && __shfl(vfull, m.base + 2);
 			bool chosen = false;
-			for (int j = 0; j < L->nsrc; j++) chosen |= (L->src[j].phys == r);
+			if (PREP) chosen = sd.phys == r;
+			else
+				for (int j = 0; j < L->nsrc; j++) chosen |= (L->src[j].phys == r);
 			lv = valid && parent >= 0 && !chosen && !full;
 			if (valid && m.v == 0) live[r] = (unsigned char)(lv | (parent >= 0) << 1);
 		} else {
@@ -808,7 +836,8 @@ __global__ __launch_bounds__(kBlock) void k_gen(DRecv R, Loop *L, int k, unsigne
 // rays there are, and each one's result, do not depend on the order.
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const unsigned char *rcode, const int *order,
-                                                    int nlanes, const int *raycnt, int nblocks, int *rays)
+                                                    int nlanes, const int *raycnt, int nblocks, int *rays,
+                                                    float4 *dB, int prep)
 {
 	if (L->done || k >= L->nsrc) return;
 	constexpr int kRounds = (6 * kRecvPerBlock<G> + kBlock - 1) / kBlock;
@@ -876,7 +905,13 @@ __global__ __launch_bounds__(kBlock) void k_compact(Loop *L, int k, const unsign
 #pragma unroll
 	for (int r = 0; r < kRounds; r++)
 		if (rk[r] >= 0) rays[slot[r][w][rk[r]] + __popcll(rm[r] & below)] = ri[r];
-	if (b == nblocks - 1 && threadIdx.x == 0) L->nrays = total;
+	if (b == nblocks - 1 && threadIdx.x == 0) {
+		L->nrays = total;
+		if (prep) {  // what k_prep does after the selection (k_gen with PREP)
+			L->key = 0;
+			dB[L->src[0].phys] = make_float4(0.f, 0.f, 0.f, 0.f);
+		}
+	}
 }
 
 // Key of each receiver vertex for k_compact's grouping: the list length of
@@ -2433,11 +2468,14 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 	const int cb = (d.R.nslots + kBlock - 1) / kBlock;
 	// one shooting iteration's launches; with events, the ray-tracing kernel
 	// of the first source is bracketed by event pair ev
+	// one source per iteration: k_gen selects it (k_prep only without receivers)
+	const int prep = multi || grid <= 0 ? 0 : d.lowest ? 2 : 1;
 	auto iteration = [&](int ev) {
 		if (multi) {
 			hipLaunchKernelGGL(k_collect, dim3(cb > 0 ? cb : 1), dim3(kBlock), 0, st, d.R, d.loop.p, d.cand.p, d.converge);
 			if (d.fma) hipLaunchKernelGGL(k_topk<true>, dim3(1), dim3(1024), 0, st, d.R, d.loop.p, d.cand.p, d.parallel_src);
 			else hipLaunchKernelGGL(k_topk<false>, dim3(1), dim3(1024), 0, st, d.R, d.loop.p, d.cand.p, d.parallel_src);
+		} else if (prep) {
 		} else if (d.fma) {
 			if (d.lowest) hipLaunchKernelGGL((k_prep<true, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
 			else hipLaunchKernelGGL((k_prep<true, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
@@ -2451,10 +2489,12 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 			// k_gen and k_compact for G receiver groups per block
 			auto gen = [&](auto gc) {
 				constexpr int G = decltype(gc)::value;
-				if (d.fma) hipLaunchKernelGGL((k_gen<true, G>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.rkey.p, d.rcode.p, d.bcnt.p, d.nblocks, d.raycnt.p);
-				else hipLaunchKernelGGL((k_gen<false, G>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.rkey.p, d.rcode.p, d.bcnt.p, d.nblocks, d.raycnt.p);
+#define GEN(F, P) hipLaunchKernelGGL((k_gen<F, G, P>), g, blk, 0, st, d.R, d.loop.p, k, d.live.p, d.ffbuf.p, d.rkey.p, d.rcode.p, d.bcnt.p, d.nblocks, d.raycnt.p)
+				if (d.fma) { if (prep == 2) GEN(true, 2); else if (prep) GEN(true, 1); else GEN(true, 0); }
+				else { if (prep == 2) GEN(false, 2); else if (prep) GEN(false, 1); else GEN(false, 0); }
+#undef GEN
 				hipLaunchKernelGGL(k_compact<G>, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.rcode.p, d.order.p, 6 * d.R.nslots,
-				                   d.raycnt.p, d.nblocks, d.rays.p);
+				                   d.raycnt.p, d.nblocks, d.rays.p, d.dB.p, prep);
 			};
 			if (d.groups == 1) gen(std::integral_constant<int, 1>{});
 			else gen(std::integral_constant<int, 2>{});
