@@ -14,22 +14,27 @@
 //       floats, dB[r] = deltaB
 //
 // One shooting iteration (rad.cpp:406-451) is four launches on one stream:
-//   k_prep   (one lane) turns the best candidate key into a source descriptor
-//            -- the whole source and the halves AdaptCalFF shoots from
-//            (shade.cpp:141-168, 76-99) -- and clears the source's deltaB;
-//   k_gen    one lane per (receiver, vertex), 168 receivers per block:
-//            CalFF up to the ray (shade.cpp:107-123) for each half, into a
-//            per-(receiver, vertex, half) form-factor buffer;
-//   k_trace  one lane per (receiver, vertex, half) in slot order: the rays
-//            to cast (RayHitted, raycast.cpp:335-360); a ray that misses the
-//            source zeroes its form factor;
-//   k_update CalRadiosity (shade.cpp:244-259), the three vertex lanes of a
-//            receiver combined by shuffles in the reference's order, and the
-//            next iteration's candidate key (listCanditate, rad.cpp:387-404)
-//            by wave, block and one 64-bit atomicMax per block.
-// With k sources per iteration (-parallel_src) gen/trace/update repeat per
-// source in the reference's order.  The host polls a done flag every -batch
-// iterations.
+//   k_gen     every block turns the best candidate key into the source
+//             descriptor -- the whole source and the halves AdaptCalFF
+//             shoots from (shade.cpp:141-168, 76-99) -- and one lane per
+//             (receiver, vertex), 84 or 168 receivers per block, computes
+//             CalFF up to the ray (shade.cpp:107-123) for each half into a
+//             per-(receiver, vertex, half) form-factor buffer, with the
+//             block's ray count;
+//   k_compact the rays as a dense list in the receivers' spatial order,
+//             grouped by origin key within a block; it also clears the
+//             source's deltaB and the key;
+//   k_trace   two lanes per ray (four and eight over the end of each XCD's
+//             share, or for few rays): RayHitted (raycast.cpp:335-360); a
+//             ray that misses the source zeroes its form factor;
+//   k_update  CalRadiosity (shade.cpp:244-259), the three vertex lanes of a
+//             receiver combined by shuffles in the reference's order, and
+//             the next iteration's candidate key (listCanditate,
+//             rad.cpp:387-404) by wave, block and one 64-bit atomicMax per
+//             block.
+// With k sources per iteration (-parallel_src) k_collect/k_topk select them
+// and gen/compact/trace/update repeat per source in the reference's order.
+// The host polls a done flag every -batch iterations.
 //
 // Every float operation is IEEE single or double in source order
 // (-ffp-contract=off, correctly rounded division and square root); with
