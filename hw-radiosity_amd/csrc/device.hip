@@ -23,7 +23,8 @@
 //             block's ray count;
 //   k_compact the rays as a dense list in the receivers' spatial order,
 //             grouped by origin key within a block; it also clears the
-//             source's deltaB and the key;
+//             source's deltaB and the key (at most 8192 slots: no list,
+//             k_trace_direct traces each ray position in place);
 //   k_trace   two lanes per ray (four and eight over the end of each XCD's
 //             share, or for few rays): RayHitted (raycast.cpp:335-360); a
 //             ray that misses the source zeroes its form factor;
@@ -1033,6 +1034,48 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 
 		                : lanes == 4 ? walk<F, 4>(S, px, py, pz, vx, vy, vz, ban, w, sub)
 		                             : walk<F, 2>(S, px, py, pz, vx, vy, vz, ban, w, sub);
 		// every lane of a group counts its own tests; the rest once
+		c.tests = w.tests;
+		if (sub == 0) {
+			c.rays = c.ref_rays = 1;
+			c.cells = w.cells;
+			c.levels = w.levels;
+			c.precision = w.precision;
+			c.stalled = w.stalled;
+			if (hit != s.log) ffbuf[i] = 0.0f;
+		}
+	}
+	const int vals[8] = {c.rays, c.ref_rays, c.cells, c.tests, c.levels, c.precision, c.stalled, 0};
+	add_block_counters<kTraceBlock / kWave>(vals, bcnt);
+}
+
+// Source k's rays with no dense list, where every one of the 6 n ray
+// positions fits the machine at eight lanes per ray (48 n <= kTraceSlots):
+// lane group q (eight lanes) takes position q of k_compact's order, lane
+// i = 6 order[q / 6] + q % 6, and traces it if rcode marks a ray there.
+// There are lane groups to spare, so the list and its launch buy nothing;
+// what k_compact does besides (the key and the source's deltaB with PREP)
+// block 0 does here.
+template <bool F>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_trace_direct(
+	DScene S, DRecv R, Loop *L, int k, float *ffbuf, long long *bcnt, const unsigned char *rcode, int nlanes,
+	float4 *dB, int prep)
+{
+	if (L->done || k >= L->nsrc) return;
+	const SrcDesc &s = L->src[k];
+	if (prep && blockIdx.x == 0 && threadIdx.x == 0) {
+		L->key = 0;
+		dB[s.phys] = make_float4(0.f, 0.f, 0.f, 0.f);
+	}
+	const int q = blockIdx.x * (kTraceBlock / 8) + (int)threadIdx.x / 8, sub = (int)threadIdx.x & 7;
+	Cnt c = {0, 0, 0, 0, 0, 0, 0};
+	if (q < nlanes && ((rcode[q >> 1] >> (q & 1)) & 1)) {
+		const int i = 6 * R.order[q / 6] + q % 6;
+		const int sv = i >> 1, h = i & 1, r = sv / 3;
+		const float px = R.vx[sv], py = R.vy[sv], pz = R.vz[sv];
+		const int ban = __float_as_int(R.nrm[r].w);
+		const SrcPatch &sp = s.split ? s.half[h] : s.whole;
+		Cnt w = {0, 0, 0, 0, 0, 0, 0};
+		const int hit = walk<F, 8>(S, px, py, pz, sp.c[0] - px, sp.c[1] - py, sp.c[2] - pz, ban, w, sub);
 		c.tests = w.tests;
 		if (sub == 0) {
 			c.rays = c.ref_rays = 1;
@@ -2475,6 +2518,9 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 	// of the first source is bracketed by event pair ev
 	// one source per iteration: k_gen selects it (k_prep only without receivers)
 	const int prep = multi || grid <= 0 ? 0 : d.lowest ? 2 : 1;
+	// every ray position fits the machine at eight lanes per ray: no dense list
+	const bool direct = 48LL * d.R.nslots <= kTraceSlots;
+	const int dgrid = (int)((6LL * d.R.nslots + kTraceBlock / 8 - 1) / (kTraceBlock / 8));
 	auto iteration = [&](int ev) {
 		if (multi) {
 			hipLaunchKernelGGL(k_collect, dim3(cb > 0 ? cb : 1), dim3(kBlock), 0, st, d.R, d.loop.p, d.cand.p, d.converge);
@@ -2498,14 +2544,22 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 				if (d.fma) { if (prep == 2) GEN(true, 2); else if (prep) GEN(true, 1); else GEN(true, 0); }
 				else { if (prep == 2) GEN(false, 2); else if (prep) GEN(false, 1); else GEN(false, 0); }
 #undef GEN
-				hipLaunchKernelGGL(k_compact<G>, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.rcode.p, d.order.p, 6 * d.R.nslots,
-				                   d.raycnt.p, d.nblocks, d.rays.p, d.dB.p, prep);
+				if (!direct)
+					hipLaunchKernelGGL(k_compact<G>, dim3(d.nblocks), blk, 0, st, d.loop.p, k, d.rcode.p, d.order.p,
+					                   6 * d.R.nslots, d.raycnt.p, d.nblocks, d.rays.p, d.dB.p, prep);
 			};
 			if (d.groups == 1) gen(std::integral_constant<int, 1>{});
 			else gen(std::integral_constant<int, 2>{});
 			if (ev >= 0 && k == 0) (void)hipEventRecord(d.kev[2 * ev], st);
-			if (d.fma) hipLaunchKernelGGL(k_trace<true>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
-			else hipLaunchKernelGGL(k_trace<false>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
+			if (direct) {
+				const dim3 dg((unsigned)dgrid);
+				if (d.fma) hipLaunchKernelGGL(k_trace_direct<true>, dg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rcode.p, 6 * d.R.nslots, d.dB.p, prep);
+				else hipLaunchKernelGGL(k_trace_direct<false>, dg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rcode.p, 6 * d.R.nslots, d.dB.p, prep);
+			} else if (d.fma) {
+				hipLaunchKernelGGL(k_trace<true>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
+			} else {
+				hipLaunchKernelGGL(k_trace<false>, tg, dim3(kTraceBlock), 0, st, d.S, d.R, d.loop.p, k, d.ffbuf.p, d.bcnt.p, d.rays.p);
+			}
 			if (ev >= 0 && k == 0) (void)hipEventRecord(d.kev[2 * ev + 1], st);
 			auto update = [&](auto gc) {
 				constexpr int G = decltype(gc)::value;
