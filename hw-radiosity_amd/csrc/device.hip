@@ -2526,13 +2526,14 @@ int DeviceEngine::shoot(int max_shots, int *shots, int *converged, std::string &
 			hipLaunchKernelGGL(k_collect, dim3(cb > 0 ? cb : 1), dim3(kBlock), 0, st, d.R, d.loop.p, d.cand.p, d.converge);
 			if (d.fma) hipLaunchKernelGGL(k_topk<true>, dim3(1), dim3(1024), 0, st, d.R, d.loop.p, d.cand.p, d.parallel_src);
 			else hipLaunchKernelGGL(k_topk<false>, dim3(1), dim3(1024), 0, st, d.R, d.loop.p, d.cand.p, d.parallel_src);
-		} else if (prep) {
-		} else if (d.fma) {
-			if (d.lowest) hipLaunchKernelGGL((k_prep<true, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
-			else hipLaunchKernelGGL((k_prep<true, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
-		} else {
-			if (d.lowest) hipLaunchKernelGGL((k_prep<false, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
-			else hipLaunchKernelGGL((k_prep<false, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+		} else if (!prep) {  // no receivers: k_prep still ends the loop when no candidate is left
+			if (d.fma) {
+				if (d.lowest) hipLaunchKernelGGL((k_prep<true, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+				else hipLaunchKernelGGL((k_prep<true, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+			} else {
+				if (d.lowest) hipLaunchKernelGGL((k_prep<false, true>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+				else hipLaunchKernelGGL((k_prep<false, false>), dim3(1), dim3(64), 0, st, d.R, d.loop.p);
+			}
 		}
 		const int nk = multi ? d.parallel_src : 1;
 		for (int k = 0; grid > 0 && k < nk; k++) {
