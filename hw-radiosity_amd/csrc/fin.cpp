@@ -9,6 +9,7 @@
 #include <memory>
 #include <string>
 #include <vector>
+#include <omp.h>
 
 namespace rgpu {
 
@@ -170,15 +171,17 @@ int FinWriter::write(const std::string &path, const Patch *pool, int trinum, boo
 		return -1;
 	}
 	// records are formatted in chunks in parallel; each chunk starts from the
-	// `last` state that the records before it leave
-	const int kChunk = 4096;
-	const int nch = (trinum + kChunk - 1) / kChunk;
+	// `last` state that the records before it leave (so any chunking gives the
+	// same bytes): about four chunks per thread, of 256 to 4096 records
+	const int nt = std::max(1, omp_get_max_threads());
+	const int chunk = std::min(4096, std::max(256, (trinum + 4 * nt - 1) / (4 * nt)));
+	const int nch = (trinum + chunk - 1) / chunk;
 	std::vector<std::array<char, 48>> start(nch);
 	for (int c = 0; c < nch; c++) {
 		std::memcpy(start[c].data(), last_, sizeof last_);
 		if (old_format) continue;
 		// the latest written record of the chunk with a value that converts
-		for (int i = std::min(trinum, (c + 1) * kChunk) - 1; i >= c * kChunk; i--)
+		for (int i = std::min(trinum, (c + 1) * chunk) - 1; i >= c * chunk; i--)
 			if (pool[i].parent >= 0 && last_after(pool[i], last_)) break;
 	}
 	bool ok = true;
@@ -186,14 +189,14 @@ int FinWriter::write(const std::string &path, const Patch *pool, int trinum, boo
 	// each thread formats into one buffer of its own
 #pragma omp parallel
 	{
-		std::unique_ptr<char[]> buf(new char[(size_t)kChunk * kRecordMax]);  // not zeroed
+		std::unique_ptr<char[]> buf(new char[(size_t)chunk * kRecordMax]);  // not zeroed
 #pragma omp for ordered schedule(dynamic, 1)
 		for (int c = 0; c < nch; c++) {
 			char last[48];
 			std::memcpy(last, start[c].data(), sizeof last);
 			int ll = (int)std::strlen(last);
 			char *o = buf.get();
-			for (int i = c * kChunk; i < std::min(trinum, (c + 1) * kChunk); i++)
+			for (int i = c * chunk; i < std::min(trinum, (c + 1) * chunk); i++)
 				if (pool[i].parent >= 0) o = format_record(pool[i], old_format, last, ll, o);  // logical slots are not written
 			const size_t len = (size_t)(o - buf.get());
 #pragma omp ordered
